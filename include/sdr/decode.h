@@ -1,0 +1,218 @@
+/*
+ * sdr/decode.h — C ABI of the MI355X DuckDB segment decoder.
+ *
+ * Plain C, no STL, no torch or HIP types: a stream is passed as an opaque
+ * `sdr_stream_t` (a hipStream_t).  Every pointer named `d_*` is device memory
+ * on the current HIP device; the caller owns every output buffer.
+ *
+ * Each entry point replaces one function of the reference's decode-operator
+ * interface (ducndh/sirius); the file:line it replaces is given beside it.
+ * Conventions shared by every entry:
+ *   - the return value is an sdr_status; SDR_OK is 0, errors are negative and
+ *     `sdr_last_error()` returns a thread-local message for the last failure;
+ *   - viability violations (unsupported codec or width, rows out of bounds,
+ *     misaligned validity, string offsets past INT32_MAX) are returned as
+ *     errors where the reference throws std::runtime_error;
+ *   - malformed segment bytes never fault: the affected rows decode to zero
+ *     (strings: to empty), as in the reference kernels;
+ *   - all device work is enqueued on `stream`; entries are reentrant per
+ *     stream and take no global lock.
+ */
+#ifndef SDR_DECODE_H
+#define SDR_DECODE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SDR_ABI_VERSION 1
+
+typedef void* sdr_stream_t; /* hipStream_t */
+
+typedef enum sdr_status {
+  SDR_OK                  = 0,
+  SDR_E_INVALID_ARG       = -1, /* null pointer, zero type size, bad enum      */
+  SDR_E_UNSUPPORTED_CODEC = -2, /* codec not decodable on this path            */
+  SDR_E_BAD_WIDTH         = -3, /* type_size not supported by the codec        */
+  SDR_E_BOUNDS            = -4, /* row range outside [0,total_rows), short buf */
+  SDR_E_OVERFLOW          = -5, /* chars past capacity or INT32_MAX            */
+  SDR_E_ALIGNMENT         = -6, /* validity row_offset not a multiple of 8     */
+  SDR_E_HIP               = -7, /* HIP runtime error (message has the name)    */
+  SDR_E_CAPACITY          = -8, /* queue / pool / scratch capacity exceeded    */
+  SDR_E_NOT_FOUND         = -9, /* unknown handle                              */
+  SDR_E_STATE             = -10 /* call not valid in the object's state        */
+} sdr_status;
+
+/* Codec ids.  The numbers follow DuckDB's CompressionType order; callers that
+ * read DuckDB's pragma_storage_info get the id from the codec's name through
+ * sdr_codec_from_name ("Uncompressed", "Constant", "RLE", "Dictionary",
+ * "BitPacking", "FSST", "ALP", "ALPRD", "DICT_FSST"). */
+typedef enum sdr_codec {
+  SDR_CODEC_UNCOMPRESSED = 1,
+  SDR_CODEC_CONSTANT     = 2,
+  SDR_CODEC_RLE          = 3,
+  SDR_CODEC_DICTIONARY   = 4,
+  SDR_CODEC_BITPACKING   = 6,
+  SDR_CODEC_FSST         = 7,
+  SDR_CODEC_ALP          = 10,
+  SDR_CODEC_ALPRD        = 11,
+  SDR_CODEC_DICT_FSST    = 15
+} sdr_codec;
+
+/* One DuckDB column segment already staged on the device.
+ * Replaces gpu_segment_desc (src/include/cuda/scan/gpu_native_decode.cuh:50-55)
+ * and gpu_string_segment_desc (src/include/cuda/scan/gpu_decode_strings.cuh:41-49).
+ * `bytes_size` is an upper bound on the segment's bytes (it may include block
+ * slack); decoders bound every read by it.  `max_string_length` is the
+ * segment-statistics bound on decoded chars per row (strings only; 0 means
+ * unknown). */
+typedef struct sdr_segment {
+  const uint8_t* d_bytes;
+  uint32_t bytes_size;
+  uint32_t row_offset;
+  uint32_t row_count;
+  uint32_t max_string_length;
+} sdr_segment;
+
+/* A run of segments sharing one codec.
+ * Replaces gpu_codec_run (gpu_native_decode.cuh:60-63) and
+ * gpu_string_codec_run (gpu_decode_strings.cuh:52-55). */
+typedef struct sdr_run {
+  uint32_t codec; /* sdr_codec */
+  uint32_t n_segments;
+  const sdr_segment* segments;
+} sdr_run;
+
+/* One fixed-width column: inputs, caller-owned outputs, and the null count.
+ * Replaces gpu_column_decode_input (gpu_native_decode.cuh:67-73) plus the
+ * cudf::column the reference allocates for it.
+ *   d_out        total_rows * type_size bytes
+ *   d_null_mask  ceil(total_rows/8) bytes (bit set = row valid), or NULL;
+ *                written only when has_nulls and validity runs are given
+ *   null_count   out: number of null rows (0 when no mask is built) */
+typedef struct sdr_column {
+  uint32_t type_size; /* bytes per value: 1, 2, 4, 8 (16 for UNCOMPRESSED/CONSTANT) */
+  uint32_t total_rows;
+  const sdr_run* data;
+  uint32_t n_data;
+  uint32_t has_nulls;
+  const sdr_run* validity;
+  uint32_t n_validity;
+  uint32_t reserved;
+  void* d_out;
+  uint8_t* d_null_mask;
+  int64_t null_count;
+} sdr_column;
+
+/* One VARCHAR column.  Replaces gpu_string_column_decode_input
+ * (gpu_decode_strings.cuh:60-65) plus the strings column the reference builds.
+ *   d_offsets    int32[total_rows + 1]
+ *   d_chars      chars_capacity bytes
+ *   chars_used   out: total decoded chars (offsets[total_rows]); when it
+ *                exceeds chars_capacity the call returns SDR_E_OVERFLOW
+ *                after writing the offsets, and d_chars is left untouched */
+typedef struct sdr_string_column {
+  uint32_t total_rows;
+  uint32_t has_nulls;
+  const sdr_run* data;
+  uint32_t n_data;
+  uint32_t n_validity;
+  const sdr_run* validity;
+  int32_t* d_offsets;
+  uint8_t* d_chars;
+  uint64_t chars_capacity;
+  uint8_t* d_null_mask;
+  uint64_t chars_used;
+  int64_t null_count;
+} sdr_string_column;
+
+/* Launch strategies for the fixed-width and string paths. */
+typedef enum sdr_launch_mode {
+  SDR_LAUNCH_BATCHED     = 0, /* one launch per codec run (default)            */
+  SDR_LAUNCH_PER_SEGMENT = 1  /* one launch per segment (the plain baseline)   */
+} sdr_launch_mode;
+
+/* ---- library ----------------------------------------------------------- */
+
+/* Message of the last failing call on this thread ("" if none). */
+const char* sdr_last_error(void);
+/* Codec id for a DuckDB storage-info name, or SDR_E_UNSUPPORTED_CODEC. */
+int sdr_codec_from_name(const char* name);
+/* DuckDB storage-info name of a codec id, or "" */
+const char* sdr_codec_name(uint32_t codec);
+/* ABI version and gfx target the library was built for. */
+int sdr_abi_version(void);
+const char* sdr_build_target(void);
+
+/* ---- fixed-width decode ------------------------------------------------ */
+
+/* Decode every column (all must share total_rows).  Replaces
+ * gpu_decode_table (src/include/cuda/scan/gpu_native_decode.cuh:84-86,
+ * src/cuda/scan/gpu_native_decode.cu:546-608).  Synchronises `stream` once
+ * when a null mask is built (to return null_count), as the reference does in
+ * its batched null count (gpu_native_decode.cu:590-591); otherwise the call
+ * is asynchronous. */
+int sdr_decode_table(sdr_column* cols, uint32_t n_cols, sdr_stream_t stream);
+/* Same, with an explicit launch strategy. */
+int sdr_decode_table_mode(sdr_column* cols, uint32_t n_cols, sdr_stream_t stream,
+                          uint32_t launch_mode);
+
+/* Per-codec entries: decode one run into d_out (sized for the column; each
+ * segment writes row_count values at row_offset).  Replace
+ * decode_bitpacking_data (src/include/cuda/scan/gpu_decode_bitpacking.cuh:74-79),
+ * decode_rle_data (gpu_decode_rle.cuh:35-40), decode_alp_data and
+ * decode_alprd_data (gpu_decode_alp.cuh:50-62). */
+int sdr_decode_bitpacking_data(const sdr_run* run, void* d_out, uint32_t type_size,
+                               sdr_stream_t stream);
+int sdr_decode_rle_data(const sdr_run* run, void* d_out, uint32_t type_size, sdr_stream_t stream);
+int sdr_decode_alp_data(const sdr_run* run, void* d_out, uint32_t type_size, sdr_stream_t stream);
+int sdr_decode_alprd_data(const sdr_run* run, void* d_out, uint32_t type_size,
+                          sdr_stream_t stream);
+
+/* ---- prepared plans ---------------------------------------------------- */
+
+/* A plan uploads the segment descriptors of a set of fixed-width columns
+ * once; sdr_plan_launch then only enqueues kernels (no allocation, no host
+ * copy, no synchronisation), so it can be repeated and captured in a
+ * hipGraph.  It has no counterpart in the reference, whose gpu_decode_table
+ * rebuilds and uploads descriptors on every call
+ * (src/cuda/scan/gpu_native_decode.cu:546-608); a plan's launches decode
+ * exactly what sdr_decode_table_mode would.  The column structs, segment
+ * bytes and outputs must stay valid while the plan is used.  Null masks
+ * are rebuilt by every launch; null counts stay on the device until
+ * sdr_plan_null_counts (which synchronises `stream`). */
+typedef struct sdr_plan sdr_plan;
+
+int sdr_plan_create(const sdr_column* cols, uint32_t n_cols, uint32_t launch_mode, sdr_stream_t stream,
+                    sdr_plan** out);
+int sdr_plan_launch(sdr_plan* plan, sdr_stream_t stream);
+/* Launch only the kernels of one codec (codec 0 = all). */
+int sdr_plan_launch_codec(sdr_plan* plan, uint32_t codec, sdr_stream_t stream);
+/* Number of kernel launches one sdr_plan_launch enqueues. */
+int sdr_plan_num_launches(const sdr_plan* plan);
+int sdr_plan_null_counts(sdr_plan* plan, int64_t* counts, uint32_t n_cols, sdr_stream_t stream);
+int sdr_plan_destroy(sdr_plan* plan);
+
+/* ---- string decode ----------------------------------------------------- */
+
+/* Upper bound on decoded chars: sum(row_count * max_string_length), or
+ * UINT64_MAX if any segment reports max_string_length == 0.  Mirrors the
+ * reference's host sizing (src/cuda/scan/gpu_decode_strings.cu:183-191). */
+uint64_t sdr_strings_upper_bound(const sdr_run* runs, uint32_t n_runs);
+
+/* Decode one VARCHAR column.  Replaces gpu_decode_strings_column
+ * (src/include/cuda/scan/gpu_decode_strings.cuh:71-73,
+ * src/cuda/scan/gpu_decode_strings.cu:100-402).  Synchronises `stream` once
+ * to read back chars_used. */
+int sdr_decode_strings_column(sdr_string_column* col, sdr_stream_t stream);
+int sdr_decode_strings_column_mode(sdr_string_column* col, sdr_stream_t stream,
+                                   uint32_t launch_mode);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SDR_DECODE_H */

@@ -1,0 +1,105 @@
+"""ctypes mirror of include/sdr/decode.h (structs, enums, status codes)."""
+from __future__ import annotations
+
+import ctypes as C
+
+SDR_OK = 0
+SDR_E_INVALID_ARG = -1
+SDR_E_UNSUPPORTED_CODEC = -2
+SDR_E_BAD_WIDTH = -3
+SDR_E_BOUNDS = -4
+SDR_E_OVERFLOW = -5
+SDR_E_ALIGNMENT = -6
+SDR_E_HIP = -7
+SDR_E_CAPACITY = -8
+SDR_E_NOT_FOUND = -9
+SDR_E_STATE = -10
+
+STATUS_NAMES = {
+    0: "SDR_OK", -1: "SDR_E_INVALID_ARG", -2: "SDR_E_UNSUPPORTED_CODEC", -3: "SDR_E_BAD_WIDTH",
+    -4: "SDR_E_BOUNDS", -5: "SDR_E_OVERFLOW", -6: "SDR_E_ALIGNMENT", -7: "SDR_E_HIP",
+    -8: "SDR_E_CAPACITY", -9: "SDR_E_NOT_FOUND", -10: "SDR_E_STATE",
+}
+
+UNCOMPRESSED = 1
+CONSTANT = 2
+RLE = 3
+DICTIONARY = 4
+BITPACKING = 6
+FSST = 7
+ALP = 10
+ALPRD = 11
+DICT_FSST = 15
+
+CODEC_NAMES = {
+    UNCOMPRESSED: "Uncompressed", CONSTANT: "Constant", RLE: "RLE", DICTIONARY: "Dictionary",
+    BITPACKING: "BitPacking", FSST: "FSST", ALP: "ALP", ALPRD: "ALPRD", DICT_FSST: "DICT_FSST",
+}
+
+LAUNCH_BATCHED = 0
+LAUNCH_PER_SEGMENT = 1
+
+
+class Segment(C.Structure):
+    _fields_ = [
+        ("d_bytes", C.c_void_p),
+        ("bytes_size", C.c_uint32),
+        ("row_offset", C.c_uint32),
+        ("row_count", C.c_uint32),
+        ("max_string_length", C.c_uint32),
+    ]
+
+
+class Run(C.Structure):
+    _fields_ = [
+        ("codec", C.c_uint32),
+        ("n_segments", C.c_uint32),
+        ("segments", C.POINTER(Segment)),
+    ]
+
+
+class Column(C.Structure):
+    _fields_ = [
+        ("type_size", C.c_uint32),
+        ("total_rows", C.c_uint32),
+        ("data", C.POINTER(Run)),
+        ("n_data", C.c_uint32),
+        ("has_nulls", C.c_uint32),
+        ("validity", C.POINTER(Run)),
+        ("n_validity", C.c_uint32),
+        ("reserved", C.c_uint32),
+        ("d_out", C.c_void_p),
+        ("d_null_mask", C.c_void_p),
+        ("null_count", C.c_int64),
+    ]
+
+
+class StringColumn(C.Structure):
+    _fields_ = [
+        ("total_rows", C.c_uint32),
+        ("has_nulls", C.c_uint32),
+        ("data", C.POINTER(Run)),
+        ("n_data", C.c_uint32),
+        ("n_validity", C.c_uint32),
+        ("validity", C.POINTER(Run)),
+        ("d_offsets", C.c_void_p),
+        ("d_chars", C.c_void_p),
+        ("chars_capacity", C.c_uint64),
+        ("d_null_mask", C.c_void_p),
+        ("chars_used", C.c_uint64),
+        ("null_count", C.c_int64),
+    ]
+
+
+def make_runs(runs):
+    """[(codec, [Segment...]), ...] -> (Run array, keepalive list)."""
+    keep = []
+    arr = (Run * max(1, len(runs)))()
+    for i, (codec, segs) in enumerate(runs):
+        sa = (Segment * max(1, len(segs)))(*segs)
+        keep.append(sa)
+        arr[i].codec = codec
+        arr[i].n_segments = len(segs)
+        arr[i].segments = C.cast(sa, C.POINTER(Segment))
+    keep.append(arr)
+    return arr, keep

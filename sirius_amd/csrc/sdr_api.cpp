@@ -1,0 +1,499 @@
+// sdr_api.cpp — C ABI of the decoder (include/sdr/decode.h): validation,
+// descriptor upload and kernel dispatch for the fixed-width path.
+//
+// Validation mirrors the reference's viability throws
+// (src/cuda/scan/gpu_native_decode.cu:86-101, :382-385, :414-417, :496-500,
+// :555-570) but runs over every column before any device work is queued, so
+// a failing call leaves the outputs untouched.
+#include "sdr_internal.hpp"
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+namespace sdr {
+
+namespace {
+thread_local std::string g_err;
+}
+
+int fail(int code, const char* fmt, ...)
+{
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  std::vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+void clear_error() { g_err.clear(); }
+const char* last_error() { return g_err.c_str(); }
+
+// The stream-ordered allocator keeps freed blocks cached instead of
+// returning them to the driver at every synchronisation.
+void ensure_pool(int dev)
+{
+  static std::once_flag flags[64];
+  if (dev < 0 || dev >= 64) return;
+  std::call_once(flags[dev], [dev]() {
+    hipMemPool_t pool;
+    if (hipDeviceGetDefaultMemPool(&pool, dev) == hipSuccess) {
+      uint64_t thr = UINT64_MAX;
+      (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &thr);
+    }
+  });
+}
+
+int device_alloc(void** p, size_t bytes, hipStream_t s)
+{
+  *p = nullptr;
+  if (bytes == 0) return SDR_OK;
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  ensure_pool(dev);
+  HIP_TRY(hipMallocAsync(p, bytes, s));
+  return SDR_OK;
+}
+
+void device_free(void* p, hipStream_t s)
+{
+  if (p) (void)hipFreeAsync(p, s);
+}
+
+bool type_ok_for(uint32_t codec, uint32_t ts)
+{
+  switch (codec) {
+    case SDR_CODEC_BITPACKING:
+    case SDR_CODEC_RLE: return ts == 1 || ts == 2 || ts == 4 || ts == 8;
+    case SDR_CODEC_ALP:
+    case SDR_CODEC_ALPRD: return ts == 4 || ts == 8;
+    case SDR_CODEC_UNCOMPRESSED:
+    case SDR_CODEC_CONSTANT: return ts == 1 || ts == 2 || ts == 4 || ts == 8 || ts == 16;
+    default: return false;
+  }
+}
+
+bool codec_fixed(uint32_t codec)
+{
+  return codec == SDR_CODEC_BITPACKING || codec == SDR_CODEC_RLE || codec == SDR_CODEC_ALP ||
+         codec == SDR_CODEC_ALPRD || codec == SDR_CODEC_UNCOMPRESSED || codec == SDR_CODEC_CONSTANT;
+}
+
+int check_runs(const sdr_run* runs, uint32_t n, uint32_t total_rows, const char* what)
+{
+  if (n && !runs) return fail(SDR_E_INVALID_ARG, "%s runs pointer is NULL", what);
+  for (uint32_t r = 0; r < n; ++r) {
+    if (runs[r].n_segments && !runs[r].segments)
+      return fail(SDR_E_INVALID_ARG, "%s run %u has a NULL segment array", what, r);
+    for (uint32_t s = 0; s < runs[r].n_segments; ++s) {
+      const sdr_segment& g = runs[r].segments[s];
+      if (uint64_t{g.row_offset} + g.row_count > total_rows)
+        return fail(SDR_E_BOUNDS, "%s segment row_offset (%u) + row_count (%u) > total_rows (%u)", what,
+                    g.row_offset, g.row_count, total_rows);
+      if (g.row_count && !g.d_bytes) return fail(SDR_E_INVALID_ARG, "%s segment %u has NULL bytes", what, s);
+    }
+  }
+  return SDR_OK;
+}
+
+int check_validity_runs(const sdr_run* runs, uint32_t n, uint32_t total_rows)
+{
+  int rc = check_runs(runs, n, total_rows, "validity");
+  if (rc) return rc;
+  for (uint32_t r = 0; r < n; ++r) {
+    if (runs[r].codec != SDR_CODEC_UNCOMPRESSED)
+      return fail(SDR_E_UNSUPPORTED_CODEC, "validity codec %u not implemented", runs[r].codec);
+    for (uint32_t s = 0; s < runs[r].n_segments; ++s) {
+      const sdr_segment& g = runs[r].segments[s];
+      if (!g.row_count) continue;
+      if (g.row_offset % 8u) return fail(SDR_E_ALIGNMENT, "validity row_offset (%u) not byte-aligned", g.row_offset);
+      if (g.bytes_size < (g.row_count + 7u) / 8u)
+        return fail(SDR_E_BOUNDS, "validity segment bytes_size (%u) < required %u", g.bytes_size,
+                    (g.row_count + 7u) / 8u);
+    }
+  }
+  return SDR_OK;
+}
+
+// Overlay the UNCOMPRESSED validity bytes on an all-valid mask and count the
+// nulls into *d_count (src/cuda/scan/gpu_native_decode.cu:396-476, :519-538).
+int build_null_mask(const sdr_run* runs, uint32_t n, uint32_t total_rows, uint8_t* d_mask,
+                    unsigned long long* d_count, hipStream_t s)
+{
+  HIP_TRY(launch_fill_u8(d_mask, 0xFF, (total_rows + 7u) / 8u, s));
+  for (uint32_t r = 0; r < n; ++r)
+    for (uint32_t k = 0; k < runs[r].n_segments; ++k) {
+      const sdr_segment& g = runs[r].segments[k];
+      if (!g.row_count) continue;
+      HIP_TRY(hipMemcpyAsync(d_mask + g.row_offset / 8u, g.d_bytes, (g.row_count + 7u) / 8u,
+                             hipMemcpyDeviceToDevice, s));
+    }
+  HIP_TRY(launch_count_nulls(d_mask, total_rows, d_count, s));
+  return SDR_OK;
+}
+
+namespace {
+
+int validate_columns(const sdr_column* cols, uint32_t n_cols)
+{
+  if (n_cols && !cols) return fail(SDR_E_INVALID_ARG, "cols is NULL");
+  for (uint32_t c = 0; c < n_cols; ++c) {
+    const sdr_column& col = cols[c];
+    if (col.total_rows != cols[0].total_rows)
+      return fail(SDR_E_BOUNDS, "all columns must share the same total_rows");
+    if (col.total_rows > 0x7FFFFFFFu) return fail(SDR_E_BOUNDS, "total_rows (%u) > INT32_MAX", col.total_rows);
+    if (!(col.type_size == 1 || col.type_size == 2 || col.type_size == 4 || col.type_size == 8 ||
+          col.type_size == 16))
+      return fail(SDR_E_BAD_WIDTH, "column %u: type_size %u is not fixed-width", c, col.type_size);
+    if (col.total_rows && !col.d_out) return fail(SDR_E_INVALID_ARG, "column %u: d_out is NULL", c);
+    int rc = check_runs(col.data, col.n_data, col.total_rows, "data");
+    if (rc) return rc;
+    for (uint32_t r = 0; r < col.n_data; ++r) {
+      const sdr_run& run = col.data[r];
+      if (!codec_fixed(run.codec))
+        return fail(SDR_E_UNSUPPORTED_CODEC, "viability invariant violated: data codec %u not implemented", run.codec);
+      if (!type_ok_for(run.codec, col.type_size))
+        return fail(SDR_E_BAD_WIDTH, "viability invariant violated: %s type_size %u", sdr_codec_name(run.codec),
+                    col.type_size);
+      for (uint32_t s = 0; s < run.n_segments; ++s) {
+        const sdr_segment& g = run.segments[s];
+        if (!g.row_count) continue;
+        if (run.codec == SDR_CODEC_UNCOMPRESSED && uint64_t{g.bytes_size} < uint64_t{g.row_count} * col.type_size)
+          return fail(SDR_E_BOUNDS, "UNCOMPRESSED segment bytes_size (%u) < required %llu", g.bytes_size,
+                      (unsigned long long)(uint64_t{g.row_count} * col.type_size));
+        if (run.codec == SDR_CODEC_CONSTANT && g.bytes_size < col.type_size)
+          return fail(SDR_E_BOUNDS, "CONSTANT segment bytes_size (%u) < type_size (%u)", g.bytes_size, col.type_size);
+      }
+    }
+    if (col.has_nulls && col.n_validity) {
+      rc = check_validity_runs(col.validity, col.n_validity, col.total_rows);
+      if (rc) return rc;
+      if (col.total_rows && !col.d_null_mask) return fail(SDR_E_INVALID_ARG, "column %u: d_null_mask is NULL", c);
+    }
+  }
+  return SDR_OK;
+}
+
+hipError_t launch_batch(const Batch& b, const FixedSeg* d, uint32_t base, uint32_t n, uint32_t max_rows,
+                        hipStream_t s)
+{
+  switch (b.codec) {
+    case SDR_CODEC_BITPACKING: return launch_bitpacking(d, base, n, max_rows, b.ts, s);
+    case SDR_CODEC_RLE: return launch_rle(d, base, n, max_rows, b.ts, s);
+    case SDR_CODEC_ALP: return launch_alp(d, base, n, max_rows, b.ts, false, s);
+    case SDR_CODEC_ALPRD: return launch_alp(d, base, n, max_rows, b.ts, true, s);
+    case SDR_CODEC_UNCOMPRESSED: return launch_uncompressed(d, base, n, max_rows, b.ts, s);
+    case SDR_CODEC_CONSTANT: return launch_constant(d, base, n, max_rows, b.ts, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+}  // namespace
+
+// Group every live segment by (codec, type size): one device descriptor
+// array, one launch per group (or one per segment in the plain mode).
+int build_batches(const std::vector<RunTarget>& targets, std::vector<Batch>& batches, std::vector<FixedSeg>& all)
+{
+  std::vector<std::vector<FixedSeg>> per;
+  for (const RunTarget& t : targets) {
+    size_t bi = 0;
+    while (bi < batches.size() && !(batches[bi].codec == t.run->codec && batches[bi].ts == t.type_size)) ++bi;
+    if (bi == batches.size()) {
+      batches.push_back({t.run->codec, t.type_size, 0, 0, 0});
+      per.emplace_back();
+    }
+    for (uint32_t k = 0; k < t.run->n_segments; ++k) {
+      const sdr_segment& g = t.run->segments[k];
+      if (!g.row_count) continue;
+      FixedSeg f;
+      f.bytes      = g.d_bytes;
+      f.out        = static_cast<uint8_t*>(t.d_out) + uint64_t{g.row_offset} * t.type_size;
+      f.bytes_size = g.bytes_size;
+      f.row_count  = g.row_count;
+      per[bi].push_back(f);
+      batches[bi].max_rows = std::max(batches[bi].max_rows, g.row_count);
+    }
+  }
+  for (size_t i = 0; i < batches.size(); ++i) {
+    batches[i].first = static_cast<uint32_t>(all.size());
+    batches[i].count = static_cast<uint32_t>(per[i].size());
+    all.insert(all.end(), per[i].begin(), per[i].end());
+  }
+  return SDR_OK;
+}
+
+hipError_t launch_batches(const std::vector<Batch>& batches, const std::vector<FixedSeg>& all, const FixedSeg* d,
+                          uint32_t mode, uint32_t codec_filter, hipStream_t s, int* n_launches)
+{
+  hipError_t e = hipSuccess;
+  int n = 0;
+  for (size_t i = 0; e == hipSuccess && i < batches.size(); ++i) {
+    const Batch& b = batches[i];
+    if (!b.count || (codec_filter && b.codec != codec_filter)) continue;
+    if (mode == SDR_LAUNCH_PER_SEGMENT) {
+      for (uint32_t k = 0; k < b.count && e == hipSuccess; ++k, ++n)
+        e = launch_batch(b, d, b.first + k, 1, all[b.first + k].row_count, s);
+    } else {
+      e = launch_batch(b, d, b.first, b.count, b.max_rows, s);
+      ++n;
+    }
+  }
+  if (n_launches) *n_launches = n;
+  return e;
+}
+
+int decode_runs(const std::vector<RunTarget>& targets, hipStream_t s, uint32_t mode)
+{
+  std::vector<Batch> batches;
+  std::vector<FixedSeg> all;
+  build_batches(targets, batches, all);
+  if (all.empty()) return SDR_OK;
+  FixedSeg* d = nullptr;
+  int rc = device_alloc(reinterpret_cast<void**>(&d), all.size() * sizeof(FixedSeg), s);
+  if (rc) return rc;
+  hipError_t e = hipMemcpyAsync(d, all.data(), all.size() * sizeof(FixedSeg), hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) e = launch_batches(batches, all, d, mode, 0, s, nullptr);
+  device_free(d, s);
+  if (e != hipSuccess) return fail(SDR_E_HIP, "kernel launch failed: %s", hipGetErrorString(e));
+  return SDR_OK;
+}
+
+int decode_table(sdr_column* cols, uint32_t n_cols, hipStream_t s, uint32_t mode)
+{
+  clear_error();
+  if (mode != SDR_LAUNCH_BATCHED && mode != SDR_LAUNCH_PER_SEGMENT)
+    return fail(SDR_E_INVALID_ARG, "unknown launch mode %u", mode);
+  int rc = validate_columns(cols, n_cols);
+  if (rc) return rc;
+  if (n_cols == 0) return SDR_OK;
+  std::vector<RunTarget> targets;
+  for (uint32_t c = 0; c < n_cols; ++c) {
+    cols[c].null_count = 0;
+    for (uint32_t r = 0; r < cols[c].n_data; ++r) targets.push_back({&cols[c].data[r], cols[c].d_out, cols[c].type_size});
+  }
+  rc = decode_runs(targets, s, mode);
+  if (rc) return rc;
+  // validity: one counter per column, one synchronisation for all of them
+  std::vector<uint32_t> masked;
+  for (uint32_t c = 0; c < n_cols; ++c)
+    if (cols[c].has_nulls && cols[c].n_validity && cols[c].total_rows) masked.push_back(c);
+  if (!masked.empty()) {
+    unsigned long long* d_cnt = nullptr;
+    rc = device_alloc(reinterpret_cast<void**>(&d_cnt), masked.size() * sizeof(unsigned long long), s);
+    if (rc) return rc;
+    HIP_TRY(hipMemsetAsync(d_cnt, 0, masked.size() * sizeof(unsigned long long), s));
+    for (size_t i = 0; i < masked.size(); ++i) {
+      sdr_column& col = cols[masked[i]];
+      rc = build_null_mask(col.validity, col.n_validity, col.total_rows, col.d_null_mask, d_cnt + i, s);
+      if (rc) {
+        device_free(d_cnt, s);
+        return rc;
+      }
+    }
+    std::vector<unsigned long long> h(masked.size());
+    hipError_t e = hipMemcpyAsync(h.data(), d_cnt, h.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost, s);
+    device_free(d_cnt, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return fail(SDR_E_HIP, "null count: %s", hipGetErrorString(e));
+    for (size_t i = 0; i < masked.size(); ++i) cols[masked[i]].null_count = static_cast<int64_t>(h[i]);
+  }
+  hipError_t e = hipPeekAtLastError();
+  if (e != hipSuccess) return fail(SDR_E_HIP, "sticky launch error: %s", hipGetErrorString(e));
+  return SDR_OK;
+}
+
+int decode_one_run(uint32_t codec, const sdr_run* run, void* d_out, uint32_t ts, hipStream_t s)
+{
+  clear_error();
+  if (!run) return fail(SDR_E_INVALID_ARG, "run is NULL");
+  if (run->codec != codec) return fail(SDR_E_INVALID_ARG, "run codec %u does not match entry point %s", run->codec, sdr_codec_name(codec));
+  if (!type_ok_for(codec, ts)) return fail(SDR_E_BAD_WIDTH, "viability invariant violated: %s type_size %u", sdr_codec_name(codec), ts);
+  if (run->n_segments && !run->segments) return fail(SDR_E_INVALID_ARG, "run has a NULL segment array");
+  bool any = false;
+  for (uint32_t k = 0; k < run->n_segments; ++k) {
+    if (!run->segments[k].row_count) continue;
+    if (!run->segments[k].d_bytes) return fail(SDR_E_INVALID_ARG, "segment %u has NULL bytes", k);
+    any = true;
+  }
+  if (any && !d_out) return fail(SDR_E_INVALID_ARG, "d_out is NULL");
+  std::vector<RunTarget> t{{run, d_out, ts}};
+  return decode_runs(t, s, SDR_LAUNCH_BATCHED);
+}
+
+// ---- plans ------------------------------------------------------------------
+
+int plan_create(const sdr_column* cols, uint32_t n_cols, uint32_t mode, hipStream_t s, sdr_plan** out)
+{
+  clear_error();
+  if (!out) return fail(SDR_E_INVALID_ARG, "out is NULL");
+  *out = nullptr;
+  if (mode != SDR_LAUNCH_BATCHED && mode != SDR_LAUNCH_PER_SEGMENT)
+    return fail(SDR_E_INVALID_ARG, "unknown launch mode %u", mode);
+  int rc = validate_columns(cols, n_cols);
+  if (rc) return rc;
+  auto* p = new sdr_plan();
+  p->mode = mode;
+  std::vector<RunTarget> targets;
+  for (uint32_t c = 0; c < n_cols; ++c) {
+    for (uint32_t r = 0; r < cols[c].n_data; ++r) targets.push_back({&cols[c].data[r], cols[c].d_out, cols[c].type_size});
+    const bool m = cols[c].has_nulls && cols[c].n_validity && cols[c].total_rows;
+    p->col_masked.push_back(m);
+    if (m) p->masked.push_back(cols[c]);
+  }
+  p->n_cols = n_cols;
+  build_batches(targets, p->batches, p->all);
+  hipError_t e = hipSuccess;
+  if (!p->all.empty()) e = hipMalloc(reinterpret_cast<void**>(&p->d_segs), p->all.size() * sizeof(FixedSeg));
+  if (e == hipSuccess && !p->masked.empty())
+    e = hipMalloc(reinterpret_cast<void**>(&p->d_counts), p->masked.size() * sizeof(unsigned long long));
+  if (e == hipSuccess && p->d_segs)
+    e = hipMemcpyAsync(p->d_segs, p->all.data(), p->all.size() * sizeof(FixedSeg), hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e != hipSuccess) {
+    delete p;
+    return fail(SDR_E_HIP, "plan setup: %s", hipGetErrorString(e));
+  }
+  *out = p;
+  return SDR_OK;
+}
+
+int plan_launch(sdr_plan* p, uint32_t codec, hipStream_t s)
+{
+  clear_error();
+  if (!p) return fail(SDR_E_INVALID_ARG, "plan is NULL");
+  hipError_t e = launch_batches(p->batches, p->all, p->d_segs, p->mode, codec, s, nullptr);
+  if (e != hipSuccess) return fail(SDR_E_HIP, "kernel launch failed: %s", hipGetErrorString(e));
+  if (codec == 0 && !p->masked.empty()) {
+    HIP_TRY(launch_fill_u8(reinterpret_cast<uint8_t*>(p->d_counts), 0, p->masked.size() * sizeof(unsigned long long), s));
+    for (size_t i = 0; i < p->masked.size(); ++i) {
+      const sdr_column& col = p->masked[i];
+      int rc = build_null_mask(col.validity, col.n_validity, col.total_rows, col.d_null_mask, p->d_counts + i, s);
+      if (rc) return rc;
+    }
+  }
+  return SDR_OK;
+}
+
+}  // namespace sdr
+
+sdr_plan::~sdr_plan()
+{
+  if (d_segs) (void)hipFree(d_segs);
+  if (d_counts) (void)hipFree(d_counts);
+}
+
+// ---------------------------------------------------------------------------
+// extern "C"
+// ---------------------------------------------------------------------------
+using namespace sdr;
+
+extern "C" {
+
+const char* sdr_last_error(void) { return last_error(); }
+int sdr_abi_version(void) { return SDR_ABI_VERSION; }
+const char* sdr_build_target(void) { return "gfx950"; }
+
+static const struct {
+  const char* name;
+  uint32_t id;
+} k_codec_names[] = {{"Uncompressed", SDR_CODEC_UNCOMPRESSED}, {"Constant", SDR_CODEC_CONSTANT},
+                     {"RLE", SDR_CODEC_RLE},                   {"Dictionary", SDR_CODEC_DICTIONARY},
+                     {"BitPacking", SDR_CODEC_BITPACKING},     {"FSST", SDR_CODEC_FSST},
+                     {"ALP", SDR_CODEC_ALP},                   {"ALPRD", SDR_CODEC_ALPRD},
+                     {"DICT_FSST", SDR_CODEC_DICT_FSST}};
+
+int sdr_codec_from_name(const char* name)
+{
+  if (!name) return SDR_E_INVALID_ARG;
+  for (auto& c : k_codec_names)
+    if (std::strcmp(c.name, name) == 0) return static_cast<int>(c.id);
+  return SDR_E_UNSUPPORTED_CODEC;
+}
+
+const char* sdr_codec_name(uint32_t codec)
+{
+  for (auto& c : k_codec_names)
+    if (c.id == codec) return c.name;
+  return "";
+}
+
+int sdr_decode_table(sdr_column* cols, uint32_t n_cols, sdr_stream_t stream)
+{
+  return decode_table(cols, n_cols, static_cast<hipStream_t>(stream), SDR_LAUNCH_BATCHED);
+}
+
+int sdr_decode_table_mode(sdr_column* cols, uint32_t n_cols, sdr_stream_t stream, uint32_t launch_mode)
+{
+  return decode_table(cols, n_cols, static_cast<hipStream_t>(stream), launch_mode);
+}
+
+int sdr_plan_create(const sdr_column* cols, uint32_t n_cols, uint32_t launch_mode, sdr_stream_t stream,
+                    sdr_plan** out)
+{
+  return plan_create(cols, n_cols, launch_mode, static_cast<hipStream_t>(stream), out);
+}
+
+int sdr_plan_launch(sdr_plan* plan, sdr_stream_t stream) { return plan_launch(plan, 0, static_cast<hipStream_t>(stream)); }
+
+int sdr_plan_launch_codec(sdr_plan* plan, uint32_t codec, sdr_stream_t stream)
+{
+  return plan_launch(plan, codec, static_cast<hipStream_t>(stream));
+}
+
+int sdr_plan_num_launches(const sdr_plan* plan)
+{
+  if (!plan) return SDR_E_INVALID_ARG;
+  int n = 0;
+  for (const auto& b : plan->batches) n += b.count ? (plan->mode == SDR_LAUNCH_PER_SEGMENT ? static_cast<int>(b.count) : 1) : 0;
+  return n;
+}
+
+int sdr_plan_null_counts(sdr_plan* plan, int64_t* counts, uint32_t n_cols, sdr_stream_t stream)
+{
+  clear_error();
+  if (!plan || (n_cols && !counts)) return fail(SDR_E_INVALID_ARG, "NULL argument");
+  if (n_cols < plan->n_cols) return fail(SDR_E_INVALID_ARG, "counts holds %u entries, plan has %u columns", n_cols, plan->n_cols);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  std::vector<unsigned long long> h(plan->masked.size());
+  if (!h.empty()) {
+    HIP_TRY(hipMemcpyAsync(h.data(), plan->d_counts, h.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+  }
+  HIP_TRY(hipStreamSynchronize(s));
+  for (uint32_t c = 0; c < plan->n_cols; ++c) counts[c] = 0;
+  // masked columns are recorded in column order
+  size_t mi = 0;
+  for (uint32_t c = 0; c < plan->n_cols && mi < plan->masked.size(); ++c)
+    if (plan->col_masked.size() > c && plan->col_masked[c]) counts[c] = static_cast<int64_t>(h[mi++]);
+  return SDR_OK;
+}
+
+int sdr_plan_destroy(sdr_plan* plan)
+{
+  delete plan;
+  return SDR_OK;
+}
+
+int sdr_decode_bitpacking_data(const sdr_run* run, void* d_out, uint32_t ts, sdr_stream_t stream)
+{
+  return decode_one_run(SDR_CODEC_BITPACKING, run, d_out, ts, static_cast<hipStream_t>(stream));
+}
+int sdr_decode_rle_data(const sdr_run* run, void* d_out, uint32_t ts, sdr_stream_t stream)
+{
+  return decode_one_run(SDR_CODEC_RLE, run, d_out, ts, static_cast<hipStream_t>(stream));
+}
+int sdr_decode_alp_data(const sdr_run* run, void* d_out, uint32_t ts, sdr_stream_t stream)
+{
+  return decode_one_run(SDR_CODEC_ALP, run, d_out, ts, static_cast<hipStream_t>(stream));
+}
+int sdr_decode_alprd_data(const sdr_run* run, void* d_out, uint32_t ts, sdr_stream_t stream)
+{
+  return decode_one_run(SDR_CODEC_ALPRD, run, d_out, ts, static_cast<hipStream_t>(stream));
+}
+
+}  // extern "C"

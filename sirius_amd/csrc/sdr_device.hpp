@@ -1,0 +1,188 @@
+// sdr_device.hpp — gfx950 device primitives shared by the decode kernels.
+//
+// Wave64 throughout: a wavefront is 64 lanes; rows of 16 lanes are the DPP
+// unit.  Segment bytes may start at any byte address, so every scalar header
+// read goes through the aligned-word loaders below (they read only the
+// 4-byte words that hold the requested bytes, which can never cross a page
+// the requested bytes do not also touch).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace sdr {
+
+constexpr uint32_t WAVE = 64;
+
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
+
+// Orders LDS traffic between lanes of one wavefront.  LDS instructions of a
+// wave execute in issue order; this keeps the compiler from moving accesses
+// across the point and waits for outstanding LDS operations.
+__device__ __forceinline__ void wave_lds_sync()
+{
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// ---- unaligned loads -------------------------------------------------------
+
+__device__ __forceinline__ uint32_t ld_u8(const uint8_t* p) { return *p; }
+
+__device__ __forceinline__ uint32_t ld_u16_any(const uint8_t* p)
+{
+  uintptr_t a = reinterpret_cast<uintptr_t>(p);
+  uint32_t o  = static_cast<uint32_t>(a & 3u);
+  const uint32_t* w = reinterpret_cast<const uint32_t*>(a - o);
+  uint32_t lo = w[0];
+  uint32_t v  = (o <= 2u) ? (lo >> (8u * o)) : __builtin_amdgcn_alignbit(w[1], lo, 24u);
+  return v & 0xFFFFu;
+}
+
+__device__ __forceinline__ uint32_t ld_u32_any(const uint8_t* p)
+{
+  uintptr_t a = reinterpret_cast<uintptr_t>(p);
+  uint32_t o  = static_cast<uint32_t>(a & 3u);
+  const uint32_t* w = reinterpret_cast<const uint32_t*>(a - o);
+  uint32_t lo = w[0];
+  if (o == 0) return lo;
+  return __builtin_amdgcn_alignbit(w[1], lo, 8u * o);
+}
+
+__device__ __forceinline__ uint64_t ld_u64_any(const uint8_t* p)
+{
+  return static_cast<uint64_t>(ld_u32_any(p)) | (static_cast<uint64_t>(ld_u32_any(p + 4)) << 32);
+}
+
+// Load a T-sized little-endian value (1, 2, 4 or 8 bytes) zero-extended.
+template <uint32_t TS>
+__device__ __forceinline__ uint64_t ld_t_any(const uint8_t* p)
+{
+  if constexpr (TS == 1) return ld_u8(p);
+  else if constexpr (TS == 2) return ld_u16_any(p);
+  else if constexpr (TS == 4) return ld_u32_any(p);
+  else return ld_u64_any(p);
+}
+
+// ---- wave scans (DPP row shifts + cross-row readlane) ----------------------
+
+template <int CTRL>
+__device__ __forceinline__ uint32_t dpp_u32(uint32_t v)
+{
+  return static_cast<uint32_t>(
+    __builtin_amdgcn_update_dpp(0, static_cast<int>(v), CTRL, 0xF, 0xF, false));
+}
+
+// Inclusive prefix sum over the 64 lanes (wrapping).
+__device__ __forceinline__ uint32_t wave_incl_sum_u32(uint32_t v)
+{
+  v += dpp_u32<0x111>(v);  // row_shr:1
+  v += dpp_u32<0x112>(v);  // row_shr:2
+  v += dpp_u32<0x114>(v);  // row_shr:4
+  v += dpp_u32<0x118>(v);  // row_shr:8
+  uint32_t r15 = __builtin_amdgcn_readlane(v, 15);
+  uint32_t r31 = __builtin_amdgcn_readlane(v, 31);
+  uint32_t r47 = __builtin_amdgcn_readlane(v, 47);
+  uint32_t row = lane_id() >> 4;
+  uint32_t add = (row >= 1 ? r15 : 0u) + (row >= 2 ? r31 : 0u) + (row >= 3 ? r47 : 0u);
+  return v + add;
+}
+
+template <int CTRL>
+__device__ __forceinline__ uint64_t dpp_u64(uint64_t v)
+{
+  uint32_t lo = dpp_u32<CTRL>(static_cast<uint32_t>(v));
+  uint32_t hi = dpp_u32<CTRL>(static_cast<uint32_t>(v >> 32));
+  return static_cast<uint64_t>(lo) | (static_cast<uint64_t>(hi) << 32);
+}
+
+__device__ __forceinline__ uint64_t readlane_u64(uint64_t v, int l)
+{
+  uint32_t lo = __builtin_amdgcn_readlane(static_cast<uint32_t>(v), l);
+  uint32_t hi = __builtin_amdgcn_readlane(static_cast<uint32_t>(v >> 32), l);
+  return static_cast<uint64_t>(lo) | (static_cast<uint64_t>(hi) << 32);
+}
+
+__device__ __forceinline__ uint64_t wave_incl_sum_u64(uint64_t v)
+{
+  v += dpp_u64<0x111>(v);
+  v += dpp_u64<0x112>(v);
+  v += dpp_u64<0x114>(v);
+  v += dpp_u64<0x118>(v);
+  uint64_t r15 = readlane_u64(v, 15);
+  uint64_t r31 = readlane_u64(v, 31);
+  uint64_t r47 = readlane_u64(v, 47);
+  uint32_t row = lane_id() >> 4;
+  uint64_t add = (row >= 1 ? r15 : 0ull) + (row >= 2 ? r31 : 0ull) + (row >= 3 ? r47 : 0ull);
+  return v + add;
+}
+
+__device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
+
+// ---- LDS bit extraction ----------------------------------------------------
+
+// `w` bits (w <= 32) at bit `bitpos` of an LSB-first stream held in LDS words.
+__device__ __forceinline__ uint32_t lds_bits32(const uint32_t* lds, uint32_t bitpos, uint32_t w)
+{
+  uint32_t word = bitpos >> 5;
+  uint32_t v    = __builtin_amdgcn_alignbit(lds[word + 1], lds[word], bitpos & 31u);
+  return w >= 32u ? v : (v & ((1u << w) - 1u));
+}
+
+// `w` bits (w <= 64) at bit `bitpos`.
+__device__ __forceinline__ uint64_t lds_bits64(const uint32_t* lds, uint32_t bitpos, uint32_t w)
+{
+  uint32_t word = bitpos >> 5;
+  uint32_t sh   = bitpos & 31u;
+  uint32_t w0 = lds[word], w1 = lds[word + 1], w2 = lds[word + 2];
+  uint64_t v = static_cast<uint64_t>(__builtin_amdgcn_alignbit(w1, w0, sh)) |
+               (static_cast<uint64_t>(__builtin_amdgcn_alignbit(w2, w1, sh)) << 32);
+  return w >= 64u ? v : (v & ((1ull << w) - 1ull));
+}
+
+// Copy the bytes [p, p + nbytes) of global memory into LDS as whole 16-byte
+// blocks: LDS byte 0 holds the byte at align_down(p, 16).  Returns the bit
+// offset of p in the staged words.  Every 16-byte block read overlaps the
+// requested range, so the copy never touches a page the range does not.
+__device__ __forceinline__ uint32_t stage_to_lds(uint32_t* lds, const uint8_t* p, uint32_t nbytes)
+{
+  uintptr_t a    = reinterpret_cast<uintptr_t>(p);
+  uintptr_t a16  = a & ~uintptr_t{15};
+  uint32_t head  = static_cast<uint32_t>(a - a16);
+  uint32_t nblk  = nbytes ? (head + nbytes + 15u) >> 4 : 0u;
+  const u32x4* src = reinterpret_cast<const u32x4*>(a16);
+  u32x4* dst       = reinterpret_cast<u32x4*>(lds);
+  for (uint32_t b = lane_id(); b < nblk; b += WAVE) dst[b] = __builtin_nontemporal_load(src + b);
+  return head * 8u;
+}
+
+// ---- stores ----------------------------------------------------------------
+
+template <typename T>
+__device__ __forceinline__ void st_nt(T* p, T v)
+{
+  __builtin_nontemporal_store(v, p);
+}
+
+// Store VPL = 16 / sizeof(T) consecutive values; one 16-byte store when the
+// destination is 16-byte aligned and all values are valid.
+template <typename T>
+__device__ __forceinline__ void store_vec(T* p, const T* v, uint32_t n_valid, bool aligned16)
+{
+  constexpr uint32_t VPL = 16 / sizeof(T);
+  if (aligned16 && n_valid >= VPL) {
+    u32x4 x;
+    __builtin_memcpy(&x, v, 16);
+    __builtin_nontemporal_store(x, reinterpret_cast<u32x4*>(p));
+  } else {
+#pragma unroll
+    for (uint32_t i = 0; i < VPL; ++i)
+      if (i < n_valid) st_nt(p + i, v[i]);
+  }
+}
+
+}  // namespace sdr

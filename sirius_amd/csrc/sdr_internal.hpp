@@ -1,0 +1,62 @@
+// sdr_internal.hpp — host-side helpers shared by the C ABI translation units.
+#pragma once
+
+#include "../../include/sdr/decode.h"
+#include "sdr_kernels.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+namespace sdr {
+
+int fail(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
+void clear_error();
+const char* last_error();
+
+#define HIP_TRY(expr)                                                                   \
+  do {                                                                                  \
+    hipError_t _e = (expr);                                                             \
+    if (_e != hipSuccess) return ::sdr::fail(SDR_E_HIP, "%s: %s", #expr, hipGetErrorString(_e)); \
+  } while (0)
+
+void ensure_pool(int dev);
+int device_alloc(void** p, size_t bytes, hipStream_t s);
+void device_free(void* p, hipStream_t s);
+
+bool type_ok_for(uint32_t codec, uint32_t ts);
+bool codec_fixed(uint32_t codec);
+int check_runs(const sdr_run* runs, uint32_t n, uint32_t total_rows, const char* what);
+int check_validity_runs(const sdr_run* runs, uint32_t n, uint32_t total_rows);
+int build_null_mask(const sdr_run* runs, uint32_t n, uint32_t total_rows, uint8_t* d_mask,
+                    unsigned long long* d_count, hipStream_t s);
+
+// One fixed-width run and where its rows go.
+struct RunTarget {
+  const sdr_run* run;
+  void* d_out;
+  uint32_t type_size;
+};
+int decode_runs(const std::vector<RunTarget>& targets, hipStream_t s, uint32_t mode);
+
+// Segments grouped by (codec, type size); [first, first + count) in the
+// descriptor array.
+struct Batch {
+  uint32_t codec, ts;
+  uint32_t first, count;
+  uint32_t max_rows;
+};
+
+}  // namespace sdr
+
+struct sdr_plan {
+  uint32_t mode   = 0;
+  uint32_t n_cols = 0;
+  std::vector<sdr::Batch> batches;
+  std::vector<sdr::FixedSeg> all;
+  sdr::FixedSeg* d_segs = nullptr;
+  std::vector<sdr_column> masked;  // columns with a null mask, in column order
+  std::vector<bool> col_masked;
+  unsigned long long* d_counts = nullptr;
+  ~sdr_plan();
+};

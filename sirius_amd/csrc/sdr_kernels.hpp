@@ -1,0 +1,40 @@
+// sdr_kernels.hpp — host-side launch wrappers of the decode kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace sdr {
+
+// One fixed-width segment as the kernels see it.  `out` already points at
+// the segment's first output row (d_out + row_offset * type_size).
+struct FixedSeg {
+  const uint8_t* bytes;
+  uint8_t* out;
+  uint32_t bytes_size;
+  uint32_t row_count;
+};
+static_assert(sizeof(FixedSeg) == 24, "FixedSeg layout");
+
+// Rows each wavefront of the RLE kernel expands (one work unit).
+constexpr uint32_t RLE_UNIT_ROWS = 16384;
+
+// Launch the decode of segments [seg_base, seg_base + n) of `d_segs` (device
+// array).  `max_rows` is the largest row_count among them (sizes the grid).
+hipError_t launch_bitpacking(const FixedSeg* d_segs, uint32_t seg_base, uint32_t n, uint32_t max_rows,
+                             uint32_t type_size, hipStream_t s);
+hipError_t launch_rle(const FixedSeg* d_segs, uint32_t seg_base, uint32_t n, uint32_t max_rows,
+                      uint32_t type_size, hipStream_t s);
+hipError_t launch_alp(const FixedSeg* d_segs, uint32_t seg_base, uint32_t n, uint32_t max_rows,
+                      uint32_t type_size, bool rd, hipStream_t s);
+hipError_t launch_uncompressed(const FixedSeg* d_segs, uint32_t seg_base, uint32_t n, uint32_t max_rows,
+                               uint32_t type_size, hipStream_t s);
+hipError_t launch_constant(const FixedSeg* d_segs, uint32_t seg_base, uint32_t n, uint32_t max_rows,
+                           uint32_t type_size, hipStream_t s);
+
+// Validity: fill mask bytes, then count zero bits among the first `rows`.
+hipError_t launch_fill_u8(uint8_t* p, uint8_t v, uint64_t n, hipStream_t s);
+hipError_t launch_count_nulls(const uint8_t* mask, uint32_t rows, unsigned long long* d_count,
+                              hipStream_t s);
+
+}  // namespace sdr

@@ -1,0 +1,83 @@
+"""Loader for the HIP decode library (libsdr.so, built in-tree).
+
+The product path has no fallback: if the library is missing or cannot be
+loaded, every entry raises.  Build it with ``python -m sirius_amd.build``.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+from . import _abi
+
+LIB_PATH = Path(__file__).resolve().parent / "libsdr.so"
+
+# every symbol declared in include/sdr/decode.h (checked by the tests)
+DECLARED = [
+    "sdr_last_error", "sdr_codec_from_name", "sdr_codec_name", "sdr_abi_version", "sdr_build_target",
+    "sdr_decode_table", "sdr_decode_table_mode",
+    "sdr_decode_bitpacking_data", "sdr_decode_rle_data", "sdr_decode_alp_data", "sdr_decode_alprd_data",
+    "sdr_strings_upper_bound", "sdr_decode_strings_column", "sdr_decode_strings_column_mode",
+    "sdr_plan_create", "sdr_plan_launch", "sdr_plan_launch_codec", "sdr_plan_num_launches",
+    "sdr_plan_null_counts", "sdr_plan_destroy",
+]
+
+_lib = None
+
+
+class SdrError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"{_abi.STATUS_NAMES.get(code, code)}: {msg}")
+        self.code = code
+
+
+def load() -> C.CDLL:
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise RuntimeError(f"HIP decode library not built: {LIB_PATH} (run python -m sirius_amd.build)")
+    lib = C.CDLL(str(LIB_PATH))
+    vp, u32, i32 = C.c_void_p, C.c_uint32, C.c_int
+    P = C.POINTER
+    lib.sdr_last_error.restype = C.c_char_p
+    lib.sdr_codec_from_name.argtypes = [C.c_char_p]
+    lib.sdr_codec_from_name.restype = i32
+    lib.sdr_codec_name.argtypes = [u32]
+    lib.sdr_codec_name.restype = C.c_char_p
+    lib.sdr_abi_version.restype = i32
+    lib.sdr_build_target.restype = C.c_char_p
+    lib.sdr_decode_table.argtypes = [P(_abi.Column), u32, vp]
+    lib.sdr_decode_table.restype = i32
+    lib.sdr_decode_table_mode.argtypes = [P(_abi.Column), u32, vp, u32]
+    lib.sdr_decode_table_mode.restype = i32
+    for name in ("sdr_decode_bitpacking_data", "sdr_decode_rle_data", "sdr_decode_alp_data",
+                 "sdr_decode_alprd_data"):
+        f = getattr(lib, name)
+        f.argtypes = [P(_abi.Run), vp, u32, vp]
+        f.restype = i32
+    lib.sdr_strings_upper_bound.argtypes = [P(_abi.Run), u32]
+    lib.sdr_strings_upper_bound.restype = C.c_uint64
+    lib.sdr_decode_strings_column.argtypes = [P(_abi.StringColumn), vp]
+    lib.sdr_decode_strings_column.restype = i32
+    lib.sdr_decode_strings_column_mode.argtypes = [P(_abi.StringColumn), vp, u32]
+    lib.sdr_decode_strings_column_mode.restype = i32
+    lib.sdr_plan_create.argtypes = [P(_abi.Column), u32, u32, vp, P(vp)]
+    lib.sdr_plan_create.restype = i32
+    lib.sdr_plan_launch.argtypes = [vp, vp]
+    lib.sdr_plan_launch.restype = i32
+    lib.sdr_plan_launch_codec.argtypes = [vp, u32, vp]
+    lib.sdr_plan_launch_codec.restype = i32
+    lib.sdr_plan_num_launches.argtypes = [vp]
+    lib.sdr_plan_num_launches.restype = i32
+    lib.sdr_plan_null_counts.argtypes = [vp, P(C.c_int64), u32, vp]
+    lib.sdr_plan_null_counts.restype = i32
+    lib.sdr_plan_destroy.argtypes = [vp]
+    lib.sdr_plan_destroy.restype = i32
+    _lib = lib
+    return lib
+
+
+def check(code: int) -> None:
+    if code != _abi.SDR_OK:
+        raise SdrError(code, load().sdr_last_error().decode())

@@ -1,0 +1,94 @@
+"""The C-ABI libraries load and export every symbol the headers declare;
+argument validation answers without touching a GPU."""
+import ctypes as C
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import sirius_amd._abi as A
+from sirius_amd import lib as L
+
+ROOT = Path(__file__).resolve().parent.parent
+
+
+def declared(header: Path):
+    text = re.sub(r"/\*.*?\*/", "", header.read_text(), flags=re.S)
+    return sorted(set(re.findall(r"\b(sdr_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_decode_header_symbols_exported():
+    names = declared(ROOT / "include" / "sdr" / "decode.h")
+    assert "sdr_decode_table" in names and "sdr_decode_strings_column" in names
+    lib = L.load()
+    for n in names:
+        assert hasattr(lib, n), f"{n} declared in include/sdr/decode.h but not exported"
+    assert sorted(L.DECLARED) == names
+
+
+def test_encoder_header_symbols_exported():
+    import sirius_amd.encode as E
+
+    lib = E._load()
+    for n in declared(ROOT / "sirius_amd" / "csrc" / "encode" / "sdr_encode.h"):
+        assert hasattr(lib, n), n
+
+
+def test_codec_names_roundtrip():
+    lib = L.load()
+    for cid, name in A.CODEC_NAMES.items():
+        assert lib.sdr_codec_from_name(name.encode()) == cid
+        assert lib.sdr_codec_name(cid).decode() == name
+    assert lib.sdr_codec_from_name(b"Chimp") == A.SDR_E_UNSUPPORTED_CODEC
+    assert lib.sdr_abi_version() == 1
+    assert lib.sdr_build_target() == b"gfx950"
+
+
+def _col(codec, ts, total, rows, row_offset=0, bytes_size=64):
+    seg = A.Segment(0x1000, bytes_size, row_offset, rows, 0)
+    runs, keep = A.make_runs([(codec, [seg])])
+    col = A.Column()
+    col.type_size, col.total_rows, col.data, col.n_data, col.d_out = ts, total, runs, 1, 0x2000
+    return col, keep
+
+
+@pytest.mark.parametrize("codec,ts,total,rows,off,code", [
+    (A.FSST, 4, 10, 10, 0, A.SDR_E_UNSUPPORTED_CODEC),
+    (A.DICTIONARY, 4, 10, 10, 0, A.SDR_E_UNSUPPORTED_CODEC),
+    (A.ALP, 2, 10, 10, 0, A.SDR_E_BAD_WIDTH),
+    (A.BITPACKING, 16, 10, 10, 0, A.SDR_E_BAD_WIDTH),
+    (A.RLE, 3, 10, 10, 0, A.SDR_E_BAD_WIDTH),
+    (A.BITPACKING, 4, 10, 8, 5, A.SDR_E_BOUNDS),
+])
+def test_validation_errors_without_device_work(codec, ts, total, rows, off, code):
+    col, keep = _col(codec, ts, total, rows, off)
+    arr = (A.Column * 1)(col)
+    assert L.load().sdr_decode_table(arr, 1, None) == code
+    assert L.load().sdr_last_error()
+
+
+def test_mismatched_total_rows_and_short_uncompressed():
+    a, k1 = _col(A.BITPACKING, 4, 10, 10)
+    b, k2 = _col(A.BITPACKING, 4, 11, 11)
+    arr = (A.Column * 2)(a, b)
+    assert L.load().sdr_decode_table(arr, 2, None) == A.SDR_E_BOUNDS
+    c, k3 = _col(A.UNCOMPRESSED, 8, 10, 10, 0, bytes_size=79)
+    arr = (A.Column * 1)(c)
+    assert L.load().sdr_decode_table(arr, 1, None) == A.SDR_E_BOUNDS
+    d, k4 = _col(A.CONSTANT, 8, 10, 10, 0, bytes_size=7)
+    arr = (A.Column * 1)(d)
+    assert L.load().sdr_decode_table(arr, 1, None) == A.SDR_E_BOUNDS
+
+
+def test_empty_table_is_ok():
+    assert L.load().sdr_decode_table(None, 0, None) == A.SDR_OK
+
+
+def test_strings_upper_bound():
+    segs = [A.Segment(0x1000, 100, 0, 10, 44), A.Segment(0x2000, 100, 10, 5, 20)]
+    runs, keep = A.make_runs([(A.FSST, segs)])
+    assert L.load().sdr_strings_upper_bound(runs, 1) == 10 * 44 + 5 * 20
+    segs[1].max_string_length = 0
+    runs, keep = A.make_runs([(A.FSST, segs)])
+    assert L.load().sdr_strings_upper_bound(runs, 1) == 2 ** 64 - 1
