@@ -34,15 +34,22 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def build_workload(rows: int):
+WORKLOADS = ("int_date_mix_64M", "copy_256M")
+
+
+def build_workload(name: str, rows: int):
     import sirius_amd.encode as E
 
     t = time.time()
-    cols = [
-        ("rle_int64_runs", E.gen_runs_int64(0xC0DE0002, rows, 16.0, 1 << 40), E.encode_rle),
-        ("bp_int32_quantity", E.gen_uniform_int32(0xC0DE0012, rows, 1, 50), E.encode_bitpacking),
-        ("bp_date_sorted", E.gen_dates_sorted(0xC0DE0022, rows), E.encode_bitpacking),
-    ]
+    if name == "copy_256M":
+        # calibration: UNCOMPRESSED INT64 (a plain copy of known size)
+        cols = [("uncompressed_int64", E.gen_u64(0xC0DE00FF, rows).view(np.int64), E.encode_uncompressed)]
+    else:
+        cols = [
+            ("rle_int64_runs", E.gen_runs_int64(0xC0DE0002, rows, 16.0, 1 << 40), E.encode_rle),
+            ("bp_int32_quantity", E.gen_uniform_int32(0xC0DE0012, rows, 1, 50), E.encode_bitpacking),
+            ("bp_date_sorted", E.gen_dates_sorted(0xC0DE0022, rows), E.encode_bitpacking),
+        ]
     out = []
     for name, values, enc_f in cols:
         enc = enc_f(values)
@@ -58,9 +65,14 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--rows", type=int, default=64 << 20)
+    ap.add_argument("--rows", type=int, default=None)
+    ap.add_argument("--workload", choices=WORKLOADS, default="int_date_mix_64M")
+    ap.add_argument("--only", choices=("all", "batched"), default="all",
+                    help="batched: time only the drop-in path (for profiler runs)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
+    if args.rows is None:
+        args.rows = (32 << 20) if args.workload == "copy_256M" else (64 << 20)
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (this build decodes on a "
               f"single MI355X; multi-GPU sharding is described in DESIGN.md)", file=sys.stderr)
@@ -73,7 +85,7 @@ def main() -> int:
 
     lib = L.load()
     dev = torch.device("cuda:0")
-    work = build_workload(args.rows)
+    work = build_workload(args.workload, args.rows)
     rows = args.rows
 
     keep, cols, outs = [], [], []
@@ -123,6 +135,10 @@ def main() -> int:
     torch.cuda.synchronize()
     exact = all(np.array_equal(out.cpu().numpy(), values.view(np.uint8)) for (_, values, _), out in zip(work, outs))
     log(f"[bench] batched {ms_batched:.3f} ms/step, bit-exact={exact}")
+    if args.only == "batched":
+        print(json.dumps({"workload": args.workload, "ms_per_step": ms_batched, "bit_exact": bool(exact),
+                          "decoded_bytes": dec_bytes, "compressed_bytes": comp_bytes}), flush=True)
+        return 0
     log("[bench] timing plain (one launch per segment) path")
     ms_plain, wall_plain = timed(step_plain, max(5, args.steps // 5), 2)
     log(f"[bench] plain {ms_plain:.3f} ms/step")
@@ -168,7 +184,7 @@ def main() -> int:
         "vs_baseline": None,
         "dtype": "int64+int32",
         "data": "synthetic (DuckDB-format segments, self-encoded; DuckDB absent)",
-        "config": {"workload": "int_date_mix_64M", "rows": rows,
+        "config": {"workload": args.workload, "rows": rows,
                    "columns": ["RLE INT64 runs(mean 16)", "BITPACKING INT32 1..50", "BITPACKING DATE sorted/2048"],
                    "segments": int(sum(len(e.segs) for _, _, e in work)),
                    "compressed_bytes": comp_bytes, "decoded_bytes": dec_bytes},
@@ -220,7 +236,7 @@ def cpu_baseline(work, rows):
     # all columns must share total_rows for the table call: decode per column
     reps, elapsed = 0, 0.0
     t0 = time.perf_counter()
-    while elapsed < 10.0 and reps < 50:
+    while elapsed < 10.0 and reps < 2000:
         for col in cols:
             rc = lib.oracle_decode_table_mt(C.byref(col), 1, threads)
             assert rc == 0

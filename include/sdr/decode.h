@@ -131,8 +131,9 @@ typedef struct sdr_string_column {
 
 /* Launch strategies for the fixed-width and string paths. */
 typedef enum sdr_launch_mode {
-  SDR_LAUNCH_BATCHED     = 0, /* one launch per codec run (default)            */
-  SDR_LAUNCH_PER_SEGMENT = 1  /* one launch per segment (the plain baseline)   */
+  SDR_LAUNCH_BATCHED     = 0, /* one launch per codec and type size (default)  */
+  SDR_LAUNCH_PER_SEGMENT = 1, /* one launch per segment (the plain baseline)   */
+  SDR_LAUNCH_PERSISTENT  = 2  /* one persistent mixed-codec launch (sdr_batch) */
 } sdr_launch_mode;
 
 /* ---- library ----------------------------------------------------------- */
@@ -195,6 +196,45 @@ int sdr_plan_launch_codec(sdr_plan* plan, uint32_t codec, sdr_stream_t stream);
 int sdr_plan_num_launches(const sdr_plan* plan);
 int sdr_plan_null_counts(sdr_plan* plan, int64_t* counts, uint32_t n_cols, sdr_stream_t stream);
 int sdr_plan_destroy(sdr_plan* plan);
+
+/* ---- persistent batches ------------------------------------------------- */
+
+/* A batch collects the work of many columns (fixed-width and VARCHAR, any
+ * codec mix) into one device-resident work queue that a single persistent
+ * kernel drains; the queue is uploaded from pinned memory with one copy and
+ * the launch is replayed from a hipGraph.  It has no counterpart in the
+ * reference, which launches one kernel per codec run
+ * (src/cuda/scan/gpu_native_decode.cu:356-386, gpu_decode_strings.cu:257-366);
+ * every output is identical to sdr_decode_table / sdr_decode_strings_column.
+ * Columns, segment bytes and outputs must stay valid until sdr_batch_finish.
+ * A batch is used from one thread; its launches are ordered on the stream
+ * of the previous launch. */
+typedef struct sdr_batch sdr_batch;
+
+typedef enum sdr_batch_option {
+  SDR_BATCH_USE_GRAPH          = 1,  /* 1 (default): replay a captured hipGraph     */
+  SDR_BATCH_GRID               = 2,  /* blocks; 0 (default): resident capacity     */
+  SDR_BATCH_UNITS_BITPACKING   = 16, /* work units per queue entry, per codec:     */
+  SDR_BATCH_UNITS_RLE          = 17, /*   2048-row groups, 16384-row RLE chunks,   */
+  SDR_BATCH_UNITS_ALP          = 18, /*   1024-row vectors, 16 KiB byte ranges     */
+  SDR_BATCH_UNITS_ALPRD        = 19,
+  SDR_BATCH_UNITS_UNCOMPRESSED = 20,
+  SDR_BATCH_UNITS_CONSTANT     = 21
+} sdr_batch_option;
+
+int sdr_batch_create(sdr_batch** out);
+int sdr_batch_set_option(sdr_batch* batch, uint32_t key, int64_t value);
+int sdr_batch_add_table(sdr_batch* batch, sdr_column* cols, uint32_t n_cols);
+int sdr_batch_add_strings(sdr_batch* batch, sdr_string_column* col);
+int sdr_batch_num_entries(const sdr_batch* batch);
+/* Upload the queue and enqueue the persistent decode on `stream`. */
+int sdr_batch_launch(sdr_batch* batch, sdr_stream_t stream);
+/* Synchronise `stream`, then write chars_used and null counts back into the
+ * added columns; reports string overflow like sdr_decode_strings_column. */
+int sdr_batch_finish(sdr_batch* batch, sdr_stream_t stream);
+/* Forget the added columns (the device arena and graph are kept). */
+int sdr_batch_clear(sdr_batch* batch);
+int sdr_batch_destroy(sdr_batch* batch);
 
 /* ---- string decode ----------------------------------------------------- */
 

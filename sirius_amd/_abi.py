@@ -38,6 +38,11 @@ CODEC_NAMES = {
 
 LAUNCH_BATCHED = 0
 LAUNCH_PER_SEGMENT = 1
+LAUNCH_PERSISTENT = 2
+
+BATCH_USE_GRAPH = 1
+BATCH_GRID = 2
+BATCH_UNITS = {BITPACKING: 16, RLE: 17, ALP: 18, ALPRD: 19, UNCOMPRESSED: 20, CONSTANT: 21}
 
 
 class Segment(C.Structure):

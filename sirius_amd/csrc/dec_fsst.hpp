@@ -1,0 +1,417 @@
+// dec_fsst.hpp — gfx950 FSST chunk decoder: VARCHAR segments to int32
+// offsets + chars.  Used by the per-column kernel (k_fsst.hip) and the
+// persistent mixed-codec kernel (k_batch.hip).
+//
+// Segment layout, header checks and symbol-table format: the reference's
+// src/cuda/scan/strings/fsst.cu:17-44, :69-87 and
+// src/include/cuda/scan/detail/fsst.cuh:50-96 (a device port of DuckDB's
+// duckdb_fsst_import).  Decoded lengths and bytes follow DuckDB's serial
+// decompressor: a code emits its symbol, an escape (255) emits the next byte
+// literally, a trailing escape is dropped (fsst.cu:273-281).
+//
+// One 256-thread workgroup decodes one chunk of up to FSST_CHUNK_ROWS rows.
+// Chunks are numbered in column row order and claimed in that order (an
+// atomic ticket or the position in a claimed work queue), so a chunk only
+// ever waits on chunks already running: each chunk publishes its
+// decoded byte count, looks back over its predecessors for the column-wide
+// exclusive prefix (a decoupled look-back), writes its rows' int32 offsets
+// and then its chars.  The compressed bytes are read from HBM once: the
+// second walk (emit) hits the L1/L2 lines the first walk (lengths) loaded.
+#pragma once
+
+#include "sdr_device.hpp"
+#include "sdr_fsst.hpp"
+
+namespace sdr {
+
+constexpr uint32_t FSST_THREADS   = 256;
+constexpr uint32_t FSST_WAVES     = FSST_THREADS / WAVE;
+constexpr uint32_t ROWS_PER_THR   = FSST_CHUNK_ROWS / FSST_THREADS;
+constexpr uint32_t FSST_PIECE     = 8192;  // LDS output staging bytes
+constexpr uint32_t FSST_ESC       = 255;
+constexpr uint32_t FSST_NSYM      = 255;
+constexpr uint64_t FSST_VERSION   = 20190218ull;
+constexpr uint64_t FSST_CORRUPT   = 32774747032022883ull;  // "corrupt\0"
+constexpr uint32_t SYMTAB_MAX     = 8192;
+constexpr uint64_t ST_AGG         = 1ull << 62;  // aggregate published
+constexpr uint64_t ST_INC         = 2ull << 62;  // inclusive prefix published
+constexpr uint64_t ST_VAL         = (1ull << 62) - 1;
+
+// Bytes of a segment read in order through aligned words; bytes at or past
+// `end` read as zero.
+__device__ __forceinline__ uint32_t byte_at(const uint8_t* p, const uint8_t* end)
+{
+  if (p >= end) return 0;
+  return *p;
+}
+
+// w-bit value at bit `bitpos` of the stream starting at `base`; bytes at or
+// past `end` read as zero (only malformed segments reach them).
+__device__ __forceinline__ uint32_t stream_bits32(const uint8_t* base, const uint8_t* end, uint64_t bitpos,
+                                                  uint32_t w)
+{
+  if (w == 0) return 0;
+  const uint8_t* p   = base + (bitpos >> 3);
+  const uint32_t off = static_cast<uint32_t>(bitpos & 7u);
+  const uint32_t nb  = (off + w + 7u) >> 3;  // bytes spanned, <= 5
+  uint64_t v = 0;
+  if (p + nb <= end) {
+    uintptr_t a = reinterpret_cast<uintptr_t>(p);
+    const uint32_t sh = static_cast<uint32_t>(a & 3u);
+    const uint32_t* wp = reinterpret_cast<const uint32_t*>(a - sh);
+    const uint32_t need = (sh + nb + 3u) >> 2;  // words spanned, <= 3
+    uint32_t w0 = wp[0], w1 = need > 1 ? wp[1] : 0u, w2 = need > 2 ? wp[2] : 0u;
+    const uint64_t lo = static_cast<uint64_t>(w0) | (static_cast<uint64_t>(w1) << 32);
+    v = (lo >> (8u * sh)) | (sh ? (static_cast<uint64_t>(w2) << (64u - 8u * sh)) : 0ull);
+  } else {
+    for (uint32_t b = 0; b < nb; ++b) v |= static_cast<uint64_t>(byte_at(p + b, end)) << (8u * b);
+  }
+  v >>= off;
+  return static_cast<uint32_t>(w >= 32u ? v : (v & ((1ull << w) - 1ull)));
+}
+
+// Sequential reader of w-bit fields (w <= 32) from an LSB-first stream,
+// loading each aligned word only when its bits are needed.
+struct BitCursor {
+  const uint32_t* w;
+  uint64_t acc;
+  uint32_t nbits;
+  __device__ __forceinline__ BitCursor(const uint8_t* base, uint64_t bitpos)
+  {
+    uintptr_t a = reinterpret_cast<uintptr_t>(base + (bitpos >> 3));
+    const uint32_t skip = static_cast<uint32_t>((a & 3u) * 8u + (bitpos & 7u));
+    w     = reinterpret_cast<const uint32_t*>(a & ~uintptr_t{3});
+    acc   = static_cast<uint64_t>(*w++) >> skip;
+    nbits = 32u - skip;
+  }
+  __device__ __forceinline__ uint32_t next(uint32_t wd)
+  {
+    if (nbits < wd) {
+      acc |= static_cast<uint64_t>(*w++) << nbits;
+      nbits += 32u;
+    }
+    const uint32_t v = static_cast<uint32_t>(wd >= 32u ? acc : (acc & ((1ull << wd) - 1ull)));
+    acc >>= wd;
+    nbits -= wd;
+    return v;
+  }
+};
+
+// Sum of the w-bit fields [i0, i1) of the stream at `base`; the fields must
+// lie inside [base, end) or the slow bounded reader is used.
+__device__ __forceinline__ uint32_t sum_fields(const uint8_t* base, const uint8_t* end, uint32_t i0, uint32_t i1,
+                                               uint32_t w)
+{
+  if (i0 >= i1 || w == 0) return 0;
+  uint32_t s = 0;
+  if (base + ((uint64_t{i1} * w + 7u) >> 3) <= end) {
+    BitCursor bc(base, uint64_t{i0} * w);
+    for (uint32_t i = i0; i < i1; ++i) s += bc.next(w);
+  } else {
+    for (uint32_t i = i0; i < i1; ++i) s += stream_bits32(base, end, uint64_t{i} * w, w);
+  }
+  return s;
+}
+
+// Sequential byte reader over [p, p + n) using aligned 4-byte loads.
+struct ByteReader {
+  const uint32_t* w;
+  uint32_t cur;
+  uint32_t boff;
+  __device__ __forceinline__ explicit ByteReader(const uint8_t* p)
+  {
+    uintptr_t a = reinterpret_cast<uintptr_t>(p);
+    boff        = static_cast<uint32_t>(a & 3u);
+    w           = reinterpret_cast<const uint32_t*>(a - boff);
+    cur         = *w;
+  }
+  __device__ __forceinline__ uint32_t next()
+  {
+    if (boff == 4u) {
+      cur  = *++w;
+      boff = 0;
+    }
+    return (cur >> (8u * boff++)) & 0xFFu;
+  }
+};
+
+// Block-wide exclusive scan of one value per thread; returns the exclusive
+// prefix and writes the block total to *total.
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* s_wave, uint32_t* total)
+{
+  const uint32_t incl = wave_incl_sum_u32(v);
+  const uint32_t w    = threadIdx.x >> 6;
+  if (lane_id() == 63) s_wave[w] = incl;
+  __syncthreads();
+  uint32_t before = 0, tot = 0;
+#pragma unroll
+  for (uint32_t k = 0; k < FSST_WAVES; ++k) {
+    const uint32_t x = s_wave[k];
+    before += k < w ? x : 0u;
+    tot += x;
+  }
+  __syncthreads();
+  *total = tot;
+  return before + incl - v;
+}
+
+
+// Shared memory of one chunk decode.
+struct FsstSmem {
+  uint8_t len[256];
+  uint32_t lo[256], hi[256];
+  uint32_t comp[FSST_CHUNK_ROWS + 1];  // segment-relative inclusive compressed prefix
+  uint32_t dec[FSST_CHUNK_ROWS + 1];   // chunk-relative exclusive decoded offset
+  __attribute__((aligned(16))) uint8_t out[FSST_PIECE];
+  uint32_t wave[FSST_WAVES];
+  uint64_t base;
+};
+
+// Decode chunk `c` (its position in the column's chunk order) with the
+// calling 256-thread workgroup.  Every thread must call it.
+__device__ __forceinline__ void fsst_chunk(const FsstChunk ch, uint32_t c, uint32_t n_chunks,
+                                           const FsstSeg* __restrict__ segs, FsstState* __restrict__ st,
+                                           int32_t* __restrict__ offsets, uint8_t* __restrict__ chars,
+                                           uint64_t chars_cap, uint32_t total_rows, uint32_t spin_limit,
+                                           FsstSmem& sm)
+{
+  const uint32_t tid = threadIdx.x;
+  const bool gap     = ch.seg == FSST_GAP;
+  const FsstSeg sg   = gap ? FsstSeg{nullptr, 0, 0} : segs[ch.seg];
+  const uint8_t* seg = sg.bytes;
+  const uint8_t* end = seg + sg.bytes_size;
+
+  // ---- header (fsst.cu:79-87) ----
+  bool ok = false;
+  uint32_t dict_end = 0, width = 0, symoff = 0;
+  if (!gap && sg.bytes_size >= 16u) {
+    dict_end = ld_u32_any(seg + 4);
+    width    = ld_u32_any(seg + 8);
+    symoff   = ld_u32_any(seg + 12);
+    ok       = dict_end <= sg.bytes_size && symoff < dict_end && width <= 32u;
+  }
+
+  // ---- symbol table import, one code per thread (detail/fsst.cuh:50-96) ----
+  if (ok) {
+    const uint8_t* blob  = seg + symoff;
+    uint32_t avail       = sg.bytes_size - symoff;
+    if (avail > SYMTAB_MAX) avail = SYMTAB_MAX;
+    const uint8_t* bend  = blob + avail;
+    uint64_t version = 0;
+    for (uint32_t i = 0; i < 8; ++i) version |= static_cast<uint64_t>(byte_at(blob + i, bend)) << (8u * i);
+    const uint32_t code = tid;
+    uint32_t len = 0;
+    uint64_t sym = 0;
+    if ((version >> 32) == FSST_VERSION && code < FSST_NSYM) {
+      const uint32_t zt = byte_at(blob + 8, bend) & 1u;
+      uint32_t next_code = zt, pos = 17, found = 0;
+      len = 8;
+      sym = FSST_CORRUPT;
+      if (zt && code == 0) {
+        len = 1;
+        sym = 0;
+        found = 1;
+      }
+      // groups in code order: lengths 2,3,...,8 then 1
+      for (uint32_t l = 1; l <= 8 && !found; ++l) {
+        const uint32_t h  = l & 7u;
+        const uint32_t sl = h + 1u;
+        uint32_t cnt = byte_at(blob + 9 + h, bend);
+        if (h == 0 && zt) cnt = (cnt - 1u) & 0xFFu;
+        if (code >= next_code && code < next_code + cnt) {
+          const uint32_t at = pos + (code - next_code) * sl;
+          sym = 0;
+          for (uint32_t j = 0; j < sl; ++j) sym |= static_cast<uint64_t>(byte_at(blob + at + j, bend)) << (8u * j);
+          len = sl;
+          found = 1;
+        }
+        next_code += cnt;
+        pos += cnt * sl;
+      }
+    }
+    sm.len[tid] = static_cast<uint8_t>(len);
+    sm.lo[tid]  = static_cast<uint32_t>(sym);
+    sm.hi[tid]  = static_cast<uint32_t>(sym >> 32);
+  } else {
+    sm.len[tid] = 0;
+    sm.lo[tid]  = 0;
+    sm.hi[tid]  = 0;
+  }
+
+  // ---- compressed-length prefix (fsst.cu:132-188) ----
+  const uint32_t rows = ch.rows;
+  const uint32_t r0   = tid * ROWS_PER_THR;
+  uint32_t before = 0, total = 0;
+  {
+    // sum of the segment's lengths before this chunk
+    uint32_t part = 0;
+    if (ok && ch.row_begin) {
+      const uint32_t q  = (ch.row_begin + FSST_THREADS - 1) / FSST_THREADS;
+      const uint32_t i0 = tid * q;
+      const uint32_t i1 = i0 + q < ch.row_begin ? i0 + q : ch.row_begin;
+      part = sum_fields(seg + 16, end, i0, i1, width);
+    }
+    block_excl_scan(part, sm.wave, &before);
+  }
+  uint32_t clen[ROWS_PER_THR];
+  uint32_t acc = 0;
+  {
+    const uint32_t mine = r0 < rows ? (rows - r0 < ROWS_PER_THR ? rows - r0 : ROWS_PER_THR) : 0u;
+    const uint32_t i0 = ch.row_begin + r0;
+    const bool fast = ok && mine && width &&
+                      seg + 16 + ((uint64_t{i0 + mine} * width + 7u) >> 3) <= end;
+    if (fast) {
+      BitCursor bc(seg + 16, uint64_t{i0} * width);
+#pragma unroll
+      for (uint32_t k = 0; k < ROWS_PER_THR; ++k) clen[k] = k < mine ? bc.next(width) : 0u;
+    } else {
+#pragma unroll
+      for (uint32_t k = 0; k < ROWS_PER_THR; ++k)
+        clen[k] = (ok && k < mine) ? stream_bits32(seg + 16, end, uint64_t{i0 + k} * width, width) : 0u;
+    }
+#pragma unroll
+    for (uint32_t k = 0; k < ROWS_PER_THR; ++k) acc += clen[k];
+  }
+  uint32_t excl = block_excl_scan(acc, sm.wave, &total);
+  if (tid == 0) sm.comp[0] = before;
+  {
+    uint32_t run = before + excl;
+#pragma unroll
+    for (uint32_t k = 0; k < ROWS_PER_THR; ++k) {
+      run += clen[k];
+      if (r0 + k < rows) sm.comp[r0 + k + 1] = run;
+    }
+  }
+  __syncthreads();
+
+  // ---- decoded lengths (fsst.cu:194-286, serial escape semantics) ----
+  uint32_t dlen[ROWS_PER_THR];
+  acc = 0;
+#pragma unroll
+  for (uint32_t k = 0; k < ROWS_PER_THR; ++k) {
+    const uint32_t r = r0 + k;
+    uint32_t d = 0;
+    if (ok && r < rows) {
+      const uint32_t cum = sm.comp[r + 1], prev = sm.comp[r];
+      if (cum <= dict_end && prev <= cum && cum != prev) {
+        const uint32_t n = cum - prev;
+        ByteReader br(seg + dict_end - cum);
+        bool esc = false;
+        for (uint32_t i = 0; i < n; ++i) {
+          const uint32_t b = br.next();
+          if (esc) {
+            ++d;
+            esc = false;
+          } else if (b == FSST_ESC) {
+            esc = true;
+          } else {
+            d += sm.len[b];
+          }
+        }
+      }
+    }
+    dlen[k] = d;
+    acc += d;
+  }
+  uint32_t T = 0;
+  excl = block_excl_scan(acc, sm.wave, &T);
+  {
+    uint32_t run = excl;
+#pragma unroll
+    for (uint32_t k = 0; k < ROWS_PER_THR; ++k) {
+      if (r0 + k < rows) sm.dec[r0 + k] = run;
+      run += dlen[k];
+    }
+  }
+  if (tid == 0) sm.dec[rows] = T;
+
+  // ---- decoupled look-back for the column-wide chars offset ----
+  if (tid == 0) {
+    uint64_t* status = st->status;
+    uint64_t excl64  = 0;
+    if (c == 0) {
+      __hip_atomic_store(&status[0], ST_INC | T, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+      __hip_atomic_store(&status[c], ST_AGG | T, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+      uint32_t p = c - 1, spins = 0;
+      for (;;) {
+        const uint64_t s = __hip_atomic_load(&status[p], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+        const uint64_t flag = s & ~ST_VAL;
+        if (flag == 0) {
+          if (++spins > spin_limit) {  // never expected: give up rather than hang
+            atomicOr(&st->error, 1u);
+            break;
+          }
+          __builtin_amdgcn_s_sleep(1);
+          continue;
+        }
+        excl64 += s & ST_VAL;
+        if (flag == ST_INC || p == 0) break;
+        --p;
+      }
+      __hip_atomic_store(&status[c], ST_INC | ((excl64 + T) & ST_VAL), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    sm.base = excl64;
+    if (c == n_chunks - 1) st->total = excl64 + T;
+  }
+  __syncthreads();
+  const uint64_t base = sm.base;
+
+  // ---- offsets ----
+  for (uint32_t i = tid; i < rows; i += FSST_THREADS)
+    offsets[ch.col_row + i] = static_cast<int32_t>(static_cast<uint32_t>(base + sm.dec[i]));
+  if (c == n_chunks - 1 && tid == 0) offsets[total_rows] = static_cast<int32_t>(static_cast<uint32_t>(base + T));
+  if (T == 0 || base >= chars_cap) return;
+
+  // ---- chars: decode into LDS pieces aligned to the absolute chars
+  // address, flush whole 16-byte blocks with vector stores and the partial
+  // first/last blocks byte-wise (they share 16 bytes with the neighbours) ----
+  const uintptr_t cb = reinterpret_cast<uintptr_t>(chars);
+  const uintptr_t g0 = cb + base;
+  const uintptr_t g1 = cb + ((base + T < chars_cap) ? base + T : chars_cap);
+  for (uintptr_t pa = g0 & ~uintptr_t{15}; pa < g1; pa += FSST_PIECE) {
+    const uintptr_t pb = pa + FSST_PIECE;
+#pragma unroll
+    for (uint32_t k = 0; k < ROWS_PER_THR; ++k) {
+      const uint32_t r = r0 + k;
+      if (r >= rows || dlen[k] == 0) continue;
+      const uintptr_t o0 = g0 + sm.dec[r];
+      if (o0 + dlen[k] <= pa || o0 >= pb) continue;
+      const uint32_t cum = sm.comp[r + 1], prev = sm.comp[r];
+      const uint32_t n   = cum - prev;
+      ByteReader br(seg + dict_end - cum);
+      uintptr_t o = o0;
+      bool esc    = false;
+      for (uint32_t i = 0; i < n && o < pb; ++i) {
+        const uint32_t b = br.next();
+        if (esc) {
+          if (o >= pa) sm.out[o - pa] = static_cast<uint8_t>(b);
+          ++o;
+          esc = false;
+        } else if (b == FSST_ESC) {
+          esc = true;
+        } else {
+          const uint32_t l   = sm.len[b];
+          const uint64_t sym = static_cast<uint64_t>(sm.lo[b]) | (static_cast<uint64_t>(sm.hi[b]) << 32);
+          for (uint32_t j = 0; j < l; ++j, ++o)
+            if (o >= pa && o < pb) sm.out[o - pa] = static_cast<uint8_t>(sym >> (8u * j));
+        }
+      }
+    }
+    __syncthreads();
+    const uintptr_t lo = pa > g0 ? pa : g0;
+    const uintptr_t hi = pb < g1 ? pb : g1;
+    for (uintptr_t blk = pa + uintptr_t{tid} * 16u; blk < hi; blk += uintptr_t{FSST_THREADS} * 16u) {
+      if (blk >= lo && blk + 16u <= hi) {
+        const u32x4 v = *reinterpret_cast<const u32x4*>(&sm.out[blk - pa]);
+        __builtin_nontemporal_store(v, reinterpret_cast<u32x4*>(blk));
+      } else {
+        for (uintptr_t x = blk; x < blk + 16u; ++x)
+          if (x >= lo && x < hi) *reinterpret_cast<uint8_t*>(x) = sm.out[x - pa];
+      }
+    }
+    __syncthreads();
+  }
+}
+
+}  // namespace sdr

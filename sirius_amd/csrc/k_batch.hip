@@ -1,0 +1,123 @@
+// k_batch.hip — persistent mixed-codec decode kernel.
+//
+// One launch decodes a whole device-resident work queue (sdr_batch.hpp):
+// BITPACKING groups, RLE row chunks, ALP/ALPRD vectors, UNCOMPRESSED and
+// CONSTANT byte ranges and FSST chunks, from any number of columns and
+// segments.  A grid sized to the resident capacity of the GPU loops: a block
+// claims the next queue entry with one atomic, its four wavefronts decode the
+// entry's units (or, for FSST, the block decodes the chunk together), and it
+// claims again until the queue is drained.  Claiming in queue order is what
+// lets FSST chunks of one column wait on their predecessors' offsets: a
+// predecessor was claimed earlier by a running block.
+#include "dec_fixed.hpp"
+#include "dec_fsst.hpp"
+#include "sdr_batch.hpp"
+
+namespace sdr {
+
+constexpr uint32_t BATCH_SPIN_LIMIT = FSST_SPIN_LIMIT;
+
+union __attribute__((aligned(16))) BatchSmem {
+  uint32_t fixed[WAVES_PER_BLOCK][ALP_STAGE_WORDS];
+  unsigned long long rle_mask[WAVES_PER_BLOCK];
+  FsstSmem fsst;
+};
+
+template <typename T>
+__device__ __forceinline__ void fixed_units(const BatchItem& it, uint32_t w, BatchSmem& sm)
+{
+  const uint32_t end = it.unit0 + it.n_units;
+  for (uint32_t u = it.unit0 + w; u < end; u += WAVES_PER_BLOCK) {
+    if (it.kind == BK_BITPACKING) {
+      const uint32_t grc = it.row_count - u * BP_GROUP < BP_GROUP ? it.row_count - u * BP_GROUP : BP_GROUP;
+      bp_group<T>(it.seg, it.bytes_size, u, grc, reinterpret_cast<T*>(it.out) + size_t{u} * BP_GROUP, sm.fixed[w]);
+    } else if (it.kind == BK_RLE) {
+      const uint32_t ra = u * RLE_UNIT_ROWS;
+      const uint32_t rb = it.row_count - ra < RLE_UNIT_ROWS ? it.row_count : ra + RLE_UNIT_ROWS;
+      rle_unit<T>(it.seg, it.bytes_size, it.row_count, ra, rb, reinterpret_cast<T*>(it.out), &sm.rle_mask[w]);
+    } else if constexpr (sizeof(T) >= 4) {
+      const uint32_t fill = it.row_count - u * ALP_VEC < ALP_VEC ? it.row_count - u * ALP_VEC : ALP_VEC;
+      using F = std::conditional_t<sizeof(T) == 8, double, float>;
+      F* out = reinterpret_cast<F*>(it.out) + size_t{u} * ALP_VEC;
+      if (it.kind == BK_ALP) alp_vector<F>(it.seg, it.bytes_size, u, fill, out, sm.fixed[w]);
+      else alprd_vector<F>(it.seg, it.bytes_size, u, fill, out, sm.fixed[w]);
+    }
+  }
+}
+
+__global__ __launch_bounds__(BLOCK_THREADS) void k_batch(const BatchItem* __restrict__ items, BatchQueue* __restrict__ q)
+{
+  __shared__ BatchSmem sm;
+  __shared__ uint32_t s_idx;
+  const uint32_t n = q->n_items;
+  const uint32_t w = wave_in_block();
+  for (;;) {
+    if (threadIdx.x == 0) s_idx = atomicAdd(&q->claimed, 1u);
+    __syncthreads();
+    const uint32_t idx = s_idx;
+    __syncthreads();
+    if (idx >= n) break;
+    const BatchItem it = items[idx];
+    switch (it.kind) {
+      case BK_FSST: {
+        const FsstColumn col = *it.column;
+        fsst_chunk(col.chunks[it.chunk], it.chunk, col.n_chunks, col.segs, col.state, col.offsets, col.chars,
+                   col.chars_cap, col.total_rows, BATCH_SPIN_LIMIT, sm.fsst);
+        break;
+      }
+      case BK_COPY:
+      case BK_CONSTANT: {
+        const uint64_t total = uint64_t{it.row_count} * it.ts;
+        for (uint32_t u = it.unit0 + w; u < it.unit0 + it.n_units; u += WAVES_PER_BLOCK) {
+          const uint64_t b0 = uint64_t{u} * COPY_UNIT_BYTES;
+          const uint64_t b1 = total - b0 < COPY_UNIT_BYTES ? total : b0 + COPY_UNIT_BYTES;
+          if (it.kind == BK_COPY) copy_unit(it.seg, it.out, b0, b1);
+          else const_unit(it.seg, it.ts, it.out, b0, b1);
+        }
+        break;
+      }
+      default:
+        switch (it.ts) {
+          case 1: fixed_units<uint8_t>(it, w, sm); break;
+          case 2: fixed_units<uint16_t>(it, w, sm); break;
+          case 4: fixed_units<uint32_t>(it, w, sm); break;
+          default: fixed_units<uint64_t>(it, w, sm); break;
+        }
+        break;
+    }
+    __syncthreads();  // LDS reuse by the next entry
+  }
+}
+
+__global__ void k_batch_reset(BatchQueue* q)
+{
+  const uint32_t n = q->n_fsst_status;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) q->fsst_status[i] = 0;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    q->claimed = 0;
+    q->error   = 0;
+  }
+}
+
+hipError_t launch_batch_reset(BatchQueue* q, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_batch_reset, dim3(64), dim3(256), 0, s, q);
+  return hipGetLastError();
+}
+
+hipError_t launch_batch(const BatchItem* items, BatchQueue* q, uint32_t grid, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_batch, dim3(grid), dim3(BLOCK_THREADS), 0, s, items, q);
+  return hipGetLastError();
+}
+
+int batch_occupancy_blocks_per_cu()
+{
+  int blocks = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, reinterpret_cast<const void*>(k_batch), BLOCK_THREADS, 0) !=
+      hipSuccess)
+    return 1;
+  return blocks > 0 ? blocks : 1;
+}
+
+}  // namespace sdr

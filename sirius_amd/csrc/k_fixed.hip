@@ -1,182 +1,10 @@
 // k_fixed.hip — gfx950 kernels for the fixed-width codecs: BITPACKING, RLE,
-// ALP, ALPRD, UNCOMPRESSED, CONSTANT and the validity mask helpers.
-//
-// Work partitioning: a block is 4 wavefronts; each wavefront independently
-// decodes one work unit of one segment (blockIdx.y selects the segment,
-// blockIdx.x * 4 + wave selects the unit):
-//   BITPACKING  unit = one 2048-row metadata group
-//   ALP/ALPRD   unit = one 1024-row vector
-//   RLE         unit = RLE_UNIT_ROWS rows
-// Metadata is parsed by every lane from the same addresses, so it lives in
-// scalar registers; no block barrier is ever needed.  Malformed metadata
-// zero-fills the unit's trusted row range, as the reference kernels do.
-#include "sdr_device.hpp"
-#include "sdr_kernels.hpp"
-
-#include <type_traits>
+// ALP, ALPRD, UNCOMPRESSED, CONSTANT and the validity mask helpers.  One
+// launch covers a batch of segments: blockIdx.y selects the segment,
+// blockIdx.x * 4 + wave the work unit (decoders in dec_fixed.hpp).
+#include "dec_fixed.hpp"
 
 namespace sdr {
-
-constexpr uint32_t WAVES_PER_BLOCK = 4;
-constexpr uint32_t BLOCK_THREADS   = WAVES_PER_BLOCK * WAVE;
-// Per-wave LDS: 8 KiB of packed bits plus the 16-byte head skew, tail block
-// and two guard words read by the straddle extraction.
-constexpr uint32_t STAGE_BYTES = 8192;
-constexpr uint32_t STAGE_WORDS = (STAGE_BYTES + 48) / 4;
-
-__device__ __forceinline__ uint32_t wave_in_block()
-{
-  return __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-}
-
-template <typename T>
-using acc_t = std::conditional_t<sizeof(T) == 8, uint64_t, uint32_t>;
-
-template <typename T>
-__device__ __forceinline__ acc_t<T> wave_incl_sum(acc_t<T> v)
-{
-  if constexpr (sizeof(T) == 8) return wave_incl_sum_u64(v);
-  else return wave_incl_sum_u32(v);
-}
-
-template <typename T>
-__device__ __forceinline__ acc_t<T> readlane_acc(acc_t<T> v, int l)
-{
-  if constexpr (sizeof(T) == 8) return readlane_u64(v, l);
-  else return __builtin_amdgcn_readlane(v, l);
-}
-
-// Fill rows [0, n) of `out` with gen(row).
-template <typename T, typename F>
-__device__ __forceinline__ void wave_fill(T* out, uint32_t n, F gen)
-{
-  constexpr uint32_t VPL = 16 / sizeof(T);
-  const bool al = (reinterpret_cast<uintptr_t>(out) & 15u) == 0;
-  for (uint32_t r = lane_id() * VPL; r < n; r += WAVE * VPL) {
-    T v[VPL];
-#pragma unroll
-    for (uint32_t i = 0; i < VPL; ++i) v[i] = gen(r + i);
-    store_vec<T>(out + r, v, n - r, al);
-  }
-}
-
-// ===========================================================================
-// BITPACKING
-// Layout and parse rules: the reference's gpu_decode_bitpacking.cu:17-42 and
-// :119-213; modes :222-348.
-// ===========================================================================
-enum : uint32_t { BP_INVALID = 0, BP_CONSTANT = 2, BP_CONSTANT_DELTA = 3, BP_DELTA_FOR = 4, BP_FOR = 5 };
-constexpr uint32_t BP_GROUP = 2048;
-
-template <typename T>
-__device__ void bp_group(const uint8_t* seg, uint32_t bytes, uint32_t k, uint32_t grc, T* out,
-                         uint32_t* lds)
-{
-  constexpr uint32_t TS  = sizeof(T);
-  constexpr uint32_t VPL = 16 / TS;
-  using A                = acc_t<T>;
-
-  uint32_t mode = BP_INVALID, width = 0, rc = 0;
-  uint64_t frame = 0, aux = 0, doff = 0;
-  const uint64_t metadata_end = bytes >= 8 ? ld_u64_any(seg) : 0;
-  if (metadata_end >= 8u + (uint64_t{k} + 1u) * 4u && metadata_end <= bytes) {
-    const uint32_t enc      = ld_u32_any(seg + metadata_end - (uint64_t{k} + 1u) * 4u);
-    const uint32_t data_off = enc & 0x00FFFFFFu;
-    const uint32_t pm       = enc >> 24;
-    rc                      = grc;
-    if (uint64_t{data_off} + 2u * TS <= metadata_end) {
-      const uint64_t v0 = ld_t_any<TS>(seg + data_off);
-      const uint64_t v1 = ld_t_any<TS>(seg + data_off + TS);
-      if (pm == BP_CONSTANT) {
-        mode = pm;
-        aux  = v0;
-      } else if (pm == BP_CONSTANT_DELTA) {
-        mode  = pm;
-        frame = v0;
-        aux   = v1;
-      } else if (pm == BP_FOR) {
-        mode  = pm;
-        frame = v0;
-        width = static_cast<uint32_t>(v1);
-        doff  = uint64_t{data_off} + 2u * TS;
-      } else if (pm == BP_DELTA_FOR && uint64_t{data_off} + 3u * TS <= metadata_end) {
-        mode  = pm;
-        frame = v0;
-        width = static_cast<uint32_t>(v1);
-        aux   = ld_t_any<TS>(seg + data_off + 2u * TS);
-        doff  = uint64_t{data_off} + 3u * TS;
-      }
-      if (width > TS * 8u) {
-        mode  = BP_INVALID;
-        width = 0;
-      }
-      if (mode == BP_FOR || mode == BP_DELTA_FOR) {
-        const uint64_t words = (uint64_t{rc} * width + 31u) / 32u;
-        if (doff + words * 4u > metadata_end) {
-          mode  = BP_INVALID;
-          width = 0;
-        }
-      }
-    }
-  }
-
-  if (mode == BP_CONSTANT) {
-    const T c = static_cast<T>(aux);
-    wave_fill<T>(out, rc, [c](uint32_t) { return c; });
-    return;
-  }
-  if (mode == BP_CONSTANT_DELTA) {
-    const A f = static_cast<A>(frame), d = static_cast<A>(aux);
-    wave_fill<T>(out, rc, [f, d](uint32_t i) { return static_cast<T>(f + static_cast<A>(i) * d); });
-    return;
-  }
-  if (mode != BP_FOR && mode != BP_DELTA_FOR) {
-    wave_fill<T>(out, grc, [](uint32_t) { return T(0); });
-    return;
-  }
-
-  // FOR / DELTA_FOR: stage packed words through LDS in chunks of at most
-  // 8 KiB of bits (2048 rows unless an 8-byte type needs > 32 bits).
-  const bool al          = (reinterpret_cast<uintptr_t>(out) & 15u) == 0;
-  const uint32_t chunk   = (width * BP_GROUP <= STAGE_BYTES * 8u) ? BP_GROUP : BP_GROUP / 2;
-  const T tframe         = static_cast<T>(frame);
-  const uint8_t* stream  = seg + doff;
-  A carry                = static_cast<A>(aux);  // DELTA_FOR running value
-  for (uint32_t c0 = 0; c0 < rc; c0 += chunk) {
-    const uint32_t rows = rc - c0 < chunk ? rc - c0 : chunk;
-    wave_lds_sync();
-    const uint32_t head = stage_to_lds(lds, stream + (c0 * width) / 8u, (rows * width + 7u) / 8u);
-    wave_lds_sync();
-    for (uint32_t s0 = 0; s0 < rows; s0 += WAVE * VPL) {
-      const uint32_t r = s0 + lane_id() * VPL;
-      T v[VPL];
-#pragma unroll
-      for (uint32_t i = 0; i < VPL; ++i) {
-        const uint32_t row = r + i;
-        T u                = 0;
-        if (row < rows) {
-          const uint32_t bp = head + row * width;
-          if constexpr (TS == 8) u = static_cast<T>(lds_bits64(lds, bp, width));
-          else u = static_cast<T>(lds_bits32(lds, bp, width));
-        }
-        v[i] = row < rows ? static_cast<T>(static_cast<A>(tframe) + static_cast<A>(u)) : T(0);
-      }
-      if (mode == BP_DELTA_FOR) {
-        A loc[VPL];
-        loc[0] = v[0];
-#pragma unroll
-        for (uint32_t i = 1; i < VPL; ++i) loc[i] = loc[i - 1] + static_cast<A>(v[i]);
-        const A tot  = loc[VPL - 1];
-        const A incl = wave_incl_sum<T>(tot);
-        const A base = carry + (incl - tot);
-#pragma unroll
-        for (uint32_t i = 0; i < VPL; ++i) v[i] = static_cast<T>(base + loc[i]);
-        carry += readlane_acc<T>(incl, 63);
-      }
-      if (r < rows) store_vec<T>(out + c0 + r, v, rows - r, al);
-    }
-  }
-}
 
 template <typename T>
 __global__ __launch_bounds__(BLOCK_THREADS) void k_bitpacking(const FixedSeg* __restrict__ segs,
@@ -193,256 +21,6 @@ __global__ __launch_bounds__(BLOCK_THREADS) void k_bitpacking(const FixedSeg* __
               lds[w]);
 }
 
-// ===========================================================================
-// ALP / ALPRD
-// Layouts and parse rules: the reference's gpu_decode_alp.cu:17-72,
-// :251-314 (ALP), :426-504 (ALPRD); decode :368-393 and :578-625.
-// ===========================================================================
-constexpr uint32_t ALP_VEC = 1024;
-// ALPRD stages the left stream (<= 1024 * 3 bits + head + guard) ahead of
-// the right stream (<= 8 KiB + head + guard).
-constexpr uint32_t ALPRD_LEFT_WORDS = 112;
-constexpr uint32_t ALP_STAGE_WORDS  = ALPRD_LEFT_WORDS + STAGE_WORDS;
-
-__constant__ int64_t c_alp_fact[19] = {1LL,
-                                       10LL,
-                                       100LL,
-                                       1000LL,
-                                       10000LL,
-                                       100000LL,
-                                       1000000LL,
-                                       10000000LL,
-                                       100000000LL,
-                                       1000000000LL,
-                                       10000000000LL,
-                                       100000000000LL,
-                                       1000000000000LL,
-                                       10000000000000LL,
-                                       100000000000000LL,
-                                       1000000000000000LL,
-                                       10000000000000000LL,
-                                       100000000000000000LL,
-                                       1000000000000000000LL};
-__constant__ float c_alp_frac_f32[11] = {1.0F, 1e-1F, 1e-2F, 1e-3F, 1e-4F, 1e-5F,
-                                         1e-6F, 1e-7F, 1e-8F, 1e-9F, 1e-10F};
-__constant__ double c_alp_frac_f64[21] = {1.0,   1e-1,  1e-2,  1e-3,  1e-4,  1e-5,  1e-6,
-                                          1e-7,  1e-8,  1e-9,  1e-10, 1e-11, 1e-12, 1e-13,
-                                          1e-14, 1e-15, 1e-16, 1e-17, 1e-18, 1e-19, 1e-20};
-
-__device__ __forceinline__ uint32_t bp_req_bytes(uint32_t n, uint32_t w)
-{
-  return w == 0 ? 0u : ((n + 31u) / 32u) * 32u * w / 8u;
-}
-
-// Patch exceptions after the main store, in rounds of 64 in list order.  A
-// later exception at the same position wins (the order DuckDB's serial patch
-// loop gives): inside a round a lane stores only if no higher lane of the
-// round names the same position; rounds are separated by a store fence.
-template <typename F>
-__device__ __forceinline__ void patch_exceptions(uint32_t ec, uint32_t fill, const uint8_t* pos_base,
-                                                 F store_one)
-{
-  for (uint32_t e0 = 0; e0 < ec; e0 += WAVE) {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");  // earlier stores first
-    const uint32_t m = ec - e0 < WAVE ? ec - e0 : WAVE;
-    const uint32_t e = e0 + lane_id();
-    const uint32_t pos = e < ec ? ld_u16_any(pos_base + 2u * e) : 0xFFFFFFFFu;
-    bool live = e < ec && pos < fill;
-    for (uint32_t l = 0; l < m; ++l) {
-      const uint32_t pl = __builtin_amdgcn_readlane(pos, l);
-      live = live && !(l > lane_id() && pl == pos);
-    }
-    if (live) store_one(e, pos);
-  }
-}
-
-template <typename T>
-__device__ void alp_vector(const uint8_t* seg, uint32_t bytes, uint32_t v, uint32_t fill, T* out,
-                           uint32_t* lds)
-{
-  constexpr uint32_t TS  = sizeof(T);
-  constexpr uint32_t VPL = 16 / TS;
-  constexpr uint32_t MAX_EXP = TS == 4 ? 10u : 18u;
-  const bool al = (reinterpret_cast<uintptr_t>(out) & 15u) == 0;
-
-  // ---- parse (uniform) ----
-  uint32_t status = 0;  // 0 invalid, 1 raw, 2 compressed
-  uint32_t data_off = 0, exp = 0, fac = 0, ec = 0, bw = 0, packed = 0;
-  uint64_t for_v = 0;
-  if (bytes >= 4u) {
-    const uint32_t metadata_end = ld_u32_any(seg);
-    const uint64_t need         = (uint64_t{v} + 2u) * 4u;
-    if (metadata_end <= bytes && metadata_end >= need) {
-      data_off = ld_u32_any(seg + metadata_end - (v + 1u) * 4u);
-      if (data_off >= 4u && data_off < metadata_end) {
-        exp = ld_u8(seg + data_off);
-        if (exp == 0xFFu) {
-          if (uint64_t{data_off} + 1u + uint64_t{fill} * TS <= metadata_end) status = 1;
-        } else if (uint64_t{data_off} + 13u <= metadata_end) {
-          fac   = ld_u8(seg + data_off + 1);
-          ec    = ld_u16_any(seg + data_off + 2);
-          for_v = ld_u64_any(seg + data_off + 4);
-          bw    = ld_u8(seg + data_off + 12);
-          if (bw <= TS * 8u && exp <= MAX_EXP && fac <= 18u) {
-            packed = bp_req_bytes(fill, bw);
-            const uint64_t end = uint64_t{data_off} + 13u + packed + uint64_t{ec} * (TS + 2u);
-            if (end <= metadata_end) status = 2;
-          }
-        }
-      }
-    }
-  }
-  if (status == 0) {
-    wave_fill<T>(out, fill, [](uint32_t) { return T(0); });
-    return;
-  }
-  if (status == 1) {
-    const uint8_t* raw = seg + data_off + 1;
-    for (uint32_t i = lane_id(); i < fill; i += WAVE) {
-      const uint64_t b = ld_t_any<TS>(raw + i * TS);
-      T x;
-      if constexpr (TS == 8) x = __builtin_bit_cast(T, b);
-      else x = __builtin_bit_cast(T, static_cast<uint32_t>(b));
-      st_nt(out + i, x);
-    }
-    return;
-  }
-
-  // ---- compressed ----
-  const uint8_t* pk = seg + data_off + 13u;
-  wave_lds_sync();
-  const uint32_t head = stage_to_lds(lds, pk, (fill * bw + 7u) / 8u);
-  wave_lds_sync();
-  const T fact = static_cast<T>(c_alp_fact[fac]);
-  T frac;
-  if constexpr (TS == 8) frac = c_alp_frac_f64[exp];
-  else frac = c_alp_frac_f32[exp];
-  for (uint32_t s0 = 0; s0 < fill; s0 += WAVE * VPL) {
-    const uint32_t r = s0 + lane_id() * VPL;
-    T x[VPL];
-#pragma unroll
-    for (uint32_t i = 0; i < VPL; ++i) {
-      const uint32_t row = r + i;
-      const uint64_t u   = row < fill ? lds_bits64(lds, head + row * bw, bw) : 0;
-      const int64_t enc  = static_cast<int64_t>(u + for_v);
-      // ((T)enc * (T)FACT[fac]) * FRAC[exp], left to right, no contraction
-      x[i] = static_cast<T>(enc) * fact * frac;
-    }
-    if (r < fill) store_vec<T>(out + r, x, fill - r, al);
-  }
-  if (ec) {
-    const uint8_t* exc = pk + packed;
-    const uint8_t* pos = exc + ec * TS;
-    patch_exceptions(ec, fill, pos, [&](uint32_t e, uint32_t p) {
-      const uint64_t b = ld_t_any<TS>(exc + e * TS);
-      T x;
-      if constexpr (TS == 8) x = __builtin_bit_cast(T, b);
-      else x = __builtin_bit_cast(T, static_cast<uint32_t>(b));
-      out[p] = x;
-    });
-  }
-}
-
-template <typename T>
-__device__ void alprd_vector(const uint8_t* seg, uint32_t bytes, uint32_t v, uint32_t fill, T* out,
-                             uint32_t* lds)
-{
-  constexpr uint32_t TS    = sizeof(T);
-  constexpr uint32_t VPL   = 16 / TS;
-  constexpr uint32_t TBITS = TS * 8u;
-  using R = std::conditional_t<TS == 8, uint64_t, uint32_t>;
-  const bool al = (reinterpret_cast<uintptr_t>(out) & 15u) == 0;
-
-  uint32_t status = 0, data_off = 0, rbw = 0, lbw = 0, dsz = 0, ec = 0, lpb = 0, rpb = 0;
-  if (bytes >= 7u) {
-    const uint32_t metadata_end = ld_u32_any(seg);
-    const uint64_t need         = 7u + (uint64_t{v} + 1u) * 4u;
-    if (metadata_end <= bytes && metadata_end >= need) {
-      rbw = ld_u8(seg + 4);
-      lbw = ld_u8(seg + 5);
-      dsz = ld_u8(seg + 6);
-      const uint32_t dict_end = 7u + dsz * 2u;
-      if (rbw <= TBITS && lbw <= 3u && dsz <= 8u && dict_end <= metadata_end) {
-        data_off = ld_u32_any(seg + metadata_end - (v + 1u) * 4u);
-        if (data_off >= dict_end && uint64_t{data_off} + 2u <= metadata_end) {
-          ec = ld_u16_any(seg + data_off);
-          if (ec == 0xFFFFu) {
-            if (uint64_t{data_off} + 2u + uint64_t{fill} * TS <= metadata_end) status = 1;
-          } else {
-            lpb = bp_req_bytes(fill, lbw);
-            rpb = bp_req_bytes(fill, rbw);
-            const uint64_t end = uint64_t{data_off} + 2u + lpb + rpb + uint64_t{ec} * 4u;
-            if (end <= metadata_end) status = 2;
-          }
-        }
-      }
-    }
-  }
-  if (status == 0) {
-    wave_fill<T>(out, fill, [](uint32_t) { return T(0); });
-    return;
-  }
-  if (status == 1) {
-    const uint8_t* raw = seg + data_off + 2;
-    for (uint32_t i = lane_id(); i < fill; i += WAVE) {
-      const uint64_t b = ld_t_any<TS>(raw + i * TS);
-      T x;
-      if constexpr (TS == 8) x = __builtin_bit_cast(T, b);
-      else x = __builtin_bit_cast(T, static_cast<uint32_t>(b));
-      st_nt(out + i, x);
-    }
-    return;
-  }
-  // dictionary: unused slots read as zero
-  R dict[8];
-#pragma unroll
-  for (uint32_t d = 0; d < 8; ++d) dict[d] = d < dsz ? static_cast<R>(ld_u16_any(seg + 7u + 2u * d)) : R(0);
-
-  const uint8_t* lp = seg + data_off + 2u;
-  const uint8_t* rp = lp + lpb;
-  // left stream (<= 384 B) staged at word 0, right stream (<= 8 KiB) after it
-  uint32_t* lds_l   = lds;
-  uint32_t* lds_r   = lds + ALPRD_LEFT_WORDS;
-  wave_lds_sync();
-  const uint32_t rhead = stage_to_lds(lds_r, rp, (fill * rbw + 7u) / 8u);
-  const uint32_t lhead = stage_to_lds(lds_l, lp, (fill * lbw + 7u) / 8u);
-  wave_lds_sync();
-  const bool full_right = rbw >= TBITS;
-  for (uint32_t s0 = 0; s0 < fill; s0 += WAVE * VPL) {
-    const uint32_t r = s0 + lane_id() * VPL;
-    T x[VPL];
-#pragma unroll
-    for (uint32_t i = 0; i < VPL; ++i) {
-      const uint32_t row = r + i;
-      R bits             = 0;
-      if (row < fill) {
-        const R right = static_cast<R>(lds_bits64(lds_r, rhead + row * rbw, rbw));
-        if (full_right) {
-          bits = right;
-        } else {
-          const uint32_t li = lds_bits32(lds_l, lhead + row * lbw, lbw);
-          R dv              = dict[0];
-#pragma unroll
-          for (uint32_t d = 1; d < 8; ++d) dv = li == d ? dict[d] : dv;
-          bits = static_cast<R>(dv << rbw) | right;
-        }
-      }
-      x[i] = __builtin_bit_cast(T, bits);
-    }
-    if (r < fill) store_vec<T>(out + r, x, fill - r, al);
-  }
-  if (ec) {
-    const uint8_t* exc = rp + rpb;
-    const uint8_t* pos = exc + ec * 2u;
-    patch_exceptions(ec, fill, pos, [&](uint32_t e, uint32_t p) {
-      const R right = static_cast<R>(lds_bits64(lds_r, rhead + p * rbw, rbw));
-      const R bits  = full_right ? right
-                                 : static_cast<R>(static_cast<R>(ld_u16_any(exc + 2u * e)) << rbw) | right;
-      out[p]        = __builtin_bit_cast(T, bits);
-    });
-  }
-}
-
 template <typename T, bool RD>
 __global__ __launch_bounds__(BLOCK_THREADS) void k_alp(const FixedSeg* __restrict__ segs, uint32_t seg_base)
 {
@@ -456,173 +34,6 @@ __global__ __launch_bounds__(BLOCK_THREADS) void k_alp(const FixedSeg* __restric
   T* out = reinterpret_cast<T*>(sd.out) + size_t{v} * ALP_VEC;
   if constexpr (RD) alprd_vector<T>(sd.bytes, sd.bytes_size, v, fill, out, lds[w]);
   else alp_vector<T>(sd.bytes, sd.bytes_size, v, fill, out, lds[w]);
-}
-
-// ===========================================================================
-// RLE
-// Layout, header parse and count walk: the reference's gpu_decode_rle.cu:
-// 17-36, :140-171, :255-316 (first exact match of the running count wins; a
-// zero count or an overshoot in a 2048-entry tile with no match is
-// malformed; the walk stops at 90112 entries).  Expansion of a unit's rows
-// runs over windows of 64 runs: a DPP scan gives each run's end row, run
-// ends inside a 64-row step become bits of a 64-bit mask, and each row's run
-// is the ballot count of ends at or before the step plus the popcount of
-// mask bits at or below the row.
-// ===========================================================================
-constexpr uint32_t RLE_TILE        = 2048;
-constexpr uint32_t RLE_MAX_ENTRIES = 90112;
-
-template <typename T>
-__device__ void rle_unit(const uint8_t* seg, uint32_t bytes, uint32_t rows, uint32_t ra, uint32_t rb, T* out,
-                         unsigned long long* lds_mask)
-{
-  constexpr uint32_t TS = sizeof(T);
-  T* uout = out + ra;
-  const uint32_t n = rb - ra;
-
-  bool malformed = true;
-  uint64_t counts_off = 0;
-  uint32_t n_max = 0;
-  if (bytes >= 8u) {
-    counts_off = ld_u64_any(seg);
-    if (counts_off >= 8u && counts_off < bytes) {
-      const uint64_t vb = counts_off - 8u, cb = bytes - counts_off;
-      uint64_t n_real   = (vb % TS == 0) ? vb / TS : cb / 2u;
-      if (n_real > RLE_MAX_ENTRIES) n_real = RLE_MAX_ENTRIES;
-      if (n_real > cb / 2u) n_real = cb / 2u;
-      n_max     = static_cast<uint32_t>(n_real);
-      malformed = false;
-    }
-  }
-  const uint8_t* counts = seg + counts_off;
-
-  // ---- count walk: validate, find ec and the run containing row ra ----
-  uint32_t ec = 0, ja = 0, ja_start = 0;
-  bool have_ja = false;
-  if (!malformed) {
-    malformed     = true;
-    uint32_t run  = 0;   // running sum before this sub-step
-    bool tile_bad = false;
-    for (uint32_t e0 = 0; e0 < n_max; e0 += WAVE * 8u) {
-      const uint32_t i0 = e0 + lane_id() * 8u;
-      uint32_t c[8], L[8];
-      uint32_t acc = 0;
-      bool zero = false;
-#pragma unroll
-      for (uint32_t k = 0; k < 8; ++k) {
-        const uint32_t i = i0 + k;
-        c[k] = i < n_max ? ld_u16_any(counts + 2u * i) : 0u;
-        zero |= (i < n_max) && c[k] == 0;
-        acc += c[k];
-        L[k] = acc;
-      }
-      const uint32_t incl = wave_incl_sum_u32(acc);
-      const uint32_t base = run + incl - acc;
-      // first entry whose prefix equals rows (match) / exceeds ra (ja)
-      uint32_t mk = 8, jk = 8;
-      bool over = false;
-#pragma unroll
-      for (uint32_t k = 0; k < 8; ++k) {
-        const uint32_t i = i0 + k;
-        const uint32_t p = base + L[k];
-        if (i < n_max) {
-          if (mk == 8 && p == rows) mk = k;
-          if (mk == 8 && p > rows) over = true;
-          if (jk == 8 && p > ra) jk = k;
-        }
-      }
-      if (!have_ja) {
-        const uint64_t bj = ballot(jk < 8);
-        if (bj) {
-          const int l      = __builtin_ctzll(bj);
-          const uint32_t k = __builtin_amdgcn_readlane(jk, l);
-          uint32_t pstart  = 0;
-#pragma unroll
-          for (uint32_t q = 0; q < 8; ++q) pstart = (q == jk) ? base + L[q] - c[q] : pstart;
-          ja       = e0 + static_cast<uint32_t>(l) * 8u + k;
-          ja_start = __builtin_amdgcn_readlane(pstart, l);
-          have_ja  = true;
-        }
-      }
-      const uint64_t bm = ballot(mk < 8);
-      // zero/overshoot only count at entries before the match in this sub-step,
-      // but a match anywhere in the 2048-entry tile excuses the whole tile
-      tile_bad |= ballot(zero) != 0 || ballot(over) != 0;
-      if (bm) {
-        const int l = __builtin_ctzll(bm);
-        ec          = e0 + static_cast<uint32_t>(l) * 8u + __builtin_amdgcn_readlane(mk, l) + 1u;
-        malformed   = false;
-        break;
-      }
-      run += __builtin_amdgcn_readlane(incl, 63);
-      const bool tile_end = ((e0 + WAVE * 8u) % RLE_TILE) == 0 || e0 + WAVE * 8u >= n_max;
-      if (tile_end) {
-        if (tile_bad) break;
-        tile_bad = false;
-      }
-    }
-    if (!malformed && 8u + uint64_t{ec} * TS > bytes) malformed = true;
-  }
-  if (malformed || !have_ja) {
-    wave_fill<T>(uout, n, [](uint32_t) { return T(0); });
-    return;
-  }
-
-  // ---- expansion ----
-  const uint8_t* values = seg + 8u;
-  uint32_t j      = ja;        // first run of this window
-  uint32_t wstart = ja_start;  // first row of run j
-  while (wstart < rb && j < ec) {
-    const uint32_t jl  = j + lane_id();
-    const bool live    = jl < ec;
-    const uint32_t cnt = live ? ld_u16_any(counts + 2u * jl) : 0u;
-    const uint64_t val = live ? ld_t_any<TS>(values + uint64_t{jl} * TS) : 0u;
-    const uint32_t incl = wave_incl_sum_u32(cnt);
-    const uint32_t end  = wstart + incl;                         // exclusive end row of run jl
-    const uint32_t wend = wstart + __builtin_amdgcn_readlane(incl, 63);
-    const bool dup      = ballot(live && cnt == 0) != 0;         // zero-length runs in window
-    const uint32_t lo   = wstart > ra ? wstart : ra;
-    const uint32_t hi   = wend < rb ? wend : rb;
-    // steps are aligned to the unit start so lanes write consecutive rows
-    for (uint32_t r0 = ra + ((lo - ra) & ~(WAVE - 1u)); r0 < hi; r0 += WAVE) {
-      const uint32_t row = r0 + lane_id();
-      uint32_t rank;
-      if (!dup) {
-        const uint32_t pos = end - r0;  // run ends inside (r0, r0 + 64) become bits
-        const unsigned long long bit =
-          (end > r0 && pos < WAVE && live) ? (1ull << pos) : 0ull;
-        if (lane_id() == 0) *lds_mask = 0;
-        wave_lds_sync();
-        if (bit) atomicOr(lds_mask, bit);
-        wave_lds_sync();
-        const unsigned long long m = *lds_mask;
-        wave_lds_sync();
-        const uint32_t before = static_cast<uint32_t>(__builtin_popcountll(ballot(live && end <= r0)));
-        const unsigned long long upto =
-          lane_id() == 63 ? ~0ull : ((1ull << (lane_id() + 1u)) - 1ull);
-        rank = before + static_cast<uint32_t>(__builtin_popcountll(m & upto));
-      } else {
-        rank = 0;
-        for (uint32_t l = 0; l < WAVE; ++l) {
-          const uint32_t el = __builtin_amdgcn_readlane(end, l);
-          const bool ll     = j + l < ec;
-          rank += (ll && el <= row) ? 1u : 0u;
-        }
-      }
-      const uint32_t src = rank < WAVE ? rank : WAVE - 1u;
-      uint64_t x;
-      if constexpr (TS == 8) {
-        const uint32_t xl = __builtin_amdgcn_ds_bpermute(static_cast<int>(src << 2), static_cast<int>(static_cast<uint32_t>(val)));
-        const uint32_t xh = __builtin_amdgcn_ds_bpermute(static_cast<int>(src << 2), static_cast<int>(static_cast<uint32_t>(val >> 32)));
-        x = static_cast<uint64_t>(xl) | (static_cast<uint64_t>(xh) << 32);
-      } else {
-        x = static_cast<uint32_t>(__builtin_amdgcn_ds_bpermute(static_cast<int>(src << 2), static_cast<int>(static_cast<uint32_t>(val))));
-      }
-      if (row >= lo && row < hi) st_nt(out + row, static_cast<T>(x));
-    }
-    j += WAVE;
-    wstart = wend;
-  }
 }
 
 template <typename T>

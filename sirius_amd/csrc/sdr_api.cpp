@@ -138,9 +138,7 @@ int build_null_mask(const sdr_run* runs, uint32_t n, uint32_t total_rows, uint8_
   return SDR_OK;
 }
 
-namespace {
-
-int validate_columns(const sdr_column* cols, uint32_t n_cols)
+int validate_fixed_columns(const sdr_column* cols, uint32_t n_cols)
 {
   if (n_cols && !cols) return fail(SDR_E_INVALID_ARG, "cols is NULL");
   for (uint32_t c = 0; c < n_cols; ++c) {
@@ -179,6 +177,8 @@ int validate_columns(const sdr_column* cols, uint32_t n_cols)
   }
   return SDR_OK;
 }
+
+namespace {
 
 hipError_t launch_batch(const Batch& b, const FixedSeg* d, uint32_t base, uint32_t n, uint32_t max_rows,
                         hipStream_t s)
@@ -267,11 +267,12 @@ int decode_runs(const std::vector<RunTarget>& targets, hipStream_t s, uint32_t m
 int decode_table(sdr_column* cols, uint32_t n_cols, hipStream_t s, uint32_t mode)
 {
   clear_error();
-  if (mode != SDR_LAUNCH_BATCHED && mode != SDR_LAUNCH_PER_SEGMENT)
+  if (mode != SDR_LAUNCH_BATCHED && mode != SDR_LAUNCH_PER_SEGMENT && mode != SDR_LAUNCH_PERSISTENT)
     return fail(SDR_E_INVALID_ARG, "unknown launch mode %u", mode);
-  int rc = validate_columns(cols, n_cols);
+  int rc = validate_fixed_columns(cols, n_cols);
   if (rc) return rc;
   if (n_cols == 0) return SDR_OK;
+  if (mode == SDR_LAUNCH_PERSISTENT) return decode_table_persistent(cols, n_cols, s);
   std::vector<RunTarget> targets;
   for (uint32_t c = 0; c < n_cols; ++c) {
     cols[c].null_count = 0;
@@ -335,7 +336,7 @@ int plan_create(const sdr_column* cols, uint32_t n_cols, uint32_t mode, hipStrea
   *out = nullptr;
   if (mode != SDR_LAUNCH_BATCHED && mode != SDR_LAUNCH_PER_SEGMENT)
     return fail(SDR_E_INVALID_ARG, "unknown launch mode %u", mode);
-  int rc = validate_columns(cols, n_cols);
+  int rc = validate_fixed_columns(cols, n_cols);
   if (rc) return rc;
   auto* p = new sdr_plan();
   p->mode = mode;

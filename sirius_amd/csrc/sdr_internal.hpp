@@ -38,6 +38,11 @@ struct RunTarget {
   uint32_t type_size;
 };
 int decode_runs(const std::vector<RunTarget>& targets, hipStream_t s, uint32_t mode);
+int validate_fixed_columns(const sdr_column* cols, uint32_t n_cols);
+int validate_string_column(const sdr_string_column* col);
+// SDR_LAUNCH_PERSISTENT: one persistent launch through a thread-local batch.
+int decode_table_persistent(sdr_column* cols, uint32_t n_cols, hipStream_t s);
+int decode_strings_persistent(sdr_string_column* col, hipStream_t s);
 
 // Segments grouped by (codec, type size); [first, first + count) in the
 // descriptor array.

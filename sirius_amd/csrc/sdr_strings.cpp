@@ -1,7 +1,142 @@
 // sdr_strings.cpp — C ABI of the VARCHAR decode path (FSST).
+//
+// Mirrors gpu_decode_strings_column (src/cuda/scan/gpu_decode_strings.cu:
+// 100-402): per-run validation, one column-wide int32 offset space, chars
+// sized by the caller, validity overlay.  The reference's length pass, CUB
+// scan and gather pass are one kernel here (k_fsst.hip).
+#include "sdr_fsst.hpp"
 #include "sdr_internal.hpp"
 
+#include <algorithm>
 #include <cstdint>
+#include <vector>
+
+namespace sdr {
+
+int validate_string_column(const sdr_string_column* col)
+{
+  if (!col) return fail(SDR_E_INVALID_ARG, "col is NULL");
+  if (col->total_rows > 0x7FFFFFFFu) return fail(SDR_E_BOUNDS, "total_rows (%u) > INT32_MAX", col->total_rows);
+  int rc = check_runs(col->data, col->n_data, col->total_rows, "data");
+  if (rc) return rc;
+  for (uint32_t r = 0; r < col->n_data; ++r)
+    if (col->data[r].codec != SDR_CODEC_FSST)
+      return fail(SDR_E_UNSUPPORTED_CODEC, "viability invariant violated: data codec %u (%s) not implemented",
+                  col->data[r].codec, sdr_codec_name(col->data[r].codec));
+  if (col->has_nulls && col->n_validity) {
+    rc = check_validity_runs(col->validity, col->n_validity, col->total_rows);
+    if (rc) return rc;
+    if (col->total_rows && !col->d_null_mask) return fail(SDR_E_INVALID_ARG, "d_null_mask is NULL");
+  }
+  if (col->total_rows && !col->d_offsets) return fail(SDR_E_INVALID_ARG, "d_offsets is NULL");
+  return SDR_OK;
+}
+
+int decode_strings(sdr_string_column* col, hipStream_t s, uint32_t mode)
+{
+  clear_error();
+  if (mode != SDR_LAUNCH_BATCHED && mode != SDR_LAUNCH_PER_SEGMENT && mode != SDR_LAUNCH_PERSISTENT)
+    return fail(SDR_E_INVALID_ARG, "unknown launch mode %u", mode);
+  int rc = validate_string_column(col);
+  if (rc) return rc;
+  if (mode == SDR_LAUNCH_PERSISTENT && col->total_rows) return decode_strings_persistent(col, s);
+  col->chars_used = 0;
+  col->null_count = 0;
+  const uint32_t n = col->total_rows;
+  if (n == 0) {
+    if (col->d_offsets) HIP_TRY(hipMemsetAsync(col->d_offsets, 0, sizeof(int32_t), s));
+    return SDR_OK;
+  }
+
+  // segments in row order; rows they do not cover become empty strings
+  std::vector<const sdr_segment*> segs;
+  for (uint32_t r = 0; r < col->n_data; ++r)
+    for (uint32_t k = 0; k < col->data[r].n_segments; ++k)
+      if (col->data[r].segments[k].row_count) segs.push_back(&col->data[r].segments[k]);
+  std::stable_sort(segs.begin(), segs.end(),
+                   [](const sdr_segment* a, const sdr_segment* b) { return a->row_offset < b->row_offset; });
+  for (size_t i = 1; i < segs.size(); ++i)
+    if (segs[i]->row_offset < segs[i - 1]->row_offset + segs[i - 1]->row_count)
+      return fail(SDR_E_BOUNDS, "string segments overlap at row %u", segs[i]->row_offset);
+
+  std::vector<FsstSeg> hsegs;
+  std::vector<FsstChunk> chunks;
+  std::vector<std::pair<uint32_t, uint32_t>> groups;  // launch groups: [first, count)
+  auto add_range = [&](uint32_t seg, uint32_t col_row, uint32_t rows) {
+    const uint32_t first = static_cast<uint32_t>(chunks.size());
+    for (uint32_t o = 0; o < rows; o += FSST_CHUNK_ROWS)
+      chunks.push_back({seg, seg == FSST_GAP ? 0u : o, std::min(FSST_CHUNK_ROWS, rows - o), col_row + o});
+    groups.push_back({first, static_cast<uint32_t>(chunks.size()) - first});
+  };
+  uint32_t cursor = 0;
+  for (const sdr_segment* g : segs) {
+    if (g->row_offset > cursor) add_range(FSST_GAP, cursor, g->row_offset - cursor);
+    hsegs.push_back({g->d_bytes, g->bytes_size, g->row_count});
+    add_range(static_cast<uint32_t>(hsegs.size() - 1), g->row_offset, g->row_count);
+    cursor = g->row_offset + g->row_count;
+  }
+  if (cursor < n) add_range(FSST_GAP, cursor, n - cursor);
+  const uint32_t n_chunks = static_cast<uint32_t>(chunks.size());
+
+  const size_t seg_bytes   = std::max<size_t>(1, hsegs.size()) * sizeof(FsstSeg);
+  const size_t chunk_bytes = chunks.size() * sizeof(FsstChunk);
+  const size_t state_bytes = sizeof(FsstState) + sizeof(uint64_t) * n_chunks;
+  const size_t total_bytes = seg_bytes + chunk_bytes + state_bytes + sizeof(unsigned long long);
+  uint8_t* d = nullptr;
+  rc = device_alloc(reinterpret_cast<void**>(&d), total_bytes, s);
+  if (rc) return rc;
+  auto* d_segs   = reinterpret_cast<FsstSeg*>(d);
+  auto* d_chunks = reinterpret_cast<FsstChunk*>(d + seg_bytes);
+  auto* d_state  = reinterpret_cast<FsstState*>(d + seg_bytes + chunk_bytes);
+  auto* d_nulls  = reinterpret_cast<unsigned long long*>(d + seg_bytes + chunk_bytes + state_bytes);
+
+  hipError_t e = hipSuccess;
+  if (!hsegs.empty()) e = hipMemcpyAsync(d_segs, hsegs.data(), hsegs.size() * sizeof(FsstSeg), hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) e = hipMemcpyAsync(d_chunks, chunks.data(), chunk_bytes, hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) e = launch_fsst_reset(d_state, n_chunks, s);
+  if (e == hipSuccess) {
+    if (mode == SDR_LAUNCH_PER_SEGMENT) {
+      for (size_t i = 0; e == hipSuccess && i < groups.size(); ++i)
+        e = launch_fsst(d_chunks, groups[i].first, groups[i].second, n_chunks, d_segs, d_state, col->d_offsets,
+                        col->d_chars, col->d_chars ? col->chars_capacity : 0, n, s);
+    } else {
+      e = launch_fsst(d_chunks, 0, n_chunks, n_chunks, d_segs, d_state, col->d_offsets, col->d_chars,
+                      col->d_chars ? col->chars_capacity : 0, n, s);
+    }
+  }
+  const bool masked = col->has_nulls && col->d_null_mask;
+  if (e == hipSuccess && masked) {
+    e = launch_fill_u8(reinterpret_cast<uint8_t*>(d_nulls), 0, sizeof(unsigned long long), s);
+    if (e == hipSuccess) {
+      rc = build_null_mask(col->validity, col->n_validity, n, col->d_null_mask, d_nulls, s);
+      if (rc) {
+        device_free(d, s);
+        return rc;
+      }
+    }
+  }
+  struct {
+    uint32_t ticket, error;
+    uint64_t total;
+  } h{};
+  unsigned long long h_nulls = 0;
+  if (e == hipSuccess) e = hipMemcpyAsync(&h, d_state, sizeof(h), hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess && masked) e = hipMemcpyAsync(&h_nulls, d_nulls, sizeof(h_nulls), hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  device_free(d, s);
+  if (e != hipSuccess) return fail(SDR_E_HIP, "string decode: %s", hipGetErrorString(e));
+  if (h.error) return fail(SDR_E_HIP, "string decode: offset look-back did not complete");
+  col->chars_used = h.total;
+  col->null_count = masked ? static_cast<int64_t>(h_nulls) : 0;
+  if (h.total > 0x7FFFFFFFull)
+    return fail(SDR_E_OVERFLOW, "total_chars (%llu) exceeds int32 max", static_cast<unsigned long long>(h.total));
+  if (h.total > col->chars_capacity || (h.total && !col->d_chars))
+    return fail(SDR_E_OVERFLOW, "total_chars (%llu) exceeds chars_capacity (%llu)",
+                static_cast<unsigned long long>(h.total), static_cast<unsigned long long>(col->chars_capacity));
+  return SDR_OK;
+}
+
+}  // namespace sdr
 
 using namespace sdr;
 
@@ -22,16 +157,12 @@ uint64_t sdr_strings_upper_bound(const sdr_run* runs, uint32_t n_runs)
 
 int sdr_decode_strings_column_mode(sdr_string_column* col, sdr_stream_t stream, uint32_t launch_mode)
 {
-  (void)col;
-  (void)stream;
-  (void)launch_mode;
-  clear_error();
-  return fail(SDR_E_UNSUPPORTED_CODEC, "string decode is not built in this library yet");
+  return decode_strings(col, static_cast<hipStream_t>(stream), launch_mode);
 }
 
 int sdr_decode_strings_column(sdr_string_column* col, sdr_stream_t stream)
 {
-  return sdr_decode_strings_column_mode(col, stream, SDR_LAUNCH_BATCHED);
+  return decode_strings(col, static_cast<hipStream_t>(stream), SDR_LAUNCH_BATCHED);
 }
 
 }  // extern "C"

@@ -20,6 +20,8 @@ DECLARED = [
     "sdr_strings_upper_bound", "sdr_decode_strings_column", "sdr_decode_strings_column_mode",
     "sdr_plan_create", "sdr_plan_launch", "sdr_plan_launch_codec", "sdr_plan_num_launches",
     "sdr_plan_null_counts", "sdr_plan_destroy",
+    "sdr_batch_create", "sdr_batch_set_option", "sdr_batch_add_table", "sdr_batch_add_strings",
+    "sdr_batch_num_entries", "sdr_batch_launch", "sdr_batch_finish", "sdr_batch_clear", "sdr_batch_destroy",
 ]
 
 _lib = None
@@ -74,6 +76,19 @@ def load() -> C.CDLL:
     lib.sdr_plan_null_counts.restype = i32
     lib.sdr_plan_destroy.argtypes = [vp]
     lib.sdr_plan_destroy.restype = i32
+    lib.sdr_batch_create.argtypes = [P(vp)]
+    lib.sdr_batch_set_option.argtypes = [vp, u32, C.c_int64]
+    lib.sdr_batch_add_table.argtypes = [vp, P(_abi.Column), u32]
+    lib.sdr_batch_add_strings.argtypes = [vp, P(_abi.StringColumn)]
+    lib.sdr_batch_num_entries.argtypes = [vp]
+    lib.sdr_batch_launch.argtypes = [vp, vp]
+    lib.sdr_batch_finish.argtypes = [vp, vp]
+    lib.sdr_batch_clear.argtypes = [vp]
+    lib.sdr_batch_destroy.argtypes = [vp]
+    for f in ("sdr_batch_create", "sdr_batch_set_option", "sdr_batch_add_table", "sdr_batch_add_strings",
+              "sdr_batch_num_entries", "sdr_batch_launch", "sdr_batch_finish", "sdr_batch_clear",
+              "sdr_batch_destroy"):
+        getattr(lib, f).restype = i32
     _lib = lib
     return lib
 
