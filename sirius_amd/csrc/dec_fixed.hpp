@@ -1,13 +1,13 @@
 // dec_fixed.hpp — per-unit device decoders of the fixed-width codecs
-// (BITPACKING, RLE, ALP, ALPRD), shared by the per-codec kernels
-// (k_fixed.hip) and the persistent mixed-codec kernel (k_batch.hip).
+// (BITPACKING, ALP, ALPRD, UNCOMPRESSED, CONSTANT), shared by the per-codec
+// kernels (k_fixed.hip) and the persistent mixed-codec kernel (k_batch.hip).
+// RLE decodes block-cooperatively (dec_rle.hpp).
 //
 // Work partitioning: a block is 4 wavefronts; each wavefront independently
 // decodes one work unit of one segment (blockIdx.y selects the segment,
 // blockIdx.x * 4 + wave selects the unit):
 //   BITPACKING  unit = one 2048-row metadata group
 //   ALP/ALPRD   unit = one 1024-row vector
-//   RLE         unit = RLE_UNIT_ROWS rows
 // Metadata is parsed by every lane from the same addresses, so it lives in
 // scalar registers; no block barrier is ever needed.  Malformed metadata
 // zero-fills the unit's trusted row range, as the reference kernels do.
@@ -71,7 +71,9 @@ __device__ __forceinline__ void wave_fill(T* out, uint32_t n, F gen)
 enum : uint32_t { BP_INVALID = 0, BP_CONSTANT = 2, BP_CONSTANT_DELTA = 3, BP_DELTA_FOR = 4, BP_FOR = 5 };
 constexpr uint32_t BP_GROUP = 2048;
 
-template <typename T>
+// `lds` holds STAGE_B bytes of packed bits (plus the head and guard words);
+// groups wider than that are staged in power-of-two row chunks.
+template <typename T, uint32_t STAGE_B = STAGE_BYTES>
 __device__ void bp_group(const uint8_t* seg, uint32_t bytes, uint32_t k, uint32_t grc, T* out,
                          uint32_t* lds)
 {
@@ -81,15 +83,15 @@ __device__ void bp_group(const uint8_t* seg, uint32_t bytes, uint32_t k, uint32_
 
   uint32_t mode = BP_INVALID, width = 0, rc = 0;
   uint64_t frame = 0, aux = 0, doff = 0;
-  const uint64_t metadata_end = bytes >= 8 ? ld_u64_any(seg) : 0;
+  const uint64_t metadata_end = bytes >= 8 ? su64(seg) : 0;
   if (metadata_end >= 8u + (uint64_t{k} + 1u) * 4u && metadata_end <= bytes) {
-    const uint32_t enc      = ld_u32_any(seg + metadata_end - (uint64_t{k} + 1u) * 4u);
+    const uint32_t enc      = su32(seg + metadata_end - (uint64_t{k} + 1u) * 4u);
     const uint32_t data_off = enc & 0x00FFFFFFu;
     const uint32_t pm       = enc >> 24;
     rc                      = grc;
     if (uint64_t{data_off} + 2u * TS <= metadata_end) {
-      const uint64_t v0 = ld_t_any<TS>(seg + data_off);
-      const uint64_t v1 = ld_t_any<TS>(seg + data_off + TS);
+      const uint64_t v0 = su_t<TS>(seg + data_off);
+      const uint64_t v1 = su_t<TS>(seg + data_off + TS);
       if (pm == BP_CONSTANT) {
         mode = pm;
         aux  = v0;
@@ -106,7 +108,7 @@ __device__ void bp_group(const uint8_t* seg, uint32_t bytes, uint32_t k, uint32_
         mode  = pm;
         frame = v0;
         width = static_cast<uint32_t>(v1);
-        aux   = ld_t_any<TS>(seg + data_off + 2u * TS);
+        aux   = su_t<TS>(seg + data_off + 2u * TS);
         doff  = uint64_t{data_off} + 3u * TS;
       }
       if (width > TS * 8u) {
@@ -139,9 +141,11 @@ __device__ void bp_group(const uint8_t* seg, uint32_t bytes, uint32_t k, uint32_
   }
 
   // FOR / DELTA_FOR: stage packed words through LDS in chunks of at most
-  // 8 KiB of bits (2048 rows unless an 8-byte type needs > 32 bits).
+  // STAGE_B bytes of bits (a multiple of the 64 * VPL rows of one step).
+  static_assert(STAGE_B * 8u >= 64u * (BP_GROUP / 4u) && STAGE_B <= STAGE_BYTES, "stage size");
   const bool al          = (reinterpret_cast<uintptr_t>(out) & 15u) == 0;
-  const uint32_t chunk   = (width * BP_GROUP <= STAGE_BYTES * 8u) ? BP_GROUP : BP_GROUP / 2;
+  uint32_t chunk         = BP_GROUP;
+  while (chunk * width > STAGE_B * 8u) chunk >>= 1;
   const T tframe         = static_cast<T>(frame);
   const uint8_t* stream  = seg + doff;
   A carry                = static_cast<A>(aux);  // DELTA_FOR running value
@@ -258,19 +262,19 @@ __device__ void alp_vector(const uint8_t* seg, uint32_t bytes, uint32_t v, uint3
   uint32_t data_off = 0, exp = 0, fac = 0, ec = 0, bw = 0, packed = 0;
   uint64_t for_v = 0;
   if (bytes >= 4u) {
-    const uint32_t metadata_end = ld_u32_any(seg);
+    const uint32_t metadata_end = su32(seg);
     const uint64_t need         = (uint64_t{v} + 2u) * 4u;
     if (metadata_end <= bytes && metadata_end >= need) {
-      data_off = ld_u32_any(seg + metadata_end - (v + 1u) * 4u);
+      data_off = su32(seg + metadata_end - (v + 1u) * 4u);
       if (data_off >= 4u && data_off < metadata_end) {
-        exp = ld_u8(seg + data_off);
+        exp = su8(seg + data_off);
         if (exp == 0xFFu) {
           if (uint64_t{data_off} + 1u + uint64_t{fill} * TS <= metadata_end) status = 1;
         } else if (uint64_t{data_off} + 13u <= metadata_end) {
-          fac   = ld_u8(seg + data_off + 1);
-          ec    = ld_u16_any(seg + data_off + 2);
-          for_v = ld_u64_any(seg + data_off + 4);
-          bw    = ld_u8(seg + data_off + 12);
+          fac   = su8(seg + data_off + 1);
+          ec    = su16(seg + data_off + 2);
+          for_v = su64(seg + data_off + 4);
+          bw    = su8(seg + data_off + 12);
           if (bw <= TS * 8u && exp <= MAX_EXP && fac <= 18u) {
             packed = bp_req_bytes(fill, bw);
             const uint64_t end = uint64_t{data_off} + 13u + packed + uint64_t{ec} * (TS + 2u);
@@ -343,17 +347,17 @@ __device__ void alprd_vector(const uint8_t* seg, uint32_t bytes, uint32_t v, uin
 
   uint32_t status = 0, data_off = 0, rbw = 0, lbw = 0, dsz = 0, ec = 0, lpb = 0, rpb = 0;
   if (bytes >= 7u) {
-    const uint32_t metadata_end = ld_u32_any(seg);
+    const uint32_t metadata_end = su32(seg);
     const uint64_t need         = 7u + (uint64_t{v} + 1u) * 4u;
     if (metadata_end <= bytes && metadata_end >= need) {
-      rbw = ld_u8(seg + 4);
-      lbw = ld_u8(seg + 5);
-      dsz = ld_u8(seg + 6);
+      rbw = su8(seg + 4);
+      lbw = su8(seg + 5);
+      dsz = su8(seg + 6);
       const uint32_t dict_end = 7u + dsz * 2u;
       if (rbw <= TBITS && lbw <= 3u && dsz <= 8u && dict_end <= metadata_end) {
-        data_off = ld_u32_any(seg + metadata_end - (v + 1u) * 4u);
+        data_off = su32(seg + metadata_end - (v + 1u) * 4u);
         if (data_off >= dict_end && uint64_t{data_off} + 2u <= metadata_end) {
-          ec = ld_u16_any(seg + data_off);
+          ec = su16(seg + data_off);
           if (ec == 0xFFFFu) {
             if (uint64_t{data_off} + 2u + uint64_t{fill} * TS <= metadata_end) status = 1;
           } else {
@@ -384,7 +388,7 @@ __device__ void alprd_vector(const uint8_t* seg, uint32_t bytes, uint32_t v, uin
   // dictionary: unused slots read as zero
   R dict[8];
 #pragma unroll
-  for (uint32_t d = 0; d < 8; ++d) dict[d] = d < dsz ? static_cast<R>(ld_u16_any(seg + 7u + 2u * d)) : R(0);
+  for (uint32_t d = 0; d < 8; ++d) dict[d] = d < dsz ? static_cast<R>(su16(seg + 7u + 2u * d)) : R(0);
 
   const uint8_t* lp = seg + data_off + 2u;
   const uint8_t* rp = lp + lpb;
@@ -428,173 +432,6 @@ __device__ void alprd_vector(const uint8_t* seg, uint32_t bytes, uint32_t v, uin
                                  : static_cast<R>(static_cast<R>(ld_u16_any(exc + 2u * e)) << rbw) | right;
       out[p]        = __builtin_bit_cast(T, bits);
     });
-  }
-}
-
-// ===========================================================================
-// RLE
-// Layout, header parse and count walk: the reference's gpu_decode_rle.cu:
-// 17-36, :140-171, :255-316 (first exact match of the running count wins; a
-// zero count or an overshoot in a 2048-entry tile with no match is
-// malformed; the walk stops at 90112 entries).  Expansion of a unit's rows
-// runs over windows of 64 runs: a DPP scan gives each run's end row, run
-// ends inside a 64-row step become bits of a 64-bit mask, and each row's run
-// is the ballot count of ends at or before the step plus the popcount of
-// mask bits at or below the row.
-// ===========================================================================
-constexpr uint32_t RLE_TILE        = 2048;
-constexpr uint32_t RLE_MAX_ENTRIES = 90112;
-
-template <typename T>
-__device__ void rle_unit(const uint8_t* seg, uint32_t bytes, uint32_t rows, uint32_t ra, uint32_t rb, T* out,
-                         unsigned long long* lds_mask)
-{
-  constexpr uint32_t TS = sizeof(T);
-  T* uout = out + ra;
-  const uint32_t n = rb - ra;
-
-  bool malformed = true;
-  uint64_t counts_off = 0;
-  uint32_t n_max = 0;
-  if (bytes >= 8u) {
-    counts_off = ld_u64_any(seg);
-    if (counts_off >= 8u && counts_off < bytes) {
-      const uint64_t vb = counts_off - 8u, cb = bytes - counts_off;
-      uint64_t n_real   = (vb % TS == 0) ? vb / TS : cb / 2u;
-      if (n_real > RLE_MAX_ENTRIES) n_real = RLE_MAX_ENTRIES;
-      if (n_real > cb / 2u) n_real = cb / 2u;
-      n_max     = static_cast<uint32_t>(n_real);
-      malformed = false;
-    }
-  }
-  const uint8_t* counts = seg + counts_off;
-
-  // ---- count walk: validate, find ec and the run containing row ra ----
-  uint32_t ec = 0, ja = 0, ja_start = 0;
-  bool have_ja = false;
-  if (!malformed) {
-    malformed     = true;
-    uint32_t run  = 0;   // running sum before this sub-step
-    bool tile_bad = false;
-    for (uint32_t e0 = 0; e0 < n_max; e0 += WAVE * 8u) {
-      const uint32_t i0 = e0 + lane_id() * 8u;
-      uint32_t c[8], L[8];
-      uint32_t acc = 0;
-      bool zero = false;
-#pragma unroll
-      for (uint32_t k = 0; k < 8; ++k) {
-        const uint32_t i = i0 + k;
-        c[k] = i < n_max ? ld_u16_any(counts + 2u * i) : 0u;
-        zero |= (i < n_max) && c[k] == 0;
-        acc += c[k];
-        L[k] = acc;
-      }
-      const uint32_t incl = wave_incl_sum_u32(acc);
-      const uint32_t base = run + incl - acc;
-      // first entry whose prefix equals rows (match) / exceeds ra (ja)
-      uint32_t mk = 8, jk = 8;
-      bool over = false;
-#pragma unroll
-      for (uint32_t k = 0; k < 8; ++k) {
-        const uint32_t i = i0 + k;
-        const uint32_t p = base + L[k];
-        if (i < n_max) {
-          if (mk == 8 && p == rows) mk = k;
-          if (mk == 8 && p > rows) over = true;
-          if (jk == 8 && p > ra) jk = k;
-        }
-      }
-      if (!have_ja) {
-        const uint64_t bj = ballot(jk < 8);
-        if (bj) {
-          const int l      = __builtin_ctzll(bj);
-          const uint32_t k = __builtin_amdgcn_readlane(jk, l);
-          uint32_t pstart  = 0;
-#pragma unroll
-          for (uint32_t q = 0; q < 8; ++q) pstart = (q == jk) ? base + L[q] - c[q] : pstart;
-          ja       = e0 + static_cast<uint32_t>(l) * 8u + k;
-          ja_start = __builtin_amdgcn_readlane(pstart, l);
-          have_ja  = true;
-        }
-      }
-      const uint64_t bm = ballot(mk < 8);
-      // zero/overshoot only count at entries before the match in this sub-step,
-      // but a match anywhere in the 2048-entry tile excuses the whole tile
-      tile_bad |= ballot(zero) != 0 || ballot(over) != 0;
-      if (bm) {
-        const int l = __builtin_ctzll(bm);
-        ec          = e0 + static_cast<uint32_t>(l) * 8u + __builtin_amdgcn_readlane(mk, l) + 1u;
-        malformed   = false;
-        break;
-      }
-      run += __builtin_amdgcn_readlane(incl, 63);
-      const bool tile_end = ((e0 + WAVE * 8u) % RLE_TILE) == 0 || e0 + WAVE * 8u >= n_max;
-      if (tile_end) {
-        if (tile_bad) break;
-        tile_bad = false;
-      }
-    }
-    if (!malformed && 8u + uint64_t{ec} * TS > bytes) malformed = true;
-  }
-  if (malformed || !have_ja) {
-    wave_fill<T>(uout, n, [](uint32_t) { return T(0); });
-    return;
-  }
-
-  // ---- expansion ----
-  const uint8_t* values = seg + 8u;
-  uint32_t j      = ja;        // first run of this window
-  uint32_t wstart = ja_start;  // first row of run j
-  while (wstart < rb && j < ec) {
-    const uint32_t jl  = j + lane_id();
-    const bool live    = jl < ec;
-    const uint32_t cnt = live ? ld_u16_any(counts + 2u * jl) : 0u;
-    const uint64_t val = live ? ld_t_any<TS>(values + uint64_t{jl} * TS) : 0u;
-    const uint32_t incl = wave_incl_sum_u32(cnt);
-    const uint32_t end  = wstart + incl;                         // exclusive end row of run jl
-    const uint32_t wend = wstart + __builtin_amdgcn_readlane(incl, 63);
-    const bool dup      = ballot(live && cnt == 0) != 0;         // zero-length runs in window
-    const uint32_t lo   = wstart > ra ? wstart : ra;
-    const uint32_t hi   = wend < rb ? wend : rb;
-    // steps are aligned to the unit start so lanes write consecutive rows
-    for (uint32_t r0 = ra + ((lo - ra) & ~(WAVE - 1u)); r0 < hi; r0 += WAVE) {
-      const uint32_t row = r0 + lane_id();
-      uint32_t rank;
-      if (!dup) {
-        const uint32_t pos = end - r0;  // run ends inside (r0, r0 + 64) become bits
-        const unsigned long long bit =
-          (end > r0 && pos < WAVE && live) ? (1ull << pos) : 0ull;
-        if (lane_id() == 0) *lds_mask = 0;
-        wave_lds_sync();
-        if (bit) atomicOr(lds_mask, bit);
-        wave_lds_sync();
-        const unsigned long long m = *lds_mask;
-        wave_lds_sync();
-        const uint32_t before = static_cast<uint32_t>(__builtin_popcountll(ballot(live && end <= r0)));
-        const unsigned long long upto =
-          lane_id() == 63 ? ~0ull : ((1ull << (lane_id() + 1u)) - 1ull);
-        rank = before + static_cast<uint32_t>(__builtin_popcountll(m & upto));
-      } else {
-        rank = 0;
-        for (uint32_t l = 0; l < WAVE; ++l) {
-          const uint32_t el = __builtin_amdgcn_readlane(end, l);
-          const bool ll     = j + l < ec;
-          rank += (ll && el <= row) ? 1u : 0u;
-        }
-      }
-      const uint32_t src = rank < WAVE ? rank : WAVE - 1u;
-      uint64_t x;
-      if constexpr (TS == 8) {
-        const uint32_t xl = __builtin_amdgcn_ds_bpermute(static_cast<int>(src << 2), static_cast<int>(static_cast<uint32_t>(val)));
-        const uint32_t xh = __builtin_amdgcn_ds_bpermute(static_cast<int>(src << 2), static_cast<int>(static_cast<uint32_t>(val >> 32)));
-        x = static_cast<uint64_t>(xl) | (static_cast<uint64_t>(xh) << 32);
-      } else {
-        x = static_cast<uint32_t>(__builtin_amdgcn_ds_bpermute(static_cast<int>(src << 2), static_cast<int>(static_cast<uint32_t>(val))));
-      }
-      if (row >= lo && row < hi) st_nt(out + row, static_cast<T>(x));
-    }
-    j += WAVE;
-    wstart = wend;
   }
 }
 

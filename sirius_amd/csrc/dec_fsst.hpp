@@ -185,9 +185,9 @@ __device__ __forceinline__ void fsst_chunk(const FsstChunk ch, uint32_t c, uint3
   bool ok = false;
   uint32_t dict_end = 0, width = 0, symoff = 0;
   if (!gap && sg.bytes_size >= 16u) {
-    dict_end = ld_u32_any(seg + 4);
-    width    = ld_u32_any(seg + 8);
-    symoff   = ld_u32_any(seg + 12);
+    dict_end = su32(seg + 4);
+    width    = su32(seg + 8);
+    symoff   = su32(seg + 12);
     ok       = dict_end <= sg.bytes_size && symoff < dict_end && width <= 32u;
   }
 

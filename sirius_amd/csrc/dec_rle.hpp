@@ -1,0 +1,320 @@
+// dec_rle.hpp — block-cooperative RLE decode.
+//
+// Layout, header parse and count walk follow the reference's
+// gpu_decode_rle.cu:17-36, :140-171 and :255-316: u64 counts offset at byte
+// 0, values T[ec] from byte 8, u16 counts at the counts offset; the first
+// entry whose running (uint32) count equals the segment's row count ends the
+// segment; a zero count, or a running count above the row count, in a
+// 2048-entry tile that holds no match is malformed, and the walk stops at
+// 90112 entries.  Malformed segments decode to zeros.
+//
+// One 256-thread block decodes one slice of rows of one segment:
+//   1. walk: the counts arrive 8192 entries at a time (16 KiB, 16-byte loads
+//      into LDS); each thread sums 32 consecutive counts, a block scan turns
+//      the sums into per-chunk row bases, and block minima locate the first
+//      exact match and the first zero/overshoot.  Segments of up to 8192
+//      entries cost one pass, i.e. one memory round trip.
+//   2. expansion: the slice is split over the 4 wavefronts; each finds its
+//      first run from the LDS chunk bases, then walks windows of 64 runs:
+//      a DPP scan gives each run's end row, and for every 64-row step the
+//      ends that fall inside the step become bits of a uniform 64-bit mask
+//      (built with scalar readlanes), so a row's run index is the ballot
+//      count of ends at or before the step plus a popcount of the mask.
+//      Windows holding a zero-length run take an exact per-lane count.
+#pragma once
+
+#include "sdr_device.hpp"
+
+namespace sdr {
+
+constexpr uint32_t RLE_TILE        = 2048;
+constexpr uint32_t RLE_MAX_ENTRIES = 90112;
+constexpr uint32_t RLE_THREADS     = 256;
+constexpr uint32_t RLE_WAVES       = RLE_THREADS / WAVE;
+constexpr uint32_t RLE_PER_THREAD  = 32;
+constexpr uint32_t RLE_SUPER       = RLE_THREADS * RLE_PER_THREAD;  // 8192 entries per pass
+constexpr uint32_t RLE_MAX_SUPERS  = (RLE_MAX_ENTRIES + RLE_SUPER - 1) / RLE_SUPER;
+constexpr uint32_t RLE_STAGE_WORDS = (RLE_SUPER * 2 + 48) / 4;
+static_assert(RLE_SUPER % RLE_TILE == 0, "passes hold whole tiles");
+
+struct RleSmem {
+  uint32_t stage[RLE_STAGE_WORDS];         // counts of the resident pass
+  uint32_t chunk_base[RLE_THREADS];        // row base of each 32-entry chunk
+  uint32_t wave_sum[RLE_WAVES];
+  uint32_t pass_base[RLE_MAX_SUPERS + 1];  // row base of each pass
+  uint32_t match, bad;                     // block minima (entry index)
+};
+
+// Block-wide copy of [p, p + nbytes) into LDS as 16-byte blocks (LDS byte 0
+// holds align_down(p, 16)) with LDS-direct loads; returns the byte skew of
+// p.  Each wave has waited for its own loads; the caller's barrier makes
+// the whole stage visible.
+__device__ __forceinline__ uint32_t block_stage(uint32_t* lds, const uint8_t* p, uint32_t nbytes)
+{
+  const uintptr_t a   = reinterpret_cast<uintptr_t>(p);
+  const uintptr_t a16 = a & ~uintptr_t{15};
+  const uint32_t head = static_cast<uint32_t>(a - a16);
+  const uint32_t nblk = nbytes ? (head + nbytes + 15u) >> 4 : 0u;
+  const uint32_t wb   = (threadIdx.x / WAVE) * WAVE;  // first block of this wave per round
+  const uint8_t* src  = reinterpret_cast<const uint8_t*>(a16) + 16u * threadIdx.x;
+  for (uint32_t k = 0; k * RLE_THREADS < nblk; ++k)
+    if (k * RLE_THREADS + threadIdx.x < nblk)
+      lds_direct_16(src + 16u * RLE_THREADS * k, lds + 4u * (RLE_THREADS * k + wb));
+  __builtin_amdgcn_s_waitcnt(WAITCNT_VMEM_DONE);
+  return head;
+}
+
+__device__ __forceinline__ uint32_t lds_u16(const uint32_t* lds, uint32_t byte)
+{
+  const uint32_t w = byte >> 2;
+  return __builtin_amdgcn_alignbit(lds[w + 1], lds[w], (byte & 3u) * 8u) & 0xFFFFu;
+}
+
+// Stage pass `t0` (n entries) and compute every chunk's row base from
+// `base`.  With `detect`, also record the first exact match and the first
+// zero/overshoot entry of the pass in sm.match / sm.bad (UINT32_MAX: none).
+// Returns the pass's count total.  Called by all threads of the block.
+__device__ __forceinline__ uint32_t rle_pass(const uint8_t* counts, uint32_t t0, uint32_t n, uint32_t base,
+                                             uint32_t rows, bool detect, uint32_t& head, RleSmem& sm)
+{
+  __syncthreads();  // earlier readers of sm.stage are done
+  if (detect && threadIdx.x == 0) {
+    sm.match = UINT32_MAX;
+    sm.bad   = UINT32_MAX;
+  }
+  head = block_stage(sm.stage, counts + 2u * t0, 2u * n);
+  __syncthreads();
+
+  // Thread t owns entries [32 t, 32 t + 32) of the pass: 16 words of two
+  // counts each, re-read from LDS (re-aligned by the uniform byte skew) so
+  // the counts never all sit in registers at once.
+  const uint32_t t     = threadIdx.x;
+  const uint32_t q     = RLE_PER_THREAD / 2u * t + (head >> 2);
+  const uint32_t sh    = (head & 3u) * 8u;
+  const uint32_t e0    = RLE_PER_THREAD * t;
+  const uint32_t nlive = e0 >= n ? 0u : (n - e0 < RLE_PER_THREAD ? n - e0 : RLE_PER_THREAD);
+  // word k of the chunk, counts past the pass reading as zero
+  auto word = [&](uint32_t k) {
+    const uint32_t w = __builtin_amdgcn_alignbit(sm.stage[q + k + 1u], sm.stage[q + k], sh);
+    return w & ((2u * k + 1u < nlive ? 0xFFFF0000u : 0u) | (2u * k < nlive ? 0xFFFFu : 0u));
+  };
+  uint32_t s = 0;
+#pragma unroll 4
+  for (uint32_t k = 0; k < RLE_PER_THREAD / 2; ++k) {
+    const uint32_t w = word(k);
+    s += (w & 0xFFFFu) + (w >> 16);
+  }
+  const uint32_t incl = wave_incl_sum_u32(s);
+  const uint32_t wv   = t / WAVE;
+  if (lane_id() == WAVE - 1) sm.wave_sum[wv] = incl;
+  __syncthreads();
+  uint32_t woff = 0, total = 0;
+#pragma unroll
+  for (uint32_t k = 0; k < RLE_WAVES; ++k) {
+    const uint32_t ws = sm.wave_sum[k];
+    woff += k < wv ? ws : 0u;
+    total += ws;
+  }
+  const uint32_t cbase = base + woff + incl - s;
+  sm.chunk_base[t]     = cbase;
+  if (detect) {
+    // bit i: entry e0 + i matches / is a zero or an overshoot
+    uint32_t run = cbase, hits = 0, bads = 0;
+#pragma unroll 4
+    for (uint32_t k = 0; k < RLE_PER_THREAD / 2; ++k) {
+      const uint32_t w  = word(k);
+      const uint32_t c0 = w & 0xFFFFu, c1 = w >> 16;
+      run += c0;
+      hits |= (run == rows ? 1u : 0u) << (2u * k);
+      bads |= ((c0 == 0u) | (run > rows) ? 1u : 0u) << (2u * k);
+      run += c1;
+      hits |= (run == rows ? 1u : 0u) << (2u * k + 1u);
+      bads |= ((c1 == 0u) | (run > rows) ? 1u : 0u) << (2u * k + 1u);
+    }
+    const uint32_t live = nlive >= RLE_PER_THREAD ? ~0u : (1u << nlive) - 1u;
+    hits &= live;
+    bads &= live;
+    const uint32_t m   = hits ? t0 + e0 + static_cast<uint32_t>(__builtin_ctz(hits)) : UINT32_MAX;
+    const uint32_t bad = bads ? t0 + e0 + static_cast<uint32_t>(__builtin_ctz(bads)) : UINT32_MAX;
+    // entries are in thread order, so the first lane holding one is the minimum
+    const uint64_t bm = ballot(m != UINT32_MAX), bb = ballot(bad != UINT32_MAX);
+    if (bm && lane_id() == static_cast<uint32_t>(__builtin_ctzll(bm))) atomicMin(&sm.match, m);
+    if (bb && lane_id() == static_cast<uint32_t>(__builtin_ctzll(bb))) atomicMin(&sm.bad, bad);
+  }
+  __syncthreads();
+  return total;
+}
+
+// Value width is a runtime (block-uniform) byte count: one code path serves
+// all four widths, branching only at the value load and the store.
+__device__ __forceinline__ uint64_t ld_width(const uint8_t* p, uint32_t ts)
+{
+  switch (ts) {
+    case 1: return ld_u8(p);
+    case 2: return ld_u16_any(p);
+    case 4: return ld_u32_any(p);
+    default: return ld_u64_any(p);
+  }
+}
+
+__device__ __forceinline__ void st_width(uint8_t* out, uint32_t row, uint64_t v, uint32_t ts)
+{
+  switch (ts) {
+    case 1: st_nt(out + row, static_cast<uint8_t>(v)); break;
+    case 2: st_nt(reinterpret_cast<uint16_t*>(out) + row, static_cast<uint16_t>(v)); break;
+    case 4: st_nt(reinterpret_cast<uint32_t*>(out) + row, static_cast<uint32_t>(v)); break;
+    default: st_nt(reinterpret_cast<uint64_t*>(out) + row, v); break;
+  }
+}
+
+// Expand rows [lo, hi) of one wave: entries [t0, j_end) are the pass
+// resident in LDS (byte skew head), chunk bases in sm.
+__device__ __forceinline__ void rle_expand(const uint8_t* values, uint32_t ts, uint32_t t0, uint32_t j_end,
+                                           uint32_t head, uint32_t lo, uint32_t hi, uint8_t* out,
+                                           const RleSmem& sm)
+{
+  const uint32_t lane = lane_id();
+  // chunk holding the first run that ends after row lo: the last chunk with
+  // entries whose base is <= lo (bases never decrease up to the match)
+  uint32_t cnt = 0;
+#pragma unroll
+  for (uint32_t u = 0; u < RLE_THREADS / WAVE; ++u) {
+    const uint32_t c = lane + WAVE * u;
+    cnt += static_cast<uint32_t>(
+      __builtin_popcountll(ballot(t0 + RLE_PER_THREAD * c < j_end && sm.chunk_base[c] <= lo)));
+  }
+  if (cnt == 0) return;  // unreachable for a segment whose walk matched
+  const uint32_t ch = cnt - 1u;
+  uint32_t j, wstart;
+  {
+    const uint32_t e    = t0 + RLE_PER_THREAD * ch + lane;
+    const bool live     = lane < RLE_PER_THREAD && e < j_end;
+    const uint32_t c    = live ? lds_u16(sm.stage, head + 2u * (e - t0)) : 0u;
+    const uint32_t incl = wave_incl_sum_u32(c);
+    const uint32_t end  = sm.chunk_base[ch] + incl;
+    const uint64_t hit  = ballot(live && end > lo);
+    if (!hit) return;  // unreachable, as above
+    const int l = __builtin_ctzll(hit);
+    j           = t0 + RLE_PER_THREAD * ch + static_cast<uint32_t>(l);
+    wstart      = __builtin_amdgcn_readlane(end - c, l);
+  }
+  const unsigned long long upto = lane == WAVE - 1 ? ~0ull : ((2ull << lane) - 1ull);
+  while (wstart < hi && j < j_end) {
+    const uint32_t jl   = j + lane;
+    const bool live     = jl < j_end;
+    const uint32_t cnt1 = live ? lds_u16(sm.stage, head + 2u * (jl - t0)) : 0u;
+    const uint64_t val  = live ? ld_width(values + uint64_t{jl} * ts, ts) : 0u;
+    const uint32_t incl = wave_incl_sum_u32(cnt1);
+    const uint32_t end  = wstart + incl;  // exclusive end row of run jl
+    const uint32_t wend = wstart + __builtin_amdgcn_readlane(incl, WAVE - 1);
+    const bool dup      = ballot(live && cnt1 == 0u) != 0;
+    const uint32_t a    = wstart > lo ? wstart : lo;
+    const uint32_t b    = wend < hi ? wend : hi;
+    for (uint32_t r0 = lo + ((a - lo) & ~(WAVE - 1u)); r0 < b; r0 += WAVE) {
+      const uint32_t row = r0 + lane;
+      uint32_t rank;
+      if (!dup) {
+        rank              = static_cast<uint32_t>(__builtin_popcountll(ballot(live && end <= r0)));
+        uint64_t inside   = ballot(live && end > r0 && end - r0 < WAVE);
+        unsigned long long m = 0;
+        while (inside) {
+          const int l = __builtin_ctzll(inside);
+          inside &= inside - 1u;
+          m |= 1ull << (__builtin_amdgcn_readlane(end, l) - r0);
+        }
+        rank += static_cast<uint32_t>(__builtin_popcountll(m & upto));
+      } else {
+        rank = 0;
+#pragma unroll 1
+        for (uint32_t l = 0; l < WAVE; ++l) {
+          const uint32_t el = __builtin_amdgcn_readlane(end, l);
+          rank += (j + l < j_end && el <= row) ? 1u : 0u;
+        }
+      }
+      const int src = static_cast<int>((rank < WAVE ? rank : WAVE - 1u) << 2);
+      const uint32_t xl = __builtin_amdgcn_ds_bpermute(src, static_cast<int>(static_cast<uint32_t>(val)));
+      const uint32_t xh =
+        ts == 8u ? __builtin_amdgcn_ds_bpermute(src, static_cast<int>(static_cast<uint32_t>(val >> 32))) : 0u;
+      if (row >= a && row < b) st_width(out, row, static_cast<uint64_t>(xl) | (static_cast<uint64_t>(xh) << 32), ts);
+    }
+    j += WAVE;
+    wstart = wend;
+  }
+}
+
+// Decode rows [ra, rb) of an RLE segment of `rows` rows of `ts`-byte values
+// (1, 2, 4 or 8) into out[0, rows).  Called by all 256 threads of a block
+// (the control flow is block-uniform).
+__device__ void rle_slice(const uint8_t* seg, uint32_t bytes, uint32_t rows, uint32_t ra, uint32_t rb, uint8_t* out,
+                          uint32_t ts, RleSmem& sm)
+{
+  bool malformed        = true;
+  uint64_t counts_off   = 0;
+  uint32_t n_max        = 0;
+  if (bytes >= 8u) {
+    counts_off = su64(seg);
+    if (counts_off >= 8u && counts_off < bytes) {
+      const uint64_t vb = counts_off - 8u, cb = bytes - counts_off;
+      uint64_t n_real   = (vb % ts == 0) ? vb / ts : cb / 2u;
+      if (n_real > RLE_MAX_ENTRIES) n_real = RLE_MAX_ENTRIES;
+      if (n_real > cb / 2u) n_real = cb / 2u;
+      n_max     = static_cast<uint32_t>(n_real);
+      malformed = false;
+    }
+  }
+  const uint8_t* counts = seg + counts_off;
+
+  // ---- 1. walk ----
+  uint32_t ec = 0, n_pass = 0, head = 0, base = 0;
+  if (!malformed) {
+    malformed = true;
+    for (uint32_t t0 = 0; t0 < n_max; t0 += RLE_SUPER) {
+      const uint32_t n = n_max - t0 < RLE_SUPER ? n_max - t0 : RLE_SUPER;
+      if (threadIdx.x == 0) sm.pass_base[n_pass] = base;
+      const uint32_t total = rle_pass(counts, t0, n, base, rows, true, head, sm);
+      ++n_pass;
+      const uint32_t m = sm.match, bad = sm.bad;
+      if (m != UINT32_MAX) {
+        // a zero or overshoot before the match's tile is malformed
+        malformed = bad < (m & ~(RLE_TILE - 1u));
+        ec        = m + 1u;
+        break;
+      }
+      if (bad != UINT32_MAX) break;
+      base += total;
+    }
+    if (!malformed && 8u + uint64_t{ec} * ts > bytes) malformed = true;
+  }
+
+  const uint32_t w  = threadIdx.x / WAVE;
+  const uint32_t ns = rb - ra;
+  const uint32_t qw = ((ns + RLE_WAVES * WAVE - 1u) / (RLE_WAVES * WAVE)) * WAVE;
+  const uint32_t wa = ra + w * qw < rb ? ra + w * qw : rb;
+  const uint32_t wb = wa + qw < rb ? wa + qw : rb;
+  if (malformed) {
+    for (uint32_t r = wa + lane_id(); r < wb; r += WAVE) st_width(out, r, 0u, ts);
+    return;
+  }
+
+  // ---- 2. expansion, pass by pass (one pass unless ec > 8192) ----
+  __syncthreads();  // pass_base visible
+  const uint8_t* values = seg + 8u;
+  uint32_t resident     = n_pass - 1u;
+  for (uint32_t k = 0; k < n_pass; ++k) {
+    const uint32_t r_lo = sm.pass_base[k];
+    const uint32_t r_hi = k + 1u < n_pass ? sm.pass_base[k + 1u] : rows;
+    if (r_hi <= ra || r_lo >= rb) continue;  // block-uniform
+    const uint32_t t0 = k * RLE_SUPER;
+    if (k != resident) {
+      const uint32_t n = n_max - t0 < RLE_SUPER ? n_max - t0 : RLE_SUPER;
+      rle_pass(counts, t0, n, r_lo, rows, false, head, sm);
+      resident = k;
+    }
+    const uint32_t j_end = ec < t0 + RLE_SUPER ? ec : t0 + RLE_SUPER;
+    const uint32_t lo    = wa > r_lo ? wa : r_lo;
+    const uint32_t hi    = wb < r_hi ? wb : r_hi;
+    if (lo < hi) rle_expand(values, ts, t0, j_end, head, lo, hi, out, sm);
+  }
+}
+
+}  // namespace sdr

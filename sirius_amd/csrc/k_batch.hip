@@ -1,15 +1,16 @@
 // k_batch.hip — persistent mixed-codec decode kernel.
 //
 // One launch decodes a whole device-resident work queue (sdr_batch.hpp):
-// BITPACKING groups, RLE row chunks, ALP/ALPRD vectors, UNCOMPRESSED and
+// BITPACKING groups, RLE row slices, ALP/ALPRD vectors, UNCOMPRESSED and
 // CONSTANT byte ranges and FSST chunks, from any number of columns and
 // segments.  A grid sized to the resident capacity of the GPU loops: a block
 // claims the next queue entry with one atomic, its four wavefronts decode the
-// entry's units (or, for FSST, the block decodes the chunk together), and it
-// claims again until the queue is drained.  Claiming in queue order is what
+// entry's units (RLE slices and FSST chunks are decoded by the whole block),
+// and it claims again until the queue is drained.  Claiming in queue order is what
 // lets FSST chunks of one column wait on their predecessors' offsets: a
 // predecessor was claimed earlier by a running block.
 #include "dec_fixed.hpp"
+#include "dec_rle.hpp"
 #include "dec_fsst.hpp"
 #include "sdr_batch.hpp"
 
@@ -19,7 +20,7 @@ constexpr uint32_t BATCH_SPIN_LIMIT = FSST_SPIN_LIMIT;
 
 union __attribute__((aligned(16))) BatchSmem {
   uint32_t fixed[WAVES_PER_BLOCK][ALP_STAGE_WORDS];
-  unsigned long long rle_mask[WAVES_PER_BLOCK];
+  RleSmem rle;
   FsstSmem fsst;
 };
 
@@ -31,10 +32,6 @@ __device__ __forceinline__ void fixed_units(const BatchItem& it, uint32_t w, Bat
     if (it.kind == BK_BITPACKING) {
       const uint32_t grc = it.row_count - u * BP_GROUP < BP_GROUP ? it.row_count - u * BP_GROUP : BP_GROUP;
       bp_group<T>(it.seg, it.bytes_size, u, grc, reinterpret_cast<T*>(it.out) + size_t{u} * BP_GROUP, sm.fixed[w]);
-    } else if (it.kind == BK_RLE) {
-      const uint32_t ra = u * RLE_UNIT_ROWS;
-      const uint32_t rb = it.row_count - ra < RLE_UNIT_ROWS ? it.row_count : ra + RLE_UNIT_ROWS;
-      rle_unit<T>(it.seg, it.bytes_size, it.row_count, ra, rb, reinterpret_cast<T*>(it.out), &sm.rle_mask[w]);
     } else if constexpr (sizeof(T) >= 4) {
       const uint32_t fill = it.row_count - u * ALP_VEC < ALP_VEC ? it.row_count - u * ALP_VEC : ALP_VEC;
       using F = std::conditional_t<sizeof(T) == 8, double, float>;
@@ -65,6 +62,13 @@ __global__ __launch_bounds__(BLOCK_THREADS) void k_batch(const BatchItem* __rest
                    col.chars_cap, col.total_rows, BATCH_SPIN_LIMIT, sm.fsst);
         break;
       }
+      case BK_RLE:
+        for (uint32_t u = it.unit0; u < it.unit0 + it.n_units; ++u) {
+          const uint32_t ra = u * RLE_SLICE_ROWS;
+          const uint32_t rb = it.row_count - ra < RLE_SLICE_ROWS ? it.row_count : ra + RLE_SLICE_ROWS;
+          rle_slice(it.seg, it.bytes_size, it.row_count, ra, rb, it.out, it.ts, sm.rle);
+        }
+        break;
       case BK_COPY:
       case BK_CONSTANT: {
         const uint64_t total = uint64_t{it.row_count} * it.ts;

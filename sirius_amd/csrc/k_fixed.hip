@@ -3,6 +3,7 @@
 // launch covers a batch of segments: blockIdx.y selects the segment,
 // blockIdx.x * 4 + wave the work unit (decoders in dec_fixed.hpp).
 #include "dec_fixed.hpp"
+#include "dec_rle.hpp"
 
 namespace sdr {
 
@@ -10,15 +11,17 @@ template <typename T>
 __global__ __launch_bounds__(BLOCK_THREADS) void k_bitpacking(const FixedSeg* __restrict__ segs,
                                                               uint32_t seg_base)
 {
-  __shared__ uint32_t lds[WAVES_PER_BLOCK][STAGE_WORDS];
+  // 4 KiB of packed bits per wave keeps 8 waves per SIMD resident
+  constexpr uint32_t SB = 4096;
+  __shared__ uint32_t lds[WAVES_PER_BLOCK][(SB + 48) / 4];
   const FixedSeg sd  = segs[seg_base + blockIdx.y];
   const uint32_t w   = wave_in_block();
   const uint32_t k   = blockIdx.x * WAVES_PER_BLOCK + w;
   const uint32_t ngr = (sd.row_count + BP_GROUP - 1) / BP_GROUP;
   if (k >= ngr) return;
   const uint32_t grc = sd.row_count - k * BP_GROUP < BP_GROUP ? sd.row_count - k * BP_GROUP : BP_GROUP;
-  bp_group<T>(sd.bytes, sd.bytes_size, k, grc, reinterpret_cast<T*>(sd.out) + size_t{k} * BP_GROUP,
-              lds[w]);
+  bp_group<T, SB>(sd.bytes, sd.bytes_size, k, grc, reinterpret_cast<T*>(sd.out) + size_t{k} * BP_GROUP,
+                  lds[w]);
 }
 
 template <typename T, bool RD>
@@ -36,17 +39,16 @@ __global__ __launch_bounds__(BLOCK_THREADS) void k_alp(const FixedSeg* __restric
   else alp_vector<T>(sd.bytes, sd.bytes_size, v, fill, out, lds[w]);
 }
 
-template <typename T>
-__global__ __launch_bounds__(BLOCK_THREADS) void k_rle(const FixedSeg* __restrict__ segs, uint32_t seg_base)
+// blockIdx.x selects the slice of `slice_rows` rows (block-uniform exit).
+__global__ __launch_bounds__(RLE_THREADS) void k_rle(const FixedSeg* __restrict__ segs, uint32_t seg_base,
+                                                    uint32_t slice_rows, uint32_t type_size)
 {
-  __shared__ unsigned long long masks[WAVES_PER_BLOCK];
+  __shared__ RleSmem sm;
   const FixedSeg sd = segs[seg_base + blockIdx.y];
-  const uint32_t w  = wave_in_block();
-  const uint32_t u  = blockIdx.x * WAVES_PER_BLOCK + w;
-  const uint32_t ra = u * RLE_UNIT_ROWS;
+  const uint32_t ra = blockIdx.x * slice_rows;
   if (ra >= sd.row_count) return;
-  const uint32_t rb = sd.row_count - ra < RLE_UNIT_ROWS ? sd.row_count : ra + RLE_UNIT_ROWS;
-  rle_unit<T>(sd.bytes, sd.bytes_size, sd.row_count, ra, rb, reinterpret_cast<T*>(sd.out), &masks[w]);
+  const uint32_t rb = sd.row_count - ra < slice_rows ? sd.row_count : ra + slice_rows;
+  rle_slice(sd.bytes, sd.bytes_size, sd.row_count, ra, rb, sd.out, type_size, sm);
 }
 
 // ===========================================================================
@@ -150,16 +152,10 @@ hipError_t launch_rle(const FixedSeg* d_segs, uint32_t seg_base, uint32_t n, uin
                       hipStream_t s)
 {
   if (n == 0 || max_rows == 0) return hipSuccess;
-  const uint32_t units = (max_rows + RLE_UNIT_ROWS - 1) / RLE_UNIT_ROWS;
-  const uint32_t gx    = (units + WAVES_PER_BLOCK - 1) / WAVES_PER_BLOCK;
+  const uint32_t sr = RLE_SLICE_ROWS;
+  const uint32_t gx = (max_rows + sr - 1) / sr;
   return chunked(seg_base, n, [&](uint32_t b, uint32_t m) {
-    dim3 g(gx, m);
-    switch (ts) {
-      case 1: hipLaunchKernelGGL(k_rle<uint8_t>, g, dim3(BLOCK_THREADS), 0, s, d_segs, b); break;
-      case 2: hipLaunchKernelGGL(k_rle<uint16_t>, g, dim3(BLOCK_THREADS), 0, s, d_segs, b); break;
-      case 4: hipLaunchKernelGGL(k_rle<uint32_t>, g, dim3(BLOCK_THREADS), 0, s, d_segs, b); break;
-      default: hipLaunchKernelGGL(k_rle<uint64_t>, g, dim3(BLOCK_THREADS), 0, s, d_segs, b); break;
-    }
+    hipLaunchKernelGGL(k_rle, dim3(gx, m), dim3(RLE_THREADS), 0, s, d_segs, b, sr, ts);
   });
 }
 

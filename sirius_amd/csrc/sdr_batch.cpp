@@ -41,7 +41,7 @@ struct sdr_batch {
   hipEvent_t launch_done = nullptr;
   bool upload_pending  = false;
   // launch configuration
-  uint32_t units[8]    = {0, 4, 4, 4, 4, 4, 4, 1};  // per BatchKind
+  uint32_t units[8]    = {0, 4, 1, 4, 4, 4, 4, 1};  // per BatchKind
   uint32_t grid        = 0;                         // 0 = resident capacity
   bool use_graph       = true;
   hipGraphExec_t gexec = nullptr;
@@ -89,7 +89,7 @@ uint32_t units_in_segment(BatchKind k, uint32_t rows, uint32_t ts)
 {
   switch (k) {
     case BK_BITPACKING: return (rows + 2047u) / 2048u;
-    case BK_RLE: return (rows + RLE_UNIT_ROWS - 1u) / RLE_UNIT_ROWS;
+    case BK_RLE: return (rows + RLE_SLICE_ROWS - 1u) / RLE_SLICE_ROWS;
     case BK_ALP:
     case BK_ALPRD: return (rows + 1023u) / 1024u;
     default: return static_cast<uint32_t>((uint64_t{rows} * ts + COPY_UNIT_BYTES - 1u) / COPY_UNIT_BYTES);

@@ -68,6 +68,64 @@ __device__ __forceinline__ uint64_t ld_t_any(const uint8_t* p)
   else return ld_u64_any(p);
 }
 
+// ---- wave-uniform loads of segment headers ---------------------------------
+//
+// Header fields are the same for every lane, and segment bytes are never
+// written by a decode kernel, so they are read as constant-address-space
+// words (scalar loads into SGPRs) at an address made wave-uniform with
+// readfirstlane.  Same aligned-word discipline as above.
+
+typedef __attribute__((address_space(4))) const uint32_t const_word;
+
+__device__ __forceinline__ uint64_t uniform_addr(const uint8_t* p)
+{
+  const uint64_t a  = reinterpret_cast<uint64_t>(p);
+  const uint32_t lo = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(a));
+  const uint32_t hi = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(a >> 32));
+  return (static_cast<uint64_t>(hi) << 32) | lo;
+}
+
+__device__ __forceinline__ uint32_t su_word(uint64_t aligned4) { return *(const_word*)(aligned4); }
+
+__device__ __forceinline__ uint32_t su8(const uint8_t* p)
+{
+  const uint64_t a = uniform_addr(p);
+  const uint32_t o = static_cast<uint32_t>(a & 3u);
+  return (su_word(a - o) >> (8u * o)) & 0xFFu;
+}
+
+__device__ __forceinline__ uint32_t su16(const uint8_t* p)
+{
+  const uint64_t a = uniform_addr(p);
+  const uint32_t o = static_cast<uint32_t>(a & 3u);
+  const uint32_t lo = su_word(a - o);
+  const uint32_t v  = (o <= 2u) ? (lo >> (8u * o)) : __builtin_amdgcn_alignbit(su_word(a - o + 4u), lo, 24u);
+  return v & 0xFFFFu;
+}
+
+__device__ __forceinline__ uint32_t su32(const uint8_t* p)
+{
+  const uint64_t a = uniform_addr(p);
+  const uint32_t o = static_cast<uint32_t>(a & 3u);
+  const uint32_t lo = su_word(a - o);
+  if (o == 0) return lo;
+  return __builtin_amdgcn_alignbit(su_word(a - o + 4u), lo, 8u * o);
+}
+
+__device__ __forceinline__ uint64_t su64(const uint8_t* p)
+{
+  return static_cast<uint64_t>(su32(p)) | (static_cast<uint64_t>(su32(p + 4)) << 32);
+}
+
+template <uint32_t TS>
+__device__ __forceinline__ uint64_t su_t(const uint8_t* p)
+{
+  if constexpr (TS == 1) return su8(p);
+  else if constexpr (TS == 2) return su16(p);
+  else if constexpr (TS == 4) return su32(p);
+  else return su64(p);
+}
+
 // ---- wave scans (DPP row shifts + cross-row readlane) ----------------------
 
 template <int CTRL>
@@ -144,19 +202,38 @@ __device__ __forceinline__ uint64_t lds_bits64(const uint32_t* lds, uint32_t bit
   return w >= 64u ? v : (v & ((1ull << w) - 1ull));
 }
 
+// ---- global -> LDS staging -------------------------------------------------
+//
+// Staging uses gfx950's LDS-direct loads (global_load_lds_dwordx4): lane i of
+// an instruction writes its 16 bytes at LDS address M0 + 16 i without
+// passing through VGPRs, so every load of a stage is in flight at once and a
+// stage costs one memory round trip.  Inactive lanes write nothing.
+
+typedef __attribute__((address_space(3))) uint32_t lds_word;
+
+// s_waitcnt vmcnt(0), expcnt and lgkmcnt left at their maxima (gfx9 encoding)
+constexpr int WAITCNT_VMEM_DONE = 0x0F70;
+
+__device__ __forceinline__ void lds_direct_16(const uint8_t* g, uint32_t* lds_dst)
+{
+  __builtin_amdgcn_global_load_lds(g, (lds_word*)(lds_dst), 16, 0, 0);
+}
+
 // Copy the bytes [p, p + nbytes) of global memory into LDS as whole 16-byte
-// blocks: LDS byte 0 holds the byte at align_down(p, 16).  Returns the bit
-// offset of p in the staged words.  Every 16-byte block read overlaps the
-// requested range, so the copy never touches a page the range does not.
+// blocks, by one wavefront: LDS byte 0 holds the byte at align_down(p, 16).
+// Returns the bit offset of p in the staged words.  Every 16-byte block read
+// overlaps the requested range, so the copy never touches a page the range
+// does not.  The data is in LDS when this returns (the wave has waited).
 __device__ __forceinline__ uint32_t stage_to_lds(uint32_t* lds, const uint8_t* p, uint32_t nbytes)
 {
-  uintptr_t a    = reinterpret_cast<uintptr_t>(p);
-  uintptr_t a16  = a & ~uintptr_t{15};
-  uint32_t head  = static_cast<uint32_t>(a - a16);
-  uint32_t nblk  = nbytes ? (head + nbytes + 15u) >> 4 : 0u;
-  const u32x4* src = reinterpret_cast<const u32x4*>(a16);
-  u32x4* dst       = reinterpret_cast<u32x4*>(lds);
-  for (uint32_t b = lane_id(); b < nblk; b += WAVE) dst[b] = __builtin_nontemporal_load(src + b);
+  const uintptr_t a   = reinterpret_cast<uintptr_t>(p);
+  const uintptr_t a16 = a & ~uintptr_t{15};
+  const uint32_t head = static_cast<uint32_t>(a - a16);
+  const uint32_t nblk = nbytes ? (head + nbytes + 15u) >> 4 : 0u;
+  const uint8_t* src  = reinterpret_cast<const uint8_t*>(a16) + 16u * lane_id();
+  for (uint32_t k = 0; k * WAVE < nblk; ++k)
+    if (k * WAVE + lane_id() < nblk) lds_direct_16(src + 16u * WAVE * k, lds + 4u * WAVE * k);
+  __builtin_amdgcn_s_waitcnt(WAITCNT_VMEM_DONE);
   return head * 8u;
 }
 

@@ -16,8 +16,8 @@ struct FixedSeg {
 };
 static_assert(sizeof(FixedSeg) == 24, "FixedSeg layout");
 
-// Rows each wavefront of the RLE kernel expands (one work unit).
-constexpr uint32_t RLE_UNIT_ROWS = 16384;
+// Rows of one RLE slice (one 256-thread block decodes a slice).
+constexpr uint32_t RLE_SLICE_ROWS = 16384;
 
 // Launch the decode of segments [seg_base, seg_base + n) of `d_segs` (device
 // array).  `max_rows` is the largest row_count among them (sizes the grid).
