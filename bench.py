@@ -84,6 +84,7 @@ def main() -> int:
     from sirius_amd import lib as L
 
     lib = L.load()
+    tuning = apply_tuning(lib)
     dev = torch.device("cuda:0")
     work = build_workload(args.workload, args.rows)
     rows = args.rows
@@ -200,12 +201,26 @@ def main() -> int:
         "kernels": {k: {"us": round(v["ms"] * 1e3, 2), "alg_bytes": v["bytes"], "gbs": round(v["gbs"], 1)}
                     for k, v in kern.items()},
         "bit_exact": bool(exact),
+        "tuning": tuning,
         "roofline": {"bound": "hbm", "kernel": dom[0], "achieved": round(dom[1]["gbs"], 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(dom[1]["gbs"] / HBM_PEAK_GBS, 4), "traffic": traffic},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
     return 0
+
+
+def apply_tuning(lib):
+    """Apply the autotuner's table (sirius_amd/tuning/gfx950.txt) if present;
+    returns the knob values in effect."""
+    import sirius_amd._abi as A
+    from sirius_amd import lib as L
+    from sirius_amd.autotune import DEFAULT_TABLE
+
+    if DEFAULT_TABLE.exists():
+        L.check(lib.sdr_tune_load(str(DEFAULT_TABLE).encode()))
+        log(f"[bench] tuning table {DEFAULT_TABLE.name} applied")
+    return {name: int(lib.sdr_tune_get(k)) for k, name in A.TUNE_NAMES.items()}
 
 
 def cpu_baseline(work, rows):

@@ -236,6 +236,35 @@ int sdr_batch_finish(sdr_batch* batch, sdr_stream_t stream);
 int sdr_batch_clear(sdr_batch* batch);
 int sdr_batch_destroy(sdr_batch* batch);
 
+/* ---- launch-shape tuning ------------------------------------------------ */
+
+/* Process-wide launch-shape knobs, read whenever kernels are launched, a
+ * plan is created or work is added to a batch.  They change speed only,
+ * never results.  No counterpart in the reference, whose launch shapes are
+ * compile-time constants (src/cuda/scan/gpu_decode_bitpacking.cu:394-427,
+ * gpu_decode_rle.cu:518-560, gpu_decode_alp.cu:680-743).  The autotuner
+ * (sirius_amd/autotune.py) measures candidates on the GPU and writes a
+ * table that sdr_tune_load applies. */
+typedef enum sdr_tune_key {
+  SDR_TUNE_RLE_SLICE_ROWS      = 1, /* rows one 256-thread block expands:
+                                       1024..1048576, multiple of 64 (16384) */
+  SDR_TUNE_BP_GROUPS_PER_WAVE  = 2, /* 2048-row groups one wavefront decodes
+                                       in turn, 1..16 (1)                    */
+  SDR_TUNE_BP_CHUNK_ROWS       = 3, /* most rows a wavefront stages per LDS
+                                       round: 512, 1024 or 2048 (2048)       */
+  SDR_TUNE_ALP_VECTORS_PER_WAVE = 4 /* 1024-row ALP/ALPRD vectors one
+                                       wavefront decodes in turn, 1..16 (1)  */
+} sdr_tune_key;
+
+/* Set a knob; SDR_E_INVALID_ARG for an unknown key or a value out of range. */
+int sdr_tune_set(uint32_t key, uint64_t value);
+/* Current value of a knob, 0 for an unknown key. */
+uint64_t sdr_tune_get(uint32_t key);
+/* Apply a table of "name value" lines (names: rle_slice_rows,
+ * bp_groups_per_wave, bp_chunk_rows, alp_vectors_per_wave; '#' starts a
+ * comment).  Stops at the first bad line with SDR_E_INVALID_ARG. */
+int sdr_tune_load(const char* path);
+
 /* ---- string decode ----------------------------------------------------- */
 
 /* Upper bound on decoded chars: sum(row_count * max_string_length), or

@@ -44,6 +44,14 @@ BATCH_USE_GRAPH = 1
 BATCH_GRID = 2
 BATCH_UNITS = {BITPACKING: 16, RLE: 17, ALP: 18, ALPRD: 19, UNCOMPRESSED: 20, CONSTANT: 21}
 
+# sdr_tune_key
+TUNE_RLE_SLICE_ROWS = 1
+TUNE_BP_GROUPS_PER_WAVE = 2
+TUNE_BP_CHUNK_ROWS = 3
+TUNE_ALP_VECTORS_PER_WAVE = 4
+TUNE_NAMES = {TUNE_RLE_SLICE_ROWS: "rle_slice_rows", TUNE_BP_GROUPS_PER_WAVE: "bp_groups_per_wave",
+              TUNE_BP_CHUNK_ROWS: "bp_chunk_rows", TUNE_ALP_VECTORS_PER_WAVE: "alp_vectors_per_wave"}
+
 
 class Segment(C.Structure):
     _fields_ = [

@@ -75,7 +75,7 @@ constexpr uint32_t BP_GROUP = 2048;
 // groups wider than that are staged in power-of-two row chunks.
 template <typename T, uint32_t STAGE_B = STAGE_BYTES>
 __device__ void bp_group(const uint8_t* seg, uint32_t bytes, uint32_t k, uint32_t grc, T* out,
-                         uint32_t* lds)
+                         uint32_t* lds, uint32_t max_chunk = BP_GROUP)
 {
   constexpr uint32_t TS  = sizeof(T);
   constexpr uint32_t VPL = 16 / TS;
@@ -141,10 +141,10 @@ __device__ void bp_group(const uint8_t* seg, uint32_t bytes, uint32_t k, uint32_
   }
 
   // FOR / DELTA_FOR: stage packed words through LDS in chunks of at most
-  // STAGE_B bytes of bits (a multiple of the 64 * VPL rows of one step).
+  // STAGE_B bytes of bits and at most max_chunk rows.
   static_assert(STAGE_B * 8u >= 64u * (BP_GROUP / 4u) && STAGE_B <= STAGE_BYTES, "stage size");
   const bool al          = (reinterpret_cast<uintptr_t>(out) & 15u) == 0;
-  uint32_t chunk         = BP_GROUP;
+  uint32_t chunk         = max_chunk;  // 512, 1024 or 2048
   while (chunk * width > STAGE_B * 8u) chunk >>= 1;
   const T tframe         = static_cast<T>(frame);
   const uint8_t* stream  = seg + doff;

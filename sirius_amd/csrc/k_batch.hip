@@ -64,8 +64,9 @@ __global__ __launch_bounds__(BLOCK_THREADS) void k_batch(const BatchItem* __rest
       }
       case BK_RLE:
         for (uint32_t u = it.unit0; u < it.unit0 + it.n_units; ++u) {
-          const uint32_t ra = u * RLE_SLICE_ROWS;
-          const uint32_t rb = it.row_count - ra < RLE_SLICE_ROWS ? it.row_count : ra + RLE_SLICE_ROWS;
+          const uint32_t sr = uint32_t{it.reserved} * 64u;
+          const uint32_t ra = u * sr;
+          const uint32_t rb = it.row_count - ra < sr ? it.row_count : ra + sr;
           rle_slice(it.seg, it.bytes_size, it.row_count, ra, rb, it.out, it.ts, sm.rle);
         }
         break;

@@ -4,39 +4,45 @@
 // blockIdx.x * 4 + wave the work unit (decoders in dec_fixed.hpp).
 #include "dec_fixed.hpp"
 #include "dec_rle.hpp"
+#include "sdr_tune.hpp"
 
 namespace sdr {
 
+// Wave w of block x decodes groups [(4x + w) * gpw, (4x + w + 1) * gpw).
 template <typename T>
 __global__ __launch_bounds__(BLOCK_THREADS) void k_bitpacking(const FixedSeg* __restrict__ segs,
-                                                              uint32_t seg_base)
+                                                              uint32_t seg_base, uint32_t gpw,
+                                                              uint32_t chunk_rows)
 {
   // 4 KiB of packed bits per wave keeps 8 waves per SIMD resident
   constexpr uint32_t SB = 4096;
   __shared__ uint32_t lds[WAVES_PER_BLOCK][(SB + 48) / 4];
   const FixedSeg sd  = segs[seg_base + blockIdx.y];
   const uint32_t w   = wave_in_block();
-  const uint32_t k   = blockIdx.x * WAVES_PER_BLOCK + w;
   const uint32_t ngr = (sd.row_count + BP_GROUP - 1) / BP_GROUP;
-  if (k >= ngr) return;
-  const uint32_t grc = sd.row_count - k * BP_GROUP < BP_GROUP ? sd.row_count - k * BP_GROUP : BP_GROUP;
-  bp_group<T, SB>(sd.bytes, sd.bytes_size, k, grc, reinterpret_cast<T*>(sd.out) + size_t{k} * BP_GROUP,
-                  lds[w]);
+  const uint32_t k0  = (blockIdx.x * WAVES_PER_BLOCK + w) * gpw;
+  for (uint32_t k = k0; k < k0 + gpw && k < ngr; ++k) {
+    const uint32_t grc = sd.row_count - k * BP_GROUP < BP_GROUP ? sd.row_count - k * BP_GROUP : BP_GROUP;
+    bp_group<T, SB>(sd.bytes, sd.bytes_size, k, grc, reinterpret_cast<T*>(sd.out) + size_t{k} * BP_GROUP,
+                    lds[w], chunk_rows);
+  }
 }
 
 template <typename T, bool RD>
-__global__ __launch_bounds__(BLOCK_THREADS) void k_alp(const FixedSeg* __restrict__ segs, uint32_t seg_base)
+__global__ __launch_bounds__(BLOCK_THREADS) void k_alp(const FixedSeg* __restrict__ segs, uint32_t seg_base,
+                                                       uint32_t vpw)
 {
   __shared__ uint32_t lds[WAVES_PER_BLOCK][ALP_STAGE_WORDS];
   const FixedSeg sd  = segs[seg_base + blockIdx.y];
   const uint32_t w   = wave_in_block();
-  const uint32_t v   = blockIdx.x * WAVES_PER_BLOCK + w;
   const uint32_t nv  = (sd.row_count + ALP_VEC - 1) / ALP_VEC;
-  if (v >= nv) return;
-  const uint32_t fill = sd.row_count - v * ALP_VEC < ALP_VEC ? sd.row_count - v * ALP_VEC : ALP_VEC;
-  T* out = reinterpret_cast<T*>(sd.out) + size_t{v} * ALP_VEC;
-  if constexpr (RD) alprd_vector<T>(sd.bytes, sd.bytes_size, v, fill, out, lds[w]);
-  else alp_vector<T>(sd.bytes, sd.bytes_size, v, fill, out, lds[w]);
+  const uint32_t v0  = (blockIdx.x * WAVES_PER_BLOCK + w) * vpw;
+  for (uint32_t v = v0; v < v0 + vpw && v < nv; ++v) {
+    const uint32_t fill = sd.row_count - v * ALP_VEC < ALP_VEC ? sd.row_count - v * ALP_VEC : ALP_VEC;
+    T* out = reinterpret_cast<T*>(sd.out) + size_t{v} * ALP_VEC;
+    if constexpr (RD) alprd_vector<T>(sd.bytes, sd.bytes_size, v, fill, out, lds[w]);
+    else alp_vector<T>(sd.bytes, sd.bytes_size, v, fill, out, lds[w]);
+  }
 }
 
 // blockIdx.x selects the slice of `slice_rows` rows (block-uniform exit).
@@ -135,15 +141,17 @@ hipError_t launch_bitpacking(const FixedSeg* d_segs, uint32_t seg_base, uint32_t
                              uint32_t ts, hipStream_t s)
 {
   if (n == 0 || max_rows == 0) return hipSuccess;
+  const uint32_t gpw   = tune(SDR_TUNE_BP_GROUPS_PER_WAVE);
+  const uint32_t chunk = tune(SDR_TUNE_BP_CHUNK_ROWS);
   const uint32_t units = (max_rows + BP_GROUP - 1) / BP_GROUP;
-  const uint32_t gx    = (units + WAVES_PER_BLOCK - 1) / WAVES_PER_BLOCK;
+  const uint32_t gx    = (units + WAVES_PER_BLOCK * gpw - 1) / (WAVES_PER_BLOCK * gpw);
   return chunked(seg_base, n, [&](uint32_t b, uint32_t m) {
     dim3 g(gx, m);
     switch (ts) {
-      case 1: hipLaunchKernelGGL(k_bitpacking<uint8_t>, g, dim3(BLOCK_THREADS), 0, s, d_segs, b); break;
-      case 2: hipLaunchKernelGGL(k_bitpacking<uint16_t>, g, dim3(BLOCK_THREADS), 0, s, d_segs, b); break;
-      case 4: hipLaunchKernelGGL(k_bitpacking<uint32_t>, g, dim3(BLOCK_THREADS), 0, s, d_segs, b); break;
-      default: hipLaunchKernelGGL(k_bitpacking<uint64_t>, g, dim3(BLOCK_THREADS), 0, s, d_segs, b); break;
+      case 1: hipLaunchKernelGGL(k_bitpacking<uint8_t>, g, dim3(BLOCK_THREADS), 0, s, d_segs, b, gpw, chunk); break;
+      case 2: hipLaunchKernelGGL(k_bitpacking<uint16_t>, g, dim3(BLOCK_THREADS), 0, s, d_segs, b, gpw, chunk); break;
+      case 4: hipLaunchKernelGGL(k_bitpacking<uint32_t>, g, dim3(BLOCK_THREADS), 0, s, d_segs, b, gpw, chunk); break;
+      default: hipLaunchKernelGGL(k_bitpacking<uint64_t>, g, dim3(BLOCK_THREADS), 0, s, d_segs, b, gpw, chunk); break;
     }
   });
 }
@@ -152,7 +160,7 @@ hipError_t launch_rle(const FixedSeg* d_segs, uint32_t seg_base, uint32_t n, uin
                       hipStream_t s)
 {
   if (n == 0 || max_rows == 0) return hipSuccess;
-  const uint32_t sr = RLE_SLICE_ROWS;
+  const uint32_t sr = tune(SDR_TUNE_RLE_SLICE_ROWS);
   const uint32_t gx = (max_rows + sr - 1) / sr;
   return chunked(seg_base, n, [&](uint32_t b, uint32_t m) {
     hipLaunchKernelGGL(k_rle, dim3(gx, m), dim3(RLE_THREADS), 0, s, d_segs, b, sr, ts);
@@ -163,16 +171,17 @@ hipError_t launch_alp(const FixedSeg* d_segs, uint32_t seg_base, uint32_t n, uin
                       bool rd, hipStream_t s)
 {
   if (n == 0 || max_rows == 0) return hipSuccess;
+  const uint32_t vpw   = tune(SDR_TUNE_ALP_VECTORS_PER_WAVE);
   const uint32_t units = (max_rows + ALP_VEC - 1) / ALP_VEC;
-  const uint32_t gx    = (units + WAVES_PER_BLOCK - 1) / WAVES_PER_BLOCK;
+  const uint32_t gx    = (units + WAVES_PER_BLOCK * vpw - 1) / (WAVES_PER_BLOCK * vpw);
   return chunked(seg_base, n, [&](uint32_t b, uint32_t m) {
     dim3 g(gx, m);
     if (ts == 8) {
-      if (rd) hipLaunchKernelGGL((k_alp<double, true>), g, dim3(BLOCK_THREADS), 0, s, d_segs, b);
-      else hipLaunchKernelGGL((k_alp<double, false>), g, dim3(BLOCK_THREADS), 0, s, d_segs, b);
+      if (rd) hipLaunchKernelGGL((k_alp<double, true>), g, dim3(BLOCK_THREADS), 0, s, d_segs, b, vpw);
+      else hipLaunchKernelGGL((k_alp<double, false>), g, dim3(BLOCK_THREADS), 0, s, d_segs, b, vpw);
     } else {
-      if (rd) hipLaunchKernelGGL((k_alp<float, true>), g, dim3(BLOCK_THREADS), 0, s, d_segs, b);
-      else hipLaunchKernelGGL((k_alp<float, false>), g, dim3(BLOCK_THREADS), 0, s, d_segs, b);
+      if (rd) hipLaunchKernelGGL((k_alp<float, true>), g, dim3(BLOCK_THREADS), 0, s, d_segs, b, vpw);
+      else hipLaunchKernelGGL((k_alp<float, false>), g, dim3(BLOCK_THREADS), 0, s, d_segs, b, vpw);
     }
   });
 }

@@ -4,6 +4,7 @@
 // and launches the persistent kernel, optionally as a replayed hipGraph.
 #include "sdr_batch.hpp"
 #include "sdr_internal.hpp"
+#include "sdr_tune.hpp"
 
 #include <algorithm>
 #include <cstring>
@@ -85,11 +86,11 @@ BatchKind kind_of(uint32_t codec)
   }
 }
 
-uint32_t units_in_segment(BatchKind k, uint32_t rows, uint32_t ts)
+uint32_t units_in_segment(BatchKind k, uint32_t rows, uint32_t ts, uint32_t rle_slice)
 {
   switch (k) {
     case BK_BITPACKING: return (rows + 2047u) / 2048u;
-    case BK_RLE: return (rows + RLE_SLICE_ROWS - 1u) / RLE_SLICE_ROWS;
+    case BK_RLE: return (rows + rle_slice - 1u) / rle_slice;
     case BK_ALP:
     case BK_ALPRD: return (rows + 1023u) / 1024u;
     default: return static_cast<uint32_t>((uint64_t{rows} * ts + COPY_UNIT_BYTES - 1u) / COPY_UNIT_BYTES);
@@ -115,10 +116,11 @@ int batch_add_fixed(sdr_batch* b, sdr_column* cols, uint32_t n_cols)
       const sdr_run& run = col.data[r];
       const BatchKind k  = kind_of(run.codec);
       const uint32_t per = std::max<uint32_t>(1, b->units[k]);
+      const uint32_t rle_slice = tune(SDR_TUNE_RLE_SLICE_ROWS);
       for (uint32_t s = 0; s < run.n_segments; ++s) {
         const sdr_segment& g = run.segments[s];
         if (!g.row_count) continue;
-        const uint32_t nu = units_in_segment(k, g.row_count, col.type_size);
+        const uint32_t nu = units_in_segment(k, g.row_count, col.type_size, rle_slice);
         for (uint32_t u = 0; u < nu; u += per) {
           BatchItem it{};
           it.seg        = g.d_bytes;
@@ -129,6 +131,7 @@ int batch_add_fixed(sdr_batch* b, sdr_column* cols, uint32_t n_cols)
           it.n_units    = std::min(per, nu - u);
           it.kind       = k;
           it.ts         = static_cast<uint8_t>(col.type_size);
+          if (k == BK_RLE) it.reserved = static_cast<uint16_t>(rle_slice / 64u);
           b->items.push_back(it);
         }
       }

@@ -46,7 +46,7 @@ struct BatchItem {
   uint32_t n_units;
   uint8_t kind;
   uint8_t ts;
-  uint16_t reserved;
+  uint16_t reserved;          // RLE: slice rows / 64
   uint32_t chunk;             // FSST: chunk index in the column chain
   const FsstColumn* column;   // FSST
   uint64_t pad[2];

@@ -22,6 +22,7 @@ DECLARED = [
     "sdr_plan_null_counts", "sdr_plan_destroy",
     "sdr_batch_create", "sdr_batch_set_option", "sdr_batch_add_table", "sdr_batch_add_strings",
     "sdr_batch_num_entries", "sdr_batch_launch", "sdr_batch_finish", "sdr_batch_clear", "sdr_batch_destroy",
+    "sdr_tune_set", "sdr_tune_get", "sdr_tune_load",
 ]
 
 _lib = None
@@ -76,6 +77,12 @@ def load() -> C.CDLL:
     lib.sdr_plan_null_counts.restype = i32
     lib.sdr_plan_destroy.argtypes = [vp]
     lib.sdr_plan_destroy.restype = i32
+    lib.sdr_tune_set.argtypes = [u32, C.c_uint64]
+    lib.sdr_tune_set.restype = i32
+    lib.sdr_tune_get.argtypes = [u32]
+    lib.sdr_tune_get.restype = C.c_uint64
+    lib.sdr_tune_load.argtypes = [C.c_char_p]
+    lib.sdr_tune_load.restype = i32
     lib.sdr_batch_create.argtypes = [P(vp)]
     lib.sdr_batch_set_option.argtypes = [vp, u32, C.c_int64]
     lib.sdr_batch_add_table.argtypes = [vp, P(_abi.Column), u32]

@@ -92,3 +92,32 @@ def test_strings_upper_bound():
     segs[1].max_string_length = 0
     runs, keep = A.make_runs([(A.FSST, segs)])
     assert L.load().sdr_strings_upper_bound(runs, 1) == 2 ** 64 - 1
+
+
+def test_tuning_knobs_set_get_load(tmp_path):
+    lib = L.load()
+    defaults = {k: lib.sdr_tune_get(k) for k in A.TUNE_NAMES}
+    assert defaults[A.TUNE_RLE_SLICE_ROWS] == 16384 and defaults[A.TUNE_BP_CHUNK_ROWS] == 2048
+    try:
+        assert lib.sdr_tune_set(A.TUNE_RLE_SLICE_ROWS, 8192) == A.SDR_OK
+        assert lib.sdr_tune_get(A.TUNE_RLE_SLICE_ROWS) == 8192
+        # out of range or not a multiple of 64, unknown key, bad chunk size
+        assert lib.sdr_tune_set(A.TUNE_RLE_SLICE_ROWS, 1000) == A.SDR_E_INVALID_ARG
+        assert lib.sdr_tune_set(A.TUNE_RLE_SLICE_ROWS, 8200) == A.SDR_E_INVALID_ARG
+        assert lib.sdr_tune_set(99, 1) == A.SDR_E_INVALID_ARG
+        assert lib.sdr_tune_set(A.TUNE_BP_CHUNK_ROWS, 768) == A.SDR_E_INVALID_ARG
+        assert lib.sdr_tune_get(A.TUNE_RLE_SLICE_ROWS) == 8192 and lib.sdr_tune_get(99) == 0
+        t = tmp_path / "tune.txt"
+        t.write_text("# table\nrle_slice_rows 32768\n\nbp_groups_per_wave 2  # two groups\n")
+        assert lib.sdr_tune_load(str(t).encode()) == A.SDR_OK
+        assert lib.sdr_tune_get(A.TUNE_RLE_SLICE_ROWS) == 32768
+        assert lib.sdr_tune_get(A.TUNE_BP_GROUPS_PER_WAVE) == 2
+        t.write_text("rle_slice_rows 4096\nno_such_knob 3\nbp_chunk_rows 512\n")
+        assert lib.sdr_tune_load(str(t).encode()) == A.SDR_E_INVALID_ARG
+        assert b"bad tuning line" in lib.sdr_last_error()
+        assert lib.sdr_tune_get(A.TUNE_RLE_SLICE_ROWS) == 4096  # lines before the bad one applied
+        assert lib.sdr_tune_get(A.TUNE_BP_CHUNK_ROWS) == 2048
+        assert lib.sdr_tune_load(str(tmp_path / "missing.txt").encode()) == A.SDR_E_NOT_FOUND
+    finally:
+        for k, v in defaults.items():
+            assert lib.sdr_tune_set(k, v) == A.SDR_OK
