@@ -1,15 +1,21 @@
 #!/usr/bin/env python3
 """Decode throughput of DuckDB storage segments on one MI355X.
 
-Default workload (the configuration the headline metric is quoted on):
-"int_date_mix_64M" — 64 Mi rows of three columns, DuckDB-format segments
+Headline workload ("int_date_mix_64M", the integer/date mix the metric is
+quoted on): 64 Mi rows of three columns in DuckDB-format segments
 (self-encoded; DuckDB is not installed): an INT64 column of runs (geometric
 length, mean 16, values in [0, 2^40)) encoded RLE, an INT32 quantity column
 (1..50) and a DATE column (7 years, sorted in 2048-row groups), both
 BITPACKING.  Inputs are resident in HBM before timing starts.
 
 A step is one decode of the three columns through sdr_decode_table (the
-drop-in entry, batched launches).  Prints one JSON line.
+drop-in entry, batched launches).  Besides the headline the run reports the
+same columns through the persistent batch and the plain one-launch-per-
+segment path (p50/p99 step latency of each), per-kernel durations measured
+with HIP events inside a timed loop on the launch stream, the roofline of
+the dominant kernel, the other single-GPU workloads (8M-row bitpacking,
+ALP/ALP-RD doubles, FSST strings) and the CPU baseline.  Prints one JSON
+line.
 """
 from __future__ import annotations
 
@@ -28,186 +34,231 @@ sys.path.insert(0, str(ROOT))
 
 METRIC = "decoded GB/s and % HBM roofline vs DuckDB CPU; p50/p99 latency batched vs plain"
 HBM_PEAK_GBS = 8000.0
+WORKLOADS = ("int_date_mix_64M", "copy_256M", "bp_int32_8M", "alp_double_32M", "fsst_varchar_16M")
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-WORKLOADS = ("int_date_mix_64M", "copy_256M")
-
-
-def build_workload(name: str, rows: int):
+# ---------------------------------------------------------------------------
+# workloads: (name, values, Encoded) per column; strings carry (chars, offsets)
+# ---------------------------------------------------------------------------
+def build_workload(name: str, rows: int | None = None):
     import sirius_amd.encode as E
 
     t = time.time()
-    if name == "copy_256M":
-        # calibration: UNCOMPRESSED INT64 (a plain copy of known size)
+    if name == "copy_256M":  # calibration: UNCOMPRESSED INT64, a plain copy of known size
+        rows = rows or 32 << 20
         cols = [("uncompressed_int64", E.gen_u64(0xC0DE00FF, rows).view(np.int64), E.encode_uncompressed)]
-    else:
+    elif name == "int_date_mix_64M":
+        rows = rows or 64 << 20
         cols = [
             ("rle_int64_runs", E.gen_runs_int64(0xC0DE0002, rows, 16.0, 1 << 40), E.encode_rle),
             ("bp_int32_quantity", E.gen_uniform_int32(0xC0DE0012, rows, 1, 50), E.encode_bitpacking),
             ("bp_date_sorted", E.gen_dates_sorted(0xC0DE0022, rows), E.encode_bitpacking),
         ]
+    elif name == "bp_int32_8M":
+        rows = rows or 8 << 20
+        cols = [("bp_int32_w17_w5", E.gen_bp_int32(0xC0DE0001, rows), E.encode_bitpacking)]
+    elif name == "alp_double_32M":
+        rows = rows or 32 << 20
+        cols = [("alp_double", E.gen_alp_doubles(0xC0DE0003, rows), E.encode_alp),
+                ("alprd_double", E.gen_unit_doubles(0xC0DE0013, rows // 4), E.encode_alprd)]
+    elif name == "fsst_varchar_16M":
+        rows = rows or 16 << 20
+        chars, offs = E.gen_zipf_strings(0xC0DE0004, rows)
+        enc = E.encode_fsst(chars, offs)
+        log(f"[bench] encoded fsst_varchar: {len(enc.segs)} segments, {enc.compressed_bytes / 1e6:.1f} MB "
+            f"compressed, {len(chars) / 1e6:.1f} MB of chars")
+        return [("fsst_varchar", (chars, offs), enc)]
+    else:
+        raise ValueError(name)
     out = []
-    for name, values, enc_f in cols:
+    for cname, values, enc_f in cols:
         enc = enc_f(values)
-        out.append((name, values, enc))
-        log(f"[bench] encoded {name}: {len(enc.segs)} segments, {enc.compressed_bytes / 1e6:.1f} MB "
+        out.append((cname, values, enc))
+        log(f"[bench] encoded {cname}: {len(enc.segs)} segments, {enc.compressed_bytes / 1e6:.1f} MB "
             f"compressed, {values.nbytes / 1e6:.1f} MB decoded")
-    log(f"[bench] workload built in {time.time() - t:.1f}s")
+    log(f"[bench] workload {name} built in {time.time() - t:.1f}s")
     return out
 
 
-def main() -> int:
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
-    ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--rows", type=int, default=None)
-    ap.add_argument("--workload", choices=WORKLOADS, default="int_date_mix_64M")
-    ap.add_argument("--only", choices=("all", "batched"), default="all",
-                    help="batched: time only the drop-in path (for profiler runs)")
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    args = ap.parse_args()
-    if args.rows is None:
-        args.rows = (32 << 20) if args.workload == "copy_256M" else (64 << 20)
-    if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (this build decodes on a "
-              f"single MI355X; multi-GPU sharding is described in DESIGN.md)", file=sys.stderr)
-        return 2
+class Resident:
+    """Encoded columns uploaded to HBM with their output buffers."""
 
+    def __init__(self, work, dev):
+        import torch
+
+        import sirius_amd._abi as A
+
+        self.keep, self.fixed, self.outs, self.values = [], [], [], []
+        self.strings = []  # (StringColumn, d_offs, d_chars, (chars, offs))
+        self.comp_bytes = self.dec_bytes = 0
+        self.per_codec = {}  # codec -> [compressed, decoded] bytes
+        for _, values, enc in work:
+            blob = torch.from_numpy(enc.blob).to(dev)
+            self.keep.append(blob)
+            runs, k = A.make_runs([(enc.codec, enc.segments(blob.data_ptr()))])
+            self.keep.append(k)
+            self.comp_bytes += enc.compressed_bytes
+            if enc.codec == A.FSST:
+                chars, offs = values
+                n = enc.n_rows
+                d_offs = torch.empty(n + 1, dtype=torch.int32, device=dev)
+                d_chars = torch.empty(len(chars) + 64, dtype=torch.uint8, device=dev)
+                col = A.StringColumn()
+                col.total_rows, col.data, col.n_data = n, runs, 1
+                col.d_offsets, col.d_chars, col.chars_capacity = d_offs.data_ptr(), d_chars.data_ptr(), len(chars) + 64
+                self.strings.append((col, d_offs, d_chars, values))
+                dec = len(chars) + 4 * (n + 1)
+            else:
+                out = torch.empty(enc.n_rows * enc.type_size, dtype=torch.uint8, device=dev)
+                col = A.Column()
+                col.type_size, col.total_rows, col.data, col.n_data, col.d_out = (
+                    enc.type_size, enc.n_rows, runs, 1, out.data_ptr())
+                self.fixed.append(col)
+                self.outs.append(out)
+                self.values.append(values)
+                dec = enc.n_rows * enc.type_size
+            self.dec_bytes += dec
+            cb = self.per_codec.setdefault(enc.codec, [0, 0])
+            cb[0] += enc.compressed_bytes
+            cb[1] += dec
+        self.n_fixed = len(self.fixed)
+        # a table call decodes columns of one row count: one table per count
+        self.tables = []
+        for n in sorted({c.total_rows for c in self.fixed}):
+            cs = [c for c in self.fixed if c.total_rows == n]
+            self.tables.append(((A.Column * len(cs))(*cs), len(cs)))
+
+    def decode(self, lib, sh, mode):
+        from sirius_amd import lib as L
+
+        for arr, n in self.tables:
+            L.check(lib.sdr_decode_table_mode(arr, n, sh, mode))
+        for col, *_ in self.strings:
+            L.check(lib.sdr_decode_strings_column_mode(C.byref(col), sh, mode))
+
+    def bit_exact(self) -> bool:
+        ok = all(np.array_equal(o.cpu().numpy(), v.view(np.uint8)) for o, v in zip(self.outs, self.values))
+        for col, d_offs, d_chars, (chars, offs) in self.strings:
+            ok &= col.chars_used == len(chars)
+            ok &= np.array_equal(d_offs.cpu().numpy().astype(np.uint64), offs)
+            ok &= np.array_equal(d_chars[: len(chars)].cpu().numpy(), chars)
+        return bool(ok)
+
+
+def time_steps(fn, steps, warmup, stream):
+    """Per-step durations (ms) from HIP events on `stream`, plus the total."""
+    import torch
+
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    evs = [torch.cuda.Event(enable_timing=True) for _ in range(steps + 1)]
+    evs[0].record(stream)
+    for i in range(steps):
+        fn()
+        evs[i + 1].record(stream)
+    torch.cuda.synchronize()
+    per = [evs[i].elapsed_time(evs[i + 1]) for i in range(steps)]
+    return per, evs[0].elapsed_time(evs[-1])
+
+
+def pct(xs, q):
+    return float(np.percentile(np.asarray(xs), q))
+
+
+def kernel_times(lib, res, sh, stream, steps, warmup):
+    """Average duration (ms) of each codec's kernels inside a timed loop of
+    whole decode steps (a prepared plan, one codec's launches bracketed by
+    HIP events on the launch stream)."""
     import torch
 
     import sirius_amd._abi as A
     from sirius_amd import lib as L
 
-    lib = L.load()
-    tuning = apply_tuning(lib)
-    dev = torch.device("cuda:0")
-    work = build_workload(args.workload, args.rows)
-    rows = args.rows
+    plans = []
+    try:
+        for arr, n in res.tables:
+            plan = C.c_void_p()
+            L.check(lib.sdr_plan_create(arr, n, A.LAUNCH_BATCHED, sh, C.byref(plan)))
+            plans.append(plan)
+        codecs = sorted(c for c in res.per_codec if c != A.FSST)
+        for _ in range(warmup):
+            for plan in plans:
+                L.check(lib.sdr_plan_launch(plan, sh))
+        torch.cuda.synchronize()
+        evs = {c: [] for c in codecs}
+        for _ in range(steps):
+            for c in codecs:
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record(stream)
+                for plan in plans:
+                    L.check(lib.sdr_plan_launch_codec(plan, c, sh))
+                e1.record(stream)
+                evs[c].append((e0, e1))
+        torch.cuda.synchronize()
+        return {c: float(np.mean([a.elapsed_time(b) for a, b in evs[c]])) for c in codecs}
+    finally:
+        for plan in plans:
+            lib.sdr_plan_destroy(plan)
 
-    keep, cols, outs = [], [], []
-    comp_bytes = dec_bytes = 0
-    per_codec_bytes = {}
-    for name, values, enc in work:
-        blob = torch.from_numpy(enc.blob).to(dev)
-        out = torch.empty(rows * enc.type_size, dtype=torch.uint8, device=dev)
-        runs, k = A.make_runs([(enc.codec, enc.segments(blob.data_ptr()))])
-        col = A.Column()
-        col.type_size, col.total_rows, col.data, col.n_data, col.d_out = enc.type_size, rows, runs, 1, out.data_ptr()
-        keep += [blob, k]
-        cols.append(col)
-        outs.append(out)
-        comp_bytes += enc.compressed_bytes
-        dec_bytes += rows * enc.type_size
-        key = (enc.codec, enc.type_size)
-        cb, db = per_codec_bytes.get(key, (0, 0))
-        per_codec_bytes[key] = (cb + enc.compressed_bytes, db + rows * enc.type_size)
-    arr = (A.Column * len(cols))(*cols)
+
+def persistent_times(lib, res, sh, stream, steps, warmup):
+    """The same columns through one persistent, graph-replayed launch."""
+    from sirius_amd import lib as L
+
+    b = C.c_void_p()
+    L.check(lib.sdr_batch_create(C.byref(b)))
+    try:
+        def step():
+            L.check(lib.sdr_batch_clear(b))
+            for arr, n in res.tables:
+                L.check(lib.sdr_batch_add_table(b, arr, n))
+            for col, *_ in res.strings:
+                L.check(lib.sdr_batch_add_strings(b, C.byref(col)))
+            L.check(lib.sdr_batch_launch(b, sh))
+
+        per, _ = time_steps(step, steps, warmup, stream)
+        L.check(lib.sdr_batch_finish(b, sh))
+        return per
+    finally:
+        lib.sdr_batch_destroy(b)
+
+
+def measure(lib, work, dev, steps, warmup, plain_steps, full=True):
+    import torch
+
+    import sirius_amd._abi as A
+
     stream = torch.cuda.current_stream()
     sh = stream.cuda_stream
-
-    def timed(fn, steps, warmup):
-        for _ in range(warmup):
-            fn()
-        torch.cuda.synchronize()
-        ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        t0 = time.perf_counter()
-        ev0.record(stream)
-        for _ in range(steps):
-            fn()
-        ev1.record(stream)
-        torch.cuda.synchronize()
-        wall = (time.perf_counter() - t0) * 1e3 / steps
-        return ev0.elapsed_time(ev1) / steps, wall
-
-    def step_batched():
-        L.check(lib.sdr_decode_table_mode(arr, len(cols), sh, A.LAUNCH_BATCHED))
-
-    def step_plain():
-        L.check(lib.sdr_decode_table_mode(arr, len(cols), sh, A.LAUNCH_PER_SEGMENT))
-
-    log("[bench] timing batched path")
-    ms_batched, wall_batched = timed(step_batched, args.steps, args.warmup)
-    # correctness of what was timed: bit-exact against the generated values
-    torch.cuda.synchronize()
-    exact = all(np.array_equal(out.cpu().numpy(), values.view(np.uint8)) for (_, values, _), out in zip(work, outs))
-    log(f"[bench] batched {ms_batched:.3f} ms/step, bit-exact={exact}")
-    if args.only == "batched":
-        print(json.dumps({"workload": args.workload, "ms_per_step": ms_batched, "bit_exact": bool(exact),
-                          "decoded_bytes": dec_bytes, "compressed_bytes": comp_bytes}), flush=True)
-        return 0
-    log("[bench] timing plain (one launch per segment) path")
-    ms_plain, wall_plain = timed(step_plain, max(5, args.steps // 5), 2)
-    log(f"[bench] plain {ms_plain:.3f} ms/step")
-
-    # prepared plan: descriptors resident, launch-only steps; per-kernel timing
-    plan = C.c_void_p()
-    L.check(lib.sdr_plan_create(arr, len(cols), A.LAUNCH_BATCHED, sh, C.byref(plan)))
-    ms_plan, _ = timed(lambda: L.check(lib.sdr_plan_launch(plan, sh)), args.steps, args.warmup)
-    kern = {}
-    for (codec, ts), (cb, db) in per_codec_bytes.items():
-        ms_k, _ = timed(lambda c=codec: L.check(lib.sdr_plan_launch_codec(plan, c, sh)), args.steps, args.warmup)
-        kern[A.CODEC_NAMES[codec]] = dict(ms=ms_k, bytes=cb + db, gbs=(cb + db) / (ms_k * 1e-3) / 1e9)
-    lib.sdr_plan_destroy(plan)
-    dom = max(kern.items(), key=lambda kv: kv[1]["ms"])
-    log(f"[bench] plan {ms_plan:.3f} ms/step; kernels: " +
-        ", ".join(f"{k} {v['ms']*1e3:.1f}us {v['gbs']:.0f}GB/s" for k, v in kern.items()))
-
-    value = dec_bytes / (ms_batched * 1e-3) / 1e9
-    achieved_job = (comp_bytes + dec_bytes) / (ms_batched * 1e-3) / 1e9
-
-    traffic = None
-    tf = ROOT / "profiles" / "traffic_int_date_mix_64M.json"
-    if tf.exists():
-        try:
-            traffic = json.loads(tf.read_text()).get(dom[0])
-        except Exception:
-            traffic = None
-
-    cpu = None
-    if not args.no_cpu_baseline:
-        cpu = cpu_baseline(work, rows)
-
-    line = {
-        "metric": METRIC,
-        "value": round(value, 2),
-        "unit": "GB/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(ms_batched, 4),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "int64+int32",
-        "data": "synthetic (DuckDB-format segments, self-encoded; DuckDB absent)",
-        "config": {"workload": args.workload, "rows": rows,
-                   "columns": ["RLE INT64 runs(mean 16)", "BITPACKING INT32 1..50", "BITPACKING DATE sorted/2048"],
-                   "segments": int(sum(len(e.segs) for _, _, e in work)),
-                   "compressed_bytes": comp_bytes, "decoded_bytes": dec_bytes},
-        "rows_per_s": round(rows / (ms_batched * 1e-3), 1),
-        "job_achieved_gbs": round(achieved_job, 2),
-        "job_frac_hbm": round(achieved_job / HBM_PEAK_GBS, 4),
-        "plain_ms_per_step": round(ms_plain, 4),
-        "plain_gbs": round(dec_bytes / (ms_plain * 1e-3) / 1e9, 2),
-        "batched_vs_plain": round(ms_plain / ms_batched, 3),
-        "plan_ms_per_step": round(ms_plan, 4),
-        "plan_gbs": round(dec_bytes / (ms_plan * 1e-3) / 1e9, 2),
-        "wall_ms_per_step": round(wall_batched, 4),
-        "kernels": {k: {"us": round(v["ms"] * 1e3, 2), "alg_bytes": v["bytes"], "gbs": round(v["gbs"], 1)}
-                    for k, v in kern.items()},
-        "bit_exact": bool(exact),
-        "tuning": tuning,
-        "roofline": {"bound": "hbm", "kernel": dom[0], "achieved": round(dom[1]["gbs"], 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(dom[1]["gbs"] / HBM_PEAK_GBS, 4), "traffic": traffic},
-        "cpu_baseline": cpu,
-    }
-    print(json.dumps(line), flush=True)
-    return 0
+    res = Resident(work, dev)
+    r = {"rows": int(sum(e.n_rows for _, _, e in work)), "segments": int(sum(len(e.segs) for _, _, e in work)),
+         "compressed_bytes": res.comp_bytes, "decoded_bytes": res.dec_bytes}
+    per, tot = time_steps(lambda: res.decode(lib, sh, A.LAUNCH_BATCHED), steps, warmup, stream)
+    r["bit_exact"] = res.bit_exact()
+    ms = tot / steps
+    r.update(ms_per_step=ms, decoded_gbs=res.dec_bytes / (ms * 1e-3) / 1e9,
+             achieved_gbs=(res.comp_bytes + res.dec_bytes) / (ms * 1e-3) / 1e9,
+             p50_ms=pct(per, 50), p99_ms=pct(per, 99))
+    if not full:
+        return r, res
+    pp = persistent_times(lib, res, sh, stream, steps, warmup)
+    r["persistent"] = dict(ms_per_step=float(np.mean(pp)), p50_ms=pct(pp, 50), p99_ms=pct(pp, 99),
+                           decoded_gbs=res.dec_bytes / (np.mean(pp) * 1e-3) / 1e9, bit_exact=res.bit_exact())
+    pl, tot = time_steps(lambda: res.decode(lib, sh, A.LAUNCH_PER_SEGMENT), plain_steps, 1, stream)
+    r["plain"] = dict(ms_per_step=tot / plain_steps, p50_ms=pct(pl, 50), p99_ms=pct(pl, 99),
+                      decoded_gbs=res.dec_bytes / (tot / plain_steps * 1e-3) / 1e9, bit_exact=res.bit_exact())
+    r["batched_vs_plain"] = r["plain"]["ms_per_step"] / ms
+    if res.n_fixed:
+        kt = kernel_times(lib, res, sh, stream, steps, warmup)
+        r["kernels"] = {A.CODEC_NAMES[c]: dict(us=kt[c] * 1e3, alg_bytes=sum(res.per_codec[c]),
+                                               gbs=sum(res.per_codec[c]) / (kt[c] * 1e-3) / 1e9)
+                        for c in kt}
+    return r, res
 
 
 def apply_tuning(lib):
@@ -223,23 +274,135 @@ def apply_tuning(lib):
     return {name: int(lib.sdr_tune_get(k)) for k, name in A.TUNE_NAMES.items()}
 
 
-def cpu_baseline(work, rows):
+def rounded(x, nd=4):
+    if isinstance(x, float):
+        return round(x, nd)
+    if isinstance(x, dict):
+        return {k: rounded(v, nd) for k, v in x.items()}
+    return x
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--rows", type=int, default=None)
+    ap.add_argument("--workload", choices=WORKLOADS, default="int_date_mix_64M")
+    ap.add_argument("--only", choices=("all", "batched"), default="all",
+                    help="batched: the drop-in steps and the per-kernel loop only (for profiler runs)")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-extra", action="store_true", help="skip the other workloads")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported: the decode path runs on a "
+              f"single MI355X here (how it shards across GPUs is described in DESIGN.md)", file=sys.stderr)
+        return 2
+
+    import torch
+
+    import sirius_amd._abi as A
+    from sirius_amd import lib as L
+
+    lib = L.load()
+    tuning = apply_tuning(lib)
+    dev = torch.device("cuda:0")
+    work = build_workload(args.workload, args.rows)
+
+    if args.only == "batched":
+        r, res = measure(lib, work, dev, args.steps, args.warmup, 0, full=False)
+        stream = torch.cuda.current_stream()
+        kt = kernel_times(lib, res, stream.cuda_stream, stream, args.steps, args.warmup) if res.n_fixed else {}
+        r["kernels_us"] = {A.CODEC_NAMES[c]: v * 1e3 for c, v in kt.items()}
+        print(json.dumps(rounded({"workload": args.workload, **r})), flush=True)
+        return 0
+
+    log(f"[bench] measuring {args.workload}")
+    r, res = measure(lib, work, dev, args.steps, args.warmup, max(3, args.steps // 10))
+    del res
+    log(f"[bench] {args.workload}: batched {r['ms_per_step']:.3f} ms/step, plain {r['plain']['ms_per_step']:.3f}, "
+        f"persistent {r['persistent']['ms_per_step']:.3f}, bit-exact={r['bit_exact']}")
+
+    dom_name, dom = max(r["kernels"].items(), key=lambda kv: kv[1]["us"]) if "kernels" in r else (None, None)
+    traffic = None
+    tf = ROOT / "profiles" / f"traffic_{args.workload}.json"
+    if dom_name and tf.exists():
+        traffic = json.loads(tf.read_text()).get(dom_name)
+
+    extra = {}
+    if not args.no_extra and args.workload == "int_date_mix_64M":
+        for wl in ("bp_int32_8M", "alp_double_32M", "fsst_varchar_16M"):
+            log(f"[bench] measuring {wl}")
+            w = build_workload(wl)
+            er, eres = measure(lib, w, dev, max(5, args.steps // 2), 2, 3)
+            del eres
+            extra[wl] = er
+            log(f"[bench] {wl}: batched {er['ms_per_step']:.3f} ms/step, {er['decoded_gbs']:.0f} GB/s, "
+                f"bit-exact={er['bit_exact']}")
+            del w
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(work)
+
+    cols = {"int_date_mix_64M": ["RLE INT64 runs(mean 16)", "BITPACKING INT32 1..50", "BITPACKING DATE sorted/2048"],
+            "copy_256M": ["UNCOMPRESSED INT64"]}.get(args.workload, [n for n, _, _ in work])
+    line = {
+        "metric": METRIC,
+        "value": round(r["decoded_gbs"], 2),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(r["ms_per_step"], 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": {"copy_256M": "int64"}.get(args.workload, "int64+int32"),
+        "data": "synthetic (DuckDB-format segments, self-encoded; DuckDB absent)",
+        "config": {"workload": args.workload, "rows": r["rows"] // len(work), "columns": cols,
+                   "segments": r["segments"],
+                   "compressed_bytes": r["compressed_bytes"], "decoded_bytes": r["decoded_bytes"],
+                   "path": "sdr_decode_table (batched launches)"},
+        "rows_per_s": round(r["rows"] / len(work) / (r["ms_per_step"] * 1e-3), 1),
+        "job_achieved_gbs": round(r["achieved_gbs"], 2),
+        "job_frac_hbm": round(r["achieved_gbs"] / HBM_PEAK_GBS, 4),
+        "latency_ms": {"batched": {"p50": round(r["p50_ms"], 4), "p99": round(r["p99_ms"], 4)},
+                       "persistent": {"p50": round(r["persistent"]["p50_ms"], 4),
+                                      "p99": round(r["persistent"]["p99_ms"], 4)},
+                       "plain": {"p50": round(r["plain"]["p50_ms"], 4), "p99": round(r["plain"]["p99_ms"], 4)}},
+        "persistent": rounded(r["persistent"]),
+        "plain": rounded(r["plain"]),
+        "batched_vs_plain": round(r["batched_vs_plain"], 3),
+        "kernels": rounded(r.get("kernels", {}), 2),
+        "bit_exact": r["bit_exact"] and r["plain"]["bit_exact"] and r["persistent"]["bit_exact"],
+        "tuning": tuning,
+        "workloads": {k: rounded({kk: vv for kk, vv in v.items() if kk != "kernels"}) for k, v in extra.items()},
+        "roofline": None if dom is None else {
+            "bound": "hbm", "kernel": dom_name, "achieved": round(dom["gbs"], 1), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(dom["gbs"] / HBM_PEAK_GBS, 4), "traffic": traffic,
+            "alg_bytes": dom["alg_bytes"], "avg_us": round(dom["us"], 2)},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+def cpu_baseline(work):
     """The oracle (C restatement of the decoders, 'port') on the host cores,
     on a bounded sample of the same workload: the first segments of each
-    column up to ~8M rows, repeated until ~10 s of CPU time."""
+    column up to 8 Mi rows, repeated for about 10 s."""
     import sirius_amd._abi as A
     from oracle import oracle as O
 
     lib = O.load()
-    threads = os.cpu_count() or 1
-    sample_rows = min(rows, 8 << 20)
+    threads = min(16, os.cpu_count() or 1)
+    sample_rows = 8 << 20
     cols, keep, nbytes = [], [], 0
-    for name, values, enc in work:
+    for _, values, enc in work:
         sel = enc.segs[enc.segs["row_offset"] < sample_rows]
-        segs = []
-        for s in sel:
-            segs.append(A.Segment(enc.blob.ctypes.data + int(s["offset"]), int(s["bytes_size"]),
-                                  int(s["row_offset"]), int(s["row_count"]), 0))
+        segs = [A.Segment(enc.blob.ctypes.data + int(s["offset"]), int(s["bytes_size"]), int(s["row_offset"]),
+                          int(s["row_count"]), 0) for s in sel]
         n = int(max(int(s["row_offset"]) + int(s["row_count"]) for s in sel))
         out = np.empty(n * enc.type_size, np.uint8)
         runs, k = A.make_runs([(enc.codec, segs)])
@@ -248,26 +411,24 @@ def cpu_baseline(work, rows):
         keep += [k, out]
         cols.append(col)
         nbytes += n * enc.type_size
-    # all columns must share total_rows for the table call: decode per column
     reps, elapsed = 0, 0.0
     t0 = time.perf_counter()
     while elapsed < 10.0 and reps < 2000:
-        for col in cols:
+        for col in cols:  # per column: the columns of the sample differ in row count
             rc = lib.oracle_decode_table_mt(C.byref(col), 1, threads)
             assert rc == 0
         reps += 1
         elapsed = time.perf_counter() - t0
     gbs = nbytes * reps / elapsed / 1e9
-    duck = None
     try:
         import duckdb  # noqa: F401
 
-        duck = "duckdb importable on this host (not timed in this build)"
+        duck = "duckdb importable on this host (not timed here)"
     except Exception:
         duck = "duckdb not installed: baseline is the repo's C restatement (not DuckDB)"
     log(f"[bench] cpu baseline {gbs:.2f} GB/s on {threads} threads ({reps} reps, {elapsed:.1f}s)")
     return {"value": round(gbs, 3), "unit": "GB/s", "cores": threads, "kind": "port",
-            "sample": f"first {sample_rows} rows of each of the 3 columns, decoded {reps}x; {duck}"}
+            "sample": f"first {sample_rows} rows of each column of the workload, decoded {reps}x; {duck}"}
 
 
 if __name__ == "__main__":

@@ -6,6 +6,8 @@
 #include "dec_rle.hpp"
 #include "sdr_tune.hpp"
 
+#include <algorithm>
+
 namespace sdr {
 
 // Wave w of block x decodes groups [(4x + w) * gpw, (4x + w + 1) * gpw).
@@ -45,13 +47,24 @@ __global__ __launch_bounds__(BLOCK_THREADS) void k_alp(const FixedSeg* __restric
   }
 }
 
-// blockIdx.x selects the slice of `slice_rows` rows (block-uniform exit).
+// One block per (segment, slice of `slice_rows` rows) over a 1-D grid of
+// n_blocks (a multiple of 8).  Workgroups are dispatched to the 8 XCDs
+// round-robin, so block b runs on XCD b % 8; the remap below hands each XCD
+// a contiguous range of (segment, slice) pairs, so the slices of a segment
+// share one L2 and its counts are fetched from HBM about once.
+constexpr uint32_t N_XCD = 8;
+
 __global__ __launch_bounds__(RLE_THREADS) void k_rle(const FixedSeg* __restrict__ segs, uint32_t seg_base,
-                                                    uint32_t slice_rows, uint32_t type_size)
+                                                    uint32_t n_segs, uint32_t slices, uint32_t slice_rows,
+                                                    uint32_t type_size)
 {
   __shared__ RleSmem sm;
-  const FixedSeg sd = segs[seg_base + blockIdx.y];
-  const uint32_t ra = blockIdx.x * slice_rows;
+  const uint32_t per_xcd = gridDim.x / N_XCD;
+  const uint32_t logical = (blockIdx.x % N_XCD) * per_xcd + blockIdx.x / N_XCD;
+  const uint32_t sgi     = logical / slices;
+  if (sgi >= n_segs) return;
+  const FixedSeg sd = segs[seg_base + sgi];
+  const uint32_t ra = (logical % slices) * slice_rows;
   if (ra >= sd.row_count) return;
   const uint32_t rb = sd.row_count - ra < slice_rows ? sd.row_count : ra + slice_rows;
   rle_slice(sd.bytes, sd.bytes_size, sd.row_count, ra, rb, sd.out, type_size, sm);
@@ -160,11 +173,19 @@ hipError_t launch_rle(const FixedSeg* d_segs, uint32_t seg_base, uint32_t n, uin
                       hipStream_t s)
 {
   if (n == 0 || max_rows == 0) return hipSuccess;
-  const uint32_t sr = tune(SDR_TUNE_RLE_SLICE_ROWS);
-  const uint32_t gx = (max_rows + sr - 1) / sr;
-  return chunked(seg_base, n, [&](uint32_t b, uint32_t m) {
-    hipLaunchKernelGGL(k_rle, dim3(gx, m), dim3(RLE_THREADS), 0, s, d_segs, b, sr, ts);
-  });
+  const uint32_t sr     = tune(SDR_TUNE_RLE_SLICE_ROWS);
+  const uint32_t slices = (max_rows + sr - 1) / sr;
+  // segments per launch so that the 1-D grid stays below 2^31 blocks
+  const uint32_t per = static_cast<uint32_t>(std::min<uint64_t>(MAX_GRID_Y, (0x7FFFFFF0ull / slices)));
+  for (uint32_t b = 0; b < n; b += per) {
+    const uint32_t m      = n - b < per ? n - b : per;
+    const uint64_t blocks = (uint64_t{m} * slices + N_XCD - 1) / N_XCD * N_XCD;
+    hipLaunchKernelGGL(k_rle, dim3(static_cast<uint32_t>(blocks)), dim3(RLE_THREADS), 0, s, d_segs, seg_base + b, m,
+                       slices, sr, ts);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
 }
 
 hipError_t launch_alp(const FixedSeg* d_segs, uint32_t seg_base, uint32_t n, uint32_t max_rows, uint32_t ts,
