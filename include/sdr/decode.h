@@ -229,12 +229,94 @@ int sdr_batch_add_strings(sdr_batch* batch, sdr_string_column* col);
 int sdr_batch_num_entries(const sdr_batch* batch);
 /* Upload the queue and enqueue the persistent decode on `stream`. */
 int sdr_batch_launch(sdr_batch* batch, sdr_stream_t stream);
-/* Synchronise `stream`, then write chars_used and null counts back into the
- * added columns; reports string overflow like sdr_decode_strings_column. */
+/* Wait for the last launch, then write chars_used and null counts back into
+ * the added columns; reports string overflow like sdr_decode_strings_column.
+ * (`stream` is unused: the wait is on the launch itself.) */
 int sdr_batch_finish(sdr_batch* batch, sdr_stream_t stream);
-/* Forget the added columns (the device arena and graph are kept). */
+/* Forget the added columns (the device arena and graph are kept); the
+ * write-back of a launch that was not finished is dropped. */
 int sdr_batch_clear(sdr_batch* batch);
 int sdr_batch_destroy(sdr_batch* batch);
+
+/* ---- serving ------------------------------------------------------------ */
+
+/* A decode server: requests from concurrent queries arrive over time and
+ * decode segments that stay resident on the device.  The server
+ *   - stages new segment bytes from host memory through a pinned buffer
+ *     into device memory it keeps (sdr_server_stage),
+ *   - admits queued requests in arrival order while resident bytes plus the
+ *     outputs of admitted, unreleased requests fit the HBM budget (a request
+ *     larger than the budget is admitted alone, when nothing else is held),
+ *   - allocates each admitted request's outputs from a HIP memory pool,
+ *   - and, in SDR_LAUNCH_PERSISTENT mode, coalesces every admitted request
+ *     into the next persistent graph launch while up to `max_inflight`
+ *     launches run on their own streams: requests that arrive while others
+ *     decode join the next launch as soon as a launch slot and budget are
+ *     free (continuous batching).  SDR_LAUNCH_BATCHED and
+ *     SDR_LAUNCH_PER_SEGMENT decode one request per launch slot with the
+ *     drop-in entries (the baselines).
+ * No counterpart in the reference (it decodes one split per call:
+ * src/op/scan/duckdb_native_decoder.cpp:972-1217); each request's output is
+ * what sdr_decode_table / sdr_decode_strings_column return for it.  A server
+ * is used from one thread. */
+typedef struct sdr_server sdr_server;
+
+typedef struct sdr_server_config {
+  uint64_t budget_bytes;       /* HBM admission budget; 0 = 200 GB            */
+  uint32_t max_inflight;       /* launches in flight at once; 0 = 2           */
+  uint32_t mode;               /* sdr_launch_mode                             */
+  uint32_t max_batch_requests; /* requests per persistent launch; 0 = no cap  */
+  uint32_t reserved;
+} sdr_server_config;
+
+/* One decode request: a fixed-width or VARCHAR column over resident
+ * segments.  The descriptors are copied at submit. */
+typedef struct sdr_request {
+  uint64_t tag;             /* caller's id (unique among pending requests)       */
+  uint32_t is_string;       /* 0 fixed-width, 1 VARCHAR                          */
+  uint32_t type_size;       /* fixed-width bytes per value                       */
+  uint32_t total_rows;
+  uint32_t n_data;
+  const sdr_run* data;
+  uint64_t chars_capacity;  /* VARCHAR chars to reserve; 0 = sdr_strings_upper_bound */
+} sdr_request;
+
+/* A finished request.  Times are milliseconds on the server clock (0 at
+ * creation): submitted, launch start and launch end on the GPU. */
+typedef struct sdr_completion {
+  uint64_t tag;
+  int32_t status;           /* sdr_status of the decode                          */
+  uint32_t reserved;
+  void* d_out;              /* values, or int32 offsets for VARCHAR              */
+  uint8_t* d_chars;         /* VARCHAR chars                                     */
+  uint64_t chars_used;
+  uint64_t out_bytes;       /* output bytes held until sdr_server_release        */
+  double submit_ms, start_ms, done_ms;
+} sdr_completion;
+
+int sdr_server_create(const sdr_server_config* config, sdr_server** out);
+/* Copy n host bytes to a new device allocation the server keeps until
+ * sdr_server_unstage (counted against the budget); *d_out receives it. */
+int sdr_server_stage(sdr_server* server, const void* host_bytes, uint64_t n, const uint8_t** d_out);
+int sdr_server_unstage(sdr_server* server, const uint8_t* d_ptr);
+/* Queue a request (host only); SDR_E_CAPACITY if its outputs exceed the
+ * budget left after the resident segments. */
+int sdr_server_submit(sdr_server* server, const sdr_request* request);
+/* Collect finished launches, admit, and launch what slots allow.
+ * *n_launched (optional) receives the number of launches started. */
+int sdr_server_progress(sdr_server* server, uint32_t* n_launched);
+/* Completions since the last poll, at most `cap`; *n receives the count. */
+int sdr_server_poll(sdr_server* server, sdr_completion* out, uint32_t cap, uint32_t* n);
+/* Return a completed request's outputs to the pool, ordered after the work
+ * already enqueued on `stream` (the consumer's stream). */
+int sdr_server_release(sdr_server* server, uint64_t tag, sdr_stream_t stream);
+/* Requests queued, admitted-not-launched and in flight. */
+int sdr_server_pending(const sdr_server* server);
+/* Bytes currently held against the budget (resident + outputs). */
+uint64_t sdr_server_held_bytes(const sdr_server* server);
+/* Milliseconds on the server clock. */
+double sdr_server_now_ms(const sdr_server* server);
+int sdr_server_destroy(sdr_server* server);
 
 /* ---- launch-shape tuning ------------------------------------------------ */
 

@@ -104,6 +104,43 @@ class StringColumn(C.Structure):
     ]
 
 
+class ServerConfig(C.Structure):
+    _fields_ = [
+        ("budget_bytes", C.c_uint64),
+        ("max_inflight", C.c_uint32),
+        ("mode", C.c_uint32),
+        ("max_batch_requests", C.c_uint32),
+        ("reserved", C.c_uint32),
+    ]
+
+
+class Request(C.Structure):
+    _fields_ = [
+        ("tag", C.c_uint64),
+        ("is_string", C.c_uint32),
+        ("type_size", C.c_uint32),
+        ("total_rows", C.c_uint32),
+        ("n_data", C.c_uint32),
+        ("data", C.POINTER(Run)),
+        ("chars_capacity", C.c_uint64),
+    ]
+
+
+class Completion(C.Structure):
+    _fields_ = [
+        ("tag", C.c_uint64),
+        ("status", C.c_int32),
+        ("reserved", C.c_uint32),
+        ("d_out", C.c_void_p),
+        ("d_chars", C.c_void_p),
+        ("chars_used", C.c_uint64),
+        ("out_bytes", C.c_uint64),
+        ("submit_ms", C.c_double),
+        ("start_ms", C.c_double),
+        ("done_ms", C.c_double),
+    ]
+
+
 def make_runs(runs):
     """[(codec, [Segment...]), ...] -> (Run array, keepalive list)."""
     keep = []

@@ -157,18 +157,51 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* s_wave
 
 
 // Shared memory of one chunk decode.
+constexpr uint32_t FSST_SYMSTAGE = 17u + 8u * FSST_NSYM;  // the largest symbol table (2057 bytes)
+constexpr uint32_t FSST_CSTAGE   = 16384;                  // chunk compressed bytes staged in LDS
+
 struct FsstSmem {
   uint8_t len[256];
   uint32_t lo[256], hi[256];
   uint32_t comp[FSST_CHUNK_ROWS + 1];  // segment-relative inclusive compressed prefix
   uint32_t dec[FSST_CHUNK_ROWS + 1];   // chunk-relative exclusive decoded offset
-  __attribute__((aligned(16))) uint8_t out[FSST_PIECE];
+  // emit pieces; before that, pieces of the compressed-length stream
+  __attribute__((aligned(16))) uint8_t out[FSST_PIECE + 48];
+  // the symbol table, then the chunk's compressed bytes
+  __attribute__((aligned(16))) uint8_t cst[FSST_CSTAGE + 48];
   uint32_t wave[FSST_WAVES];
   uint64_t base;
 };
 
+// Walk the n compressed bytes of one row with DuckDB's serial semantics:
+// emit(byte) for a literal after an escape, sym(code) for a symbol; a
+// trailing escape emits nothing.  `fetch(i)` returns byte i of the row.
+template <typename Fetch, typename Lit, typename Sym>
+__device__ __forceinline__ void fsst_walk(uint32_t n, Fetch fetch, Lit lit, Sym sym)
+{
+  bool esc = false;
+  for (uint32_t i = 0; i < n; ++i) {
+    const uint32_t b = fetch(i);
+    if (esc) {
+      if (!lit(b)) return;
+      esc = false;
+    } else if (b == FSST_ESC) {
+      esc = true;
+    } else if (!sym(b)) {
+      return;
+    }
+  }
+}
+
 // Decode chunk `c` (its position in the column's chunk order) with the
 // calling 256-thread workgroup.  Every thread must call it.
+//
+// Global traffic is staged through LDS with LDS-direct loads, so each stage
+// costs one memory round trip: the symbol table, the compressed-length
+// stream up to the chunk's last row (in 8 KiB pieces), and the chunk's
+// compressed bytes (when they fit in 16 KiB; rows outside the staged range
+// read global memory).  The column-wide chars offset comes from a
+// decoupled look-back done by one wavefront over 64 predecessors at a time.
 __device__ __forceinline__ void fsst_chunk(const FsstChunk ch, uint32_t c, uint32_t n_chunks,
                                            const FsstSeg* __restrict__ segs, FsstState* __restrict__ st,
                                            int32_t* __restrict__ offsets, uint8_t* __restrict__ chars,
@@ -192,37 +225,43 @@ __device__ __forceinline__ void fsst_chunk(const FsstChunk ch, uint32_t c, uint3
   }
 
   // ---- symbol table import, one code per thread (detail/fsst.cuh:50-96) ----
+  // Bytes past the segment read as zero; no parse reads past byte 2057.
+  __syncthreads();  // the previous user of sm is done
+  uint32_t sym_n = 0, sym_h = 0;
   if (ok) {
-    const uint8_t* blob  = seg + symoff;
-    uint32_t avail       = sg.bytes_size - symoff;
-    if (avail > SYMTAB_MAX) avail = SYMTAB_MAX;
-    const uint8_t* bend  = blob + avail;
+    sym_n = sg.bytes_size - symoff;
+    if (sym_n > FSST_SYMSTAGE) sym_n = FSST_SYMSTAGE;
+    sym_h = block_stage<FSST_THREADS>(reinterpret_cast<uint32_t*>(sm.cst), seg + symoff, sym_n);
+  }
+  __syncthreads();
+  if (ok) {
+    auto sb = [&](uint32_t i) -> uint32_t { return i < sym_n ? sm.cst[sym_h + i] : 0u; };
     uint64_t version = 0;
-    for (uint32_t i = 0; i < 8; ++i) version |= static_cast<uint64_t>(byte_at(blob + i, bend)) << (8u * i);
+    for (uint32_t i = 0; i < 8; ++i) version |= static_cast<uint64_t>(sb(i)) << (8u * i);
     const uint32_t code = tid;
     uint32_t len = 0;
     uint64_t sym = 0;
     if ((version >> 32) == FSST_VERSION && code < FSST_NSYM) {
-      const uint32_t zt = byte_at(blob + 8, bend) & 1u;
+      const uint32_t zt = sb(8) & 1u;
       uint32_t next_code = zt, pos = 17, found = 0;
       len = 8;
       sym = FSST_CORRUPT;
       if (zt && code == 0) {
-        len = 1;
-        sym = 0;
+        len   = 1;
+        sym   = 0;
         found = 1;
       }
       // groups in code order: lengths 2,3,...,8 then 1
       for (uint32_t l = 1; l <= 8 && !found; ++l) {
         const uint32_t h  = l & 7u;
         const uint32_t sl = h + 1u;
-        uint32_t cnt = byte_at(blob + 9 + h, bend);
+        uint32_t cnt = sb(9 + h);
         if (h == 0 && zt) cnt = (cnt - 1u) & 0xFFu;
         if (code >= next_code && code < next_code + cnt) {
           const uint32_t at = pos + (code - next_code) * sl;
           sym = 0;
-          for (uint32_t j = 0; j < sl; ++j) sym |= static_cast<uint64_t>(byte_at(blob + at + j, bend)) << (8u * j);
-          len = sl;
+          for (uint32_t j = 0; j < sl; ++j) sym |= static_cast<uint64_t>(sb(at + j)) << (8u * j);
+          len   = sl;
           found = 1;
         }
         next_code += cnt;
@@ -238,41 +277,55 @@ __device__ __forceinline__ void fsst_chunk(const FsstChunk ch, uint32_t c, uint3
     sm.hi[tid]  = 0;
   }
 
-  // ---- compressed-length prefix (fsst.cu:132-188) ----
+  // ---- compressed lengths (fsst.cu:132-188): the sum of the segment's
+  // fields before the chunk, and this thread's own fields ----
   const uint32_t rows = ch.rows;
   const uint32_t r0   = tid * ROWS_PER_THR;
-  uint32_t before = 0, total = 0;
-  {
-    // sum of the segment's lengths before this chunk
-    uint32_t part = 0;
-    if (ok && ch.row_begin) {
+  const uint32_t mine = r0 < rows ? (rows - r0 < ROWS_PER_THR ? rows - r0 : ROWS_PER_THR) : 0u;
+  uint32_t clen[ROWS_PER_THR];
+#pragma unroll
+  for (uint32_t k = 0; k < ROWS_PER_THR; ++k) clen[k] = 0;
+  uint32_t part = 0;
+  const uint32_t nf = ch.row_begin + rows;
+  const bool lfast  = ok && width && seg + 16 + ((uint64_t{nf} * width + 7u) >> 3) <= end;
+  if (lfast) {
+    const uint32_t per = ((FSST_PIECE - 16u) * 8u) / width;  // fields per staged piece
+    for (uint32_t f0 = 0; f0 < nf; f0 += per) {
+      const uint32_t f1 = nf - f0 < per ? nf : f0 + per;
+      const uint64_t b0 = (uint64_t{f0} * width) >> 3;
+      const uint64_t b1 = (uint64_t{f1} * width + 7u) >> 3;
+      __syncthreads();  // the previous piece is consumed
+      const uint32_t h = block_stage<FSST_THREADS>(reinterpret_cast<uint32_t*>(sm.out), seg + 16 + b0,
+                                                   static_cast<uint32_t>(b1 - b0));
+      __syncthreads();
+      const uint32_t bit0 = h * 8u + static_cast<uint32_t>((uint64_t{f0} * width) & 7u);
+      const uint32_t* lw  = reinterpret_cast<const uint32_t*>(sm.out);
+      const uint32_t pe   = f1 < ch.row_begin ? f1 : ch.row_begin;
+      for (uint32_t f = f0 + tid; f < pe; f += FSST_THREADS) part += lds_bits32(lw, bit0 + (f - f0) * width, width);
+#pragma unroll
+      for (uint32_t k = 0; k < ROWS_PER_THR; ++k) {
+        const uint32_t f = ch.row_begin + r0 + k;
+        if (k < mine && f >= f0 && f < f1) clen[k] = lds_bits32(lw, bit0 + (f - f0) * width, width);
+      }
+    }
+  } else if (ok) {
+    // a length stream running past the segment (malformed): bounded reads
+    if (ch.row_begin) {
       const uint32_t q  = (ch.row_begin + FSST_THREADS - 1) / FSST_THREADS;
-      const uint32_t i0 = tid * q;
+      const uint32_t i0 = tid * q < ch.row_begin ? tid * q : ch.row_begin;
       const uint32_t i1 = i0 + q < ch.row_begin ? i0 + q : ch.row_begin;
       part = sum_fields(seg + 16, end, i0, i1, width);
     }
-    block_excl_scan(part, sm.wave, &before);
+#pragma unroll
+    for (uint32_t k = 0; k < ROWS_PER_THR; ++k)
+      if (k < mine) clen[k] = stream_bits32(seg + 16, end, uint64_t{ch.row_begin + r0 + k} * width, width);
   }
-  uint32_t clen[ROWS_PER_THR];
+  uint32_t before = 0, total = 0;
+  block_excl_scan(part, sm.wave, &before);
   uint32_t acc = 0;
-  {
-    const uint32_t mine = r0 < rows ? (rows - r0 < ROWS_PER_THR ? rows - r0 : ROWS_PER_THR) : 0u;
-    const uint32_t i0 = ch.row_begin + r0;
-    const bool fast = ok && mine && width &&
-                      seg + 16 + ((uint64_t{i0 + mine} * width + 7u) >> 3) <= end;
-    if (fast) {
-      BitCursor bc(seg + 16, uint64_t{i0} * width);
 #pragma unroll
-      for (uint32_t k = 0; k < ROWS_PER_THR; ++k) clen[k] = k < mine ? bc.next(width) : 0u;
-    } else {
-#pragma unroll
-      for (uint32_t k = 0; k < ROWS_PER_THR; ++k)
-        clen[k] = (ok && k < mine) ? stream_bits32(seg + 16, end, uint64_t{i0 + k} * width, width) : 0u;
-    }
-#pragma unroll
-    for (uint32_t k = 0; k < ROWS_PER_THR; ++k) acc += clen[k];
-  }
-  uint32_t excl = block_excl_scan(acc, sm.wave, &total);
+  for (uint32_t k = 0; k < ROWS_PER_THR; ++k) acc += clen[k];
+  const uint32_t excl = block_excl_scan(acc, sm.wave, &total);
   if (tid == 0) sm.comp[0] = before;
   {
     uint32_t run = before + excl;
@@ -282,6 +335,14 @@ __device__ __forceinline__ void fsst_chunk(const FsstChunk ch, uint32_t c, uint3
       if (r0 + k < rows) sm.comp[r0 + k + 1] = run;
     }
   }
+  __syncthreads();
+
+  // ---- stage the chunk's compressed bytes.  Rows are stored back to front
+  // ending at dict_end: row r is [dict_end - comp[r+1], dict_end - comp[r]).
+  const uint32_t c_lo = sm.comp[0], c_hi = sm.comp[rows];
+  const bool staged   = ok && rows && c_hi >= c_lo && c_hi <= dict_end && c_hi - c_lo <= FSST_CSTAGE;
+  uint32_t c_h = 0;
+  if (staged) c_h = block_stage<FSST_THREADS>(reinterpret_cast<uint32_t*>(sm.cst), seg + dict_end - c_hi, c_hi - c_lo);
   __syncthreads();
 
   // ---- decoded lengths (fsst.cu:194-286, serial escape semantics) ----
@@ -295,18 +356,14 @@ __device__ __forceinline__ void fsst_chunk(const FsstChunk ch, uint32_t c, uint3
       const uint32_t cum = sm.comp[r + 1], prev = sm.comp[r];
       if (cum <= dict_end && prev <= cum && cum != prev) {
         const uint32_t n = cum - prev;
-        ByteReader br(seg + dict_end - cum);
-        bool esc = false;
-        for (uint32_t i = 0; i < n; ++i) {
-          const uint32_t b = br.next();
-          if (esc) {
-            ++d;
-            esc = false;
-          } else if (b == FSST_ESC) {
-            esc = true;
-          } else {
-            d += sm.len[b];
-          }
+        auto lit = [&](uint32_t) { ++d; return true; };
+        auto sym = [&](uint32_t b) { d += sm.len[b]; return true; };
+        if (staged && prev >= c_lo && cum <= c_hi) {
+          const uint8_t* lb = sm.cst + c_h + (c_hi - cum);
+          fsst_walk(n, [&](uint32_t i) -> uint32_t { return lb[i]; }, lit, sym);
+        } else {
+          ByteReader br(seg + dict_end - cum);
+          fsst_walk(n, [&](uint32_t) -> uint32_t { return br.next(); }, lit, sym);
         }
       }
     }
@@ -314,9 +371,9 @@ __device__ __forceinline__ void fsst_chunk(const FsstChunk ch, uint32_t c, uint3
     acc += d;
   }
   uint32_t T = 0;
-  excl = block_excl_scan(acc, sm.wave, &T);
   {
-    uint32_t run = excl;
+    const uint32_t dex = block_excl_scan(acc, sm.wave, &T);
+    uint32_t run       = dex;
 #pragma unroll
     for (uint32_t k = 0; k < ROWS_PER_THR; ++k) {
       if (r0 + k < rows) sm.dec[r0 + k] = run;
@@ -325,34 +382,49 @@ __device__ __forceinline__ void fsst_chunk(const FsstChunk ch, uint32_t c, uint3
   }
   if (tid == 0) sm.dec[rows] = T;
 
-  // ---- decoupled look-back for the column-wide chars offset ----
-  if (tid == 0) {
-    uint64_t* status = st->status;
-    uint64_t excl64  = 0;
+  // ---- decoupled look-back for the column-wide chars offset, by wave 0:
+  // publish this chunk's aggregate, then sum predecessors 64 at a time back
+  // to the nearest published inclusive prefix.  A status word carries its
+  // flag and value together and is the only datum exchanged between chunks,
+  // so device-scope relaxed atomics (coherent across the XCDs' L2s) suffice;
+  // release/acquire orderings would add an L2 write-back / invalidate per
+  // access. ----
+  if (tid < WAVE) {
+    const uint32_t lane = tid;
+    uint64_t* status    = st->status;
+    uint64_t excl64     = 0;
     if (c == 0) {
-      __hip_atomic_store(&status[0], ST_INC | T, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+      if (lane == 0) __hip_atomic_store(&status[0], ST_INC | T, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     } else {
-      __hip_atomic_store(&status[c], ST_AGG | T, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+      if (lane == 0) __hip_atomic_store(&status[c], ST_AGG | T, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       uint32_t p = c - 1, spins = 0;
       for (;;) {
-        const uint64_t s = __hip_atomic_load(&status[p], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+        const bool valid = lane <= p;
+        const uint64_t s =
+          valid ? __hip_atomic_load(&status[p - lane], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0ull;
         const uint64_t flag = s & ~ST_VAL;
-        if (flag == 0) {
-          if (++spins > spin_limit) {  // never expected: give up rather than hang
-            atomicOr(&st->error, 1u);
+        const uint64_t incm = ballot(valid && flag == ST_INC);
+        const uint32_t stop = incm ? static_cast<uint32_t>(__builtin_ctzll(incm)) : (p < WAVE - 1 ? p : WAVE - 1);
+        const uint64_t need = stop == WAVE - 1 ? ~0ull : ((2ull << stop) - 1ull);
+        if (ballot(valid && flag == 0) & need) {  // a predecessor has not published yet
+          if (++spins > spin_limit) {              // never expected: give up rather than hang
+            if (lane == 0) atomicOr(&st->error, 1u);
             break;
           }
           __builtin_amdgcn_s_sleep(1);
           continue;
         }
-        excl64 += s & ST_VAL;
-        if (flag == ST_INC || p == 0) break;
-        --p;
+        excl64 += readlane_u64(wave_incl_sum_u64(lane <= stop ? (s & ST_VAL) : 0ull), WAVE - 1);
+        if (incm || p < WAVE) break;  // chunk 0 always publishes an inclusive prefix
+        p -= WAVE;
       }
-      __hip_atomic_store(&status[c], ST_INC | ((excl64 + T) & ST_VAL), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+      if (lane == 0)
+        __hip_atomic_store(&status[c], ST_INC | ((excl64 + T) & ST_VAL), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
-    sm.base = excl64;
-    if (c == n_chunks - 1) st->total = excl64 + T;
+    if (lane == 0) {
+      sm.base = excl64;
+      if (c == n_chunks - 1) st->total = excl64 + T;
+    }
   }
   __syncthreads();
   const uint64_t base = sm.base;
@@ -378,24 +450,25 @@ __device__ __forceinline__ void fsst_chunk(const FsstChunk ch, uint32_t c, uint3
       const uintptr_t o0 = g0 + sm.dec[r];
       if (o0 + dlen[k] <= pa || o0 >= pb) continue;
       const uint32_t cum = sm.comp[r + 1], prev = sm.comp[r];
-      const uint32_t n   = cum - prev;
-      ByteReader br(seg + dict_end - cum);
       uintptr_t o = o0;
-      bool esc    = false;
-      for (uint32_t i = 0; i < n && o < pb; ++i) {
-        const uint32_t b = br.next();
-        if (esc) {
-          if (o >= pa) sm.out[o - pa] = static_cast<uint8_t>(b);
-          ++o;
-          esc = false;
-        } else if (b == FSST_ESC) {
-          esc = true;
-        } else {
-          const uint32_t l   = sm.len[b];
-          const uint64_t sym = static_cast<uint64_t>(sm.lo[b]) | (static_cast<uint64_t>(sm.hi[b]) << 32);
-          for (uint32_t j = 0; j < l; ++j, ++o)
-            if (o >= pa && o < pb) sm.out[o - pa] = static_cast<uint8_t>(sym >> (8u * j));
-        }
+      auto lit = [&](uint32_t b) {
+        if (o >= pa) sm.out[o - pa] = static_cast<uint8_t>(b);
+        ++o;
+        return o < pb;
+      };
+      auto sym = [&](uint32_t b) {
+        const uint32_t l   = sm.len[b];
+        const uint64_t val = static_cast<uint64_t>(sm.lo[b]) | (static_cast<uint64_t>(sm.hi[b]) << 32);
+        for (uint32_t j = 0; j < l; ++j, ++o)
+          if (o >= pa && o < pb) sm.out[o - pa] = static_cast<uint8_t>(val >> (8u * j));
+        return o < pb;
+      };
+      if (staged && prev >= c_lo && cum <= c_hi) {
+        const uint8_t* lb = sm.cst + c_h + (c_hi - cum);
+        fsst_walk(cum - prev, [&](uint32_t i) -> uint32_t { return lb[i]; }, lit, sym);
+      } else {
+        ByteReader br(seg + dict_end - cum);
+        fsst_walk(cum - prev, [&](uint32_t) -> uint32_t { return br.next(); }, lit, sym);
       }
     }
     __syncthreads();

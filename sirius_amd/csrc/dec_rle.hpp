@@ -45,25 +45,6 @@ struct RleSmem {
   uint32_t match, bad;                     // block minima (entry index)
 };
 
-// Block-wide copy of [p, p + nbytes) into LDS as 16-byte blocks (LDS byte 0
-// holds align_down(p, 16)) with LDS-direct loads; returns the byte skew of
-// p.  Each wave has waited for its own loads; the caller's barrier makes
-// the whole stage visible.
-__device__ __forceinline__ uint32_t block_stage(uint32_t* lds, const uint8_t* p, uint32_t nbytes)
-{
-  const uintptr_t a   = reinterpret_cast<uintptr_t>(p);
-  const uintptr_t a16 = a & ~uintptr_t{15};
-  const uint32_t head = static_cast<uint32_t>(a - a16);
-  const uint32_t nblk = nbytes ? (head + nbytes + 15u) >> 4 : 0u;
-  const uint32_t wb   = (threadIdx.x / WAVE) * WAVE;  // first block of this wave per round
-  const uint8_t* src  = reinterpret_cast<const uint8_t*>(a16) + 16u * threadIdx.x;
-  for (uint32_t k = 0; k * RLE_THREADS < nblk; ++k)
-    if (k * RLE_THREADS + threadIdx.x < nblk)
-      lds_direct_16(src + 16u * RLE_THREADS * k, lds + 4u * (RLE_THREADS * k + wb));
-  __builtin_amdgcn_s_waitcnt(WAITCNT_VMEM_DONE);
-  return head;
-}
-
 __device__ __forceinline__ uint32_t lds_u16(const uint32_t* lds, uint32_t byte)
 {
   const uint32_t w = byte >> 2;
@@ -82,7 +63,7 @@ __device__ __forceinline__ uint32_t rle_pass(const uint8_t* counts, uint32_t t0,
     sm.match = UINT32_MAX;
     sm.bad   = UINT32_MAX;
   }
-  head = block_stage(sm.stage, counts + 2u * t0, 2u * n);
+  head = block_stage<RLE_THREADS>(sm.stage, counts + 2u * t0, 2u * n);
   __syncthreads();
 
   // Thread t owns entries [32 t, 32 t + 32) of the pass: 16 words of two

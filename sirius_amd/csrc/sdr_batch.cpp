@@ -35,6 +35,9 @@ struct sdr_batch {
   size_t arena_bytes   = 0;
   uint8_t* h_stage     = nullptr;
   size_t stage_bytes   = 0;
+  uint8_t* h_readback  = nullptr;  // pinned copy of the look-back state and null counters
+  size_t readback_cap  = 0;
+  size_t readback_bytes = 0;
   size_t used_bytes    = 0;
   size_t off_items = 0, off_cols = 0, off_chunks = 0, off_segs = 0, off_state = 0, off_nulls = 0;
   uint32_t n_status    = 0;
@@ -50,6 +53,7 @@ struct sdr_batch {
   uint8_t* graph_arena = nullptr;
   uint32_t graph_grid  = 0;
   hipStream_t last_stream = nullptr;
+  bool launched           = false;  // a launch whose results are not collected yet
   ~sdr_batch();
 };
 
@@ -67,6 +71,7 @@ sdr_batch::~sdr_batch()
   if (graph) (void)hipGraphDestroy(graph);
   if (d_arena) (void)hipFree(d_arena);
   if (h_stage) (void)hipHostFree(h_stage);
+  if (h_readback) (void)hipHostFree(h_readback);
 }
 
 namespace sdr {
@@ -226,6 +231,14 @@ int batch_upload(sdr_batch* b, hipStream_t s)
     HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&b->h_stage), upload_bytes + upload_bytes / 4, hipHostMallocDefault));
     b->stage_bytes = upload_bytes + upload_bytes / 4;
   }
+  b->readback_bytes = off - b->off_state;  // state words and null counters
+  if (b->readback_bytes > b->readback_cap) {
+    if (b->h_readback) HIP_TRY(hipHostFree(b->h_readback));
+    b->h_readback   = nullptr;
+    b->readback_cap = 0;
+    HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&b->h_readback), b->readback_bytes, hipHostMallocDefault));
+    b->readback_cap = b->readback_bytes;
+  }
   uint8_t* h = b->h_stage;
   uint8_t* d = b->d_arena;
   BatchQueue q{};
@@ -328,35 +341,39 @@ int batch_launch(sdr_batch* b, hipStream_t s)
       if (rc) return rc;
     }
   }
+  // the host reads string totals and null counts from a pinned copy once
+  // the launch has finished (batch_collect)
+  HIP_TRY(hipMemcpyAsync(b->h_readback, b->d_arena + b->off_state, b->readback_bytes, hipMemcpyDeviceToHost, s));
   if (!b->launch_done) HIP_TRY(hipEventCreateWithFlags(&b->launch_done, hipEventDisableTiming));
   HIP_TRY(hipEventRecord(b->launch_done, s));
   b->last_stream = s;
+  b->launched    = true;
   return SDR_OK;
 }
 
-// Synchronise and write back string totals and null counts.
-int batch_finish(sdr_batch* b, hipStream_t s)
+// 1 when the last launch has finished, 0 while it runs.
+int batch_ready(sdr_batch* b)
 {
-  clear_error();
-  std::vector<uint64_t> st;
-  size_t words = 0;
-  for (auto& c : b->cols) words += 2 + c.chunks.size();
-  if (!b->d_arena) return SDR_OK;
-  std::vector<unsigned long long> nulls(b->masks.size());
-  BatchQueue q{};
-  HIP_TRY(hipMemcpyAsync(&q, b->d_arena, sizeof(q), hipMemcpyDeviceToHost, s));
-  if (words) {
-    st.resize(words);
-    HIP_TRY(hipMemcpyAsync(st.data(), b->d_arena + b->off_state, words * sizeof(uint64_t), hipMemcpyDeviceToHost, s));
-  }
-  if (!nulls.empty())
-    HIP_TRY(hipMemcpyAsync(nulls.data(), b->d_arena + b->off_nulls, nulls.size() * sizeof(unsigned long long),
-                           hipMemcpyDeviceToHost, s));
-  HIP_TRY(hipStreamSynchronize(s));
+  if (!b->launched || !b->launch_done) return 1;
+  const hipError_t e = hipEventQuery(b->launch_done);
+  if (e == hipSuccess) return 1;
+  if (e == hipErrorNotReady) return 0;
+  return fail(SDR_E_HIP, "batch launch: %s", hipGetErrorString(e));
+}
+
+// After the last launch finished: write string totals and null counts back
+// into the added columns and report look-back failures and overflow.
+int batch_collect(sdr_batch* b)
+{
+  if (!b->launched) return SDR_OK;
+  b->launched = false;
+  const auto* st = reinterpret_cast<const uint64_t*>(b->h_readback);
+  const auto* nulls =
+    reinterpret_cast<const unsigned long long*>(b->h_readback + (b->off_nulls - b->off_state));
   int rc = SDR_OK;
   size_t w = 0;
   for (auto& c : b->cols) {
-    const uint32_t err  = static_cast<uint32_t>(st[w] >> 32);
+    const uint32_t err   = static_cast<uint32_t>(st[w] >> 32);
     const uint64_t total = st[w + 1];
     w += 2 + c.chunks.size();
     c.user->chars_used = total;
@@ -370,6 +387,16 @@ int batch_finish(sdr_batch* b, hipStream_t s)
     else b->masks[i].str->null_count = static_cast<int64_t>(nulls[i]);
   }
   return rc;
+}
+
+// Wait for the last launch and write back string totals and null counts.
+int batch_finish(sdr_batch* b, hipStream_t s)
+{
+  clear_error();
+  (void)s;
+  if (!b->launched) return SDR_OK;
+  HIP_TRY(hipEventSynchronize(b->launch_done));
+  return batch_collect(b);
 }
 
 namespace {
@@ -388,6 +415,7 @@ int decode_table_persistent(sdr_column* cols, uint32_t n_cols, hipStream_t s)
   b->items.clear();
   b->cols.clear();
   b->masks.clear();
+  b->launched = false;  // results of a launch nobody collected are dropped
   for (uint32_t c = 0; c < n_cols; ++c) cols[c].null_count = 0;
   rc = batch_add_fixed(b, cols, n_cols);
   if (!rc) rc = batch_launch(b, s);
@@ -403,6 +431,7 @@ int decode_strings_persistent(sdr_string_column* col, hipStream_t s)
   b->items.clear();
   b->cols.clear();
   b->masks.clear();
+  b->launched = false;  // results of a launch nobody collected are dropped
   col->chars_used = 0;
   col->null_count = 0;
   rc = batch_add_strings(b, col);
@@ -491,6 +520,7 @@ int sdr_batch_clear(sdr_batch* b)
   b->items.clear();
   b->cols.clear();
   b->masks.clear();
+  b->launched = false;  // results of a launch nobody collected are dropped
   return SDR_OK;
 }
 

@@ -237,6 +237,25 @@ __device__ __forceinline__ uint32_t stage_to_lds(uint32_t* lds, const uint8_t* p
   return head * 8u;
 }
 
+// Block-wide copy of [p, p + nbytes) into LDS as 16-byte blocks by NT
+// threads (LDS byte 0 holds align_down(p, 16)); returns the byte skew of p.
+// Each wave waits for its own loads; the caller's barrier makes the whole
+// stage visible to the block.
+template <uint32_t NT>
+__device__ __forceinline__ uint32_t block_stage(uint32_t* lds, const uint8_t* p, uint32_t nbytes)
+{
+  const uintptr_t a   = reinterpret_cast<uintptr_t>(p);
+  const uintptr_t a16 = a & ~uintptr_t{15};
+  const uint32_t head = static_cast<uint32_t>(a - a16);
+  const uint32_t nblk = nbytes ? (head + nbytes + 15u) >> 4 : 0u;
+  const uint32_t wb   = (threadIdx.x / WAVE) * WAVE;  // first block of this wave per round
+  const uint8_t* src  = reinterpret_cast<const uint8_t*>(a16) + 16u * threadIdx.x;
+  for (uint32_t k = 0; k * NT < nblk; ++k)
+    if (k * NT + threadIdx.x < nblk) lds_direct_16(src + 16u * NT * k, lds + 4u * (NT * k + wb));
+  __builtin_amdgcn_s_waitcnt(WAITCNT_VMEM_DONE);
+  return head;
+}
+
 // ---- stores ----------------------------------------------------------------
 
 template <typename T>

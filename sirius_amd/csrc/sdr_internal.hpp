@@ -43,6 +43,12 @@ int validate_string_column(const sdr_string_column* col);
 // SDR_LAUNCH_PERSISTENT: one persistent launch through a thread-local batch.
 int decode_table_persistent(sdr_column* cols, uint32_t n_cols, hipStream_t s);
 int decode_strings_persistent(sdr_string_column* col, hipStream_t s);
+// Batch internals used by the server (sdr_serve.cpp); the caller validates.
+int batch_add_fixed(sdr_batch* b, sdr_column* cols, uint32_t n_cols);
+int batch_add_strings(sdr_batch* b, sdr_string_column* col);
+int batch_launch(sdr_batch* b, hipStream_t s);
+int batch_ready(sdr_batch* b);    // 1 finished, 0 running, < 0 error
+int batch_collect(sdr_batch* b);  // after batch_ready: write back totals and null counts
 
 // Segments grouped by (codec, type size); [first, first + count) in the
 // descriptor array.

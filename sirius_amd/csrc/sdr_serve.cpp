@@ -1,0 +1,413 @@
+// sdr_serve.cpp — the decode server (sdr_server_*): resident segments,
+// HBM-budget admission, pool-allocated outputs and continuous batching over
+// persistent graph launches (or one drop-in launch per request, the
+// baselines).  Runs on the calling thread; progress() does all device work.
+#include "sdr_internal.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <deque>
+#include <memory>
+#include <unordered_map>
+#include <vector>
+
+namespace {
+
+constexpr uint64_t DEFAULT_BUDGET = 200ull * 1000 * 1000 * 1000;
+constexpr size_t STAGE_CHUNK      = 64u << 20;  // pinned staging buffer
+
+struct Req {
+  sdr_request r{};
+  std::vector<sdr_segment> segs;
+  std::vector<sdr_run> runs;
+  sdr_column col{};
+  sdr_string_column scol{};
+  uint64_t out_bytes  = 0;
+  uint64_t chars_cap  = 0;
+  uint8_t* d_alloc    = nullptr;
+  double t_submit     = 0;
+  int status          = SDR_OK;
+  bool done           = false;
+};
+
+struct Slot {
+  hipStream_t s        = nullptr;
+  hipEvent_t ev_start  = nullptr;
+  hipEvent_t ev_end    = nullptr;
+  sdr_batch* batch     = nullptr;
+  bool busy            = false;
+  int launch_status    = SDR_OK;  // synchronous status of a drop-in launch
+  std::vector<Req*> reqs;
+};
+
+using Clock = std::chrono::steady_clock;
+
+}  // namespace
+
+struct sdr_server {
+  sdr_server_config cfg{};
+  int device            = 0;
+  hipMemPool_t pool     = nullptr;
+  hipStream_t aux       = nullptr;
+  hipEvent_t t0_event   = nullptr;
+  double t0_gpu_ms      = 0;  // server clock at t0_event
+  Clock::time_point t0;
+  std::vector<Slot> slots;
+  std::deque<Req*> queue;     // submitted, not admitted
+  std::deque<Req*> admitted;  // admitted, not launched
+  std::unordered_map<uint64_t, std::unique_ptr<Req>> reqs;
+  std::vector<sdr_completion> done;
+  std::unordered_map<const uint8_t*, uint64_t> staged;
+  uint8_t* h_stage      = nullptr;
+  uint64_t resident     = 0;
+  uint64_t outputs_held = 0;
+  uint32_t running      = 0;
+
+  double now_ms() const { return std::chrono::duration<double, std::milli>(Clock::now() - t0).count(); }
+  double gpu_ms(hipEvent_t ev) const
+  {
+    float ms = 0;
+    if (hipEventElapsedTime(&ms, t0_event, ev) != hipSuccess) return 0;
+    return t0_gpu_ms + ms;
+  }
+  ~sdr_server();
+};
+
+sdr_server::~sdr_server()
+{
+  for (auto& sl : slots) {
+    if (sl.s) (void)hipStreamSynchronize(sl.s);
+    if (sl.batch) (void)sdr_batch_destroy(sl.batch);
+    if (sl.ev_start) (void)hipEventDestroy(sl.ev_start);
+    if (sl.ev_end) (void)hipEventDestroy(sl.ev_end);
+  }
+  for (auto& kv : reqs)
+    if (kv.second->d_alloc) (void)hipFreeAsync(kv.second->d_alloc, aux);
+  if (aux) (void)hipStreamSynchronize(aux);
+  for (auto& kv : staged) (void)hipFree(const_cast<uint8_t*>(kv.first));
+  for (auto& sl : slots)
+    if (sl.s) (void)hipStreamDestroy(sl.s);
+  if (aux) (void)hipStreamDestroy(aux);
+  if (t0_event) (void)hipEventDestroy(t0_event);
+  if (pool) (void)hipMemPoolDestroy(pool);
+  if (h_stage) (void)hipHostFree(h_stage);
+}
+
+namespace sdr {
+namespace {
+
+uint64_t held(const sdr_server* sv) { return sv->resident + sv->outputs_held; }
+
+// Allocate the request's outputs from the pool on `s` and point its column
+// struct at them.
+int allocate_outputs(sdr_server* sv, Req* q, hipStream_t s)
+{
+  HIP_TRY(hipMallocFromPoolAsync(reinterpret_cast<void**>(&q->d_alloc), std::max<uint64_t>(q->out_bytes, 256), sv->pool, s));
+  if (q->r.is_string) {
+    const uint64_t off_bytes = (uint64_t{q->r.total_rows} + 1u) * 4u;
+    q->scol.d_offsets        = reinterpret_cast<int32_t*>(q->d_alloc);
+    q->scol.d_chars          = q->d_alloc + ((off_bytes + 255u) & ~uint64_t{255});
+    q->scol.chars_capacity   = q->chars_cap;
+  } else {
+    q->col.d_out = q->d_alloc;
+  }
+  return SDR_OK;
+}
+
+void complete(sdr_server* sv, Slot& sl, int rc)
+{
+  const double t_start = sv->gpu_ms(sl.ev_start), t_end = sv->gpu_ms(sl.ev_end);
+  for (Req* q : sl.reqs) {
+    int st = rc;
+    if (q->r.is_string && st == SDR_OK && q->scol.chars_used > q->chars_cap) st = SDR_E_OVERFLOW;
+    q->status = st;
+    q->done   = true;
+    sdr_completion c{};
+    c.tag        = q->r.tag;
+    c.status     = st;
+    c.d_out      = q->r.is_string ? static_cast<void*>(q->scol.d_offsets) : q->col.d_out;
+    c.d_chars    = q->r.is_string ? q->scol.d_chars : nullptr;
+    c.chars_used = q->r.is_string ? q->scol.chars_used : 0;
+    c.out_bytes  = q->out_bytes;
+    c.submit_ms  = q->t_submit;
+    c.start_ms   = t_start;
+    c.done_ms    = t_end;
+    sv->done.push_back(c);
+  }
+  sl.reqs.clear();
+  sl.busy = false;
+  --sv->running;
+}
+
+int collect(sdr_server* sv)
+{
+  for (auto& sl : sv->slots) {
+    if (!sl.busy) continue;
+    int rc = SDR_OK;
+    if (sl.batch) {
+      const int r = batch_ready(sl.batch);
+      if (r < 0) return r;
+      if (r == 0) continue;
+      rc = batch_collect(sl.batch);
+    } else {
+      const hipError_t e = hipEventQuery(sl.ev_end);
+      if (e == hipErrorNotReady) continue;
+      if (e != hipSuccess) return fail(SDR_E_HIP, "server launch: %s", hipGetErrorString(e));
+      rc = sl.launch_status;
+    }
+    complete(sv, sl, rc);
+  }
+  return SDR_OK;
+}
+
+void admit(sdr_server* sv)
+{
+  while (!sv->queue.empty()) {
+    Req* q = sv->queue.front();
+    // a request larger than the budget runs alone
+    if (held(sv) + q->out_bytes > sv->cfg.budget_bytes && sv->outputs_held != 0) break;
+    sv->outputs_held += q->out_bytes;
+    sv->admitted.push_back(q);
+    sv->queue.pop_front();
+  }
+}
+
+int launch(sdr_server* sv, Slot& sl, uint32_t* n_launched)
+{
+  const uint32_t cap = sv->cfg.mode == SDR_LAUNCH_PERSISTENT
+                         ? (sv->cfg.max_batch_requests ? sv->cfg.max_batch_requests : UINT32_MAX)
+                         : 1u;
+  while (!sv->admitted.empty() && sl.reqs.size() < cap) {
+    sl.reqs.push_back(sv->admitted.front());
+    sv->admitted.pop_front();
+  }
+  for (Req* q : sl.reqs) {
+    int rc = allocate_outputs(sv, q, sl.s);
+    if (rc) return rc;
+  }
+  HIP_TRY(hipEventRecord(sl.ev_start, sl.s));
+  int rc = SDR_OK;
+  if (sl.batch) {
+    rc = sdr_batch_clear(sl.batch);
+    for (Req* q : sl.reqs) {
+      if (rc) break;
+      rc = q->r.is_string ? batch_add_strings(sl.batch, &q->scol) : batch_add_fixed(sl.batch, &q->col, 1);
+    }
+    if (!rc) rc = batch_launch(sl.batch, sl.s);
+  } else {
+    Req* q = sl.reqs.front();
+    if (q->r.is_string) rc = sdr_decode_strings_column_mode(&q->scol, sl.s, sv->cfg.mode);
+    else rc = sdr_decode_table_mode(&q->col, 1, sl.s, sv->cfg.mode);
+    sl.launch_status = rc;
+    if (rc == SDR_E_OVERFLOW) rc = SDR_OK;  // reported on the request, not the server
+  }
+  if (rc) return rc;
+  HIP_TRY(hipEventRecord(sl.ev_end, sl.s));
+  sl.busy = true;
+  ++sv->running;
+  if (n_launched) ++*n_launched;
+  return SDR_OK;
+}
+
+}  // namespace
+}  // namespace sdr
+
+using namespace sdr;
+
+extern "C" {
+
+int sdr_server_create(const sdr_server_config* config, sdr_server** out)
+{
+  clear_error();
+  if (!out) return fail(SDR_E_INVALID_ARG, "out is NULL");
+  *out = nullptr;
+  auto sv = std::make_unique<sdr_server>();
+  if (config) sv->cfg = *config;
+  if (!sv->cfg.budget_bytes) sv->cfg.budget_bytes = DEFAULT_BUDGET;
+  if (!sv->cfg.max_inflight) sv->cfg.max_inflight = 2;
+  if (sv->cfg.max_inflight > 64) return fail(SDR_E_INVALID_ARG, "max_inflight %u > 64", sv->cfg.max_inflight);
+  if (sv->cfg.mode > SDR_LAUNCH_PERSISTENT) return fail(SDR_E_INVALID_ARG, "unknown launch mode %u", sv->cfg.mode);
+  sv->t0 = Clock::now();
+  HIP_TRY(hipGetDevice(&sv->device));
+  hipMemPoolProps props{};
+  props.allocType     = hipMemAllocationTypePinned;
+  props.location.type = hipMemLocationTypeDevice;
+  props.location.id   = sv->device;
+  HIP_TRY(hipMemPoolCreate(&sv->pool, &props));
+  uint64_t keep = UINT64_MAX;  // a warm pool never returns memory to the driver
+  HIP_TRY(hipMemPoolSetAttribute(sv->pool, hipMemPoolAttrReleaseThreshold, &keep));
+  HIP_TRY(hipStreamCreateWithFlags(&sv->aux, hipStreamNonBlocking));
+  sv->slots.resize(sv->cfg.max_inflight);
+  for (auto& sl : sv->slots) {
+    HIP_TRY(hipStreamCreateWithFlags(&sl.s, hipStreamNonBlocking));
+    HIP_TRY(hipEventCreate(&sl.ev_start));
+    HIP_TRY(hipEventCreate(&sl.ev_end));
+    if (sv->cfg.mode == SDR_LAUNCH_PERSISTENT) {
+      int rc = sdr_batch_create(&sl.batch);
+      if (rc) return rc;
+    }
+  }
+  HIP_TRY(hipEventCreate(&sv->t0_event));
+  HIP_TRY(hipEventRecord(sv->t0_event, sv->aux));
+  HIP_TRY(hipEventSynchronize(sv->t0_event));
+  sv->t0_gpu_ms = sv->now_ms();
+  *out = sv.release();
+  return SDR_OK;
+}
+
+int sdr_server_stage(sdr_server* sv, const void* host_bytes, uint64_t n, const uint8_t** d_out)
+{
+  clear_error();
+  if (!sv || !d_out || (n && !host_bytes)) return fail(SDR_E_INVALID_ARG, "NULL argument");
+  *d_out = nullptr;
+  if (held(sv) + n > sv->cfg.budget_bytes)
+    return fail(SDR_E_CAPACITY, "staging %llu bytes exceeds the budget", static_cast<unsigned long long>(n));
+  if (!sv->h_stage) HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&sv->h_stage), STAGE_CHUNK, hipHostMallocDefault));
+  uint8_t* d = nullptr;
+  HIP_TRY(hipMalloc(reinterpret_cast<void**>(&d), std::max<uint64_t>(n, 16)));
+  const auto* src = static_cast<const uint8_t*>(host_bytes);
+  for (uint64_t o = 0; o < n; o += STAGE_CHUNK) {
+    const size_t m = static_cast<size_t>(std::min<uint64_t>(STAGE_CHUNK, n - o));
+    std::memcpy(sv->h_stage, src + o, m);
+    hipError_t e = hipMemcpyAsync(d + o, sv->h_stage, m, hipMemcpyHostToDevice, sv->aux);
+    if (e == hipSuccess) e = hipStreamSynchronize(sv->aux);  // the staging buffer is reused
+    if (e != hipSuccess) {
+      (void)hipFree(d);
+      return fail(SDR_E_HIP, "stage: %s", hipGetErrorString(e));
+    }
+  }
+  sv->staged[d] = n;
+  sv->resident += n;
+  *d_out = d;
+  return SDR_OK;
+}
+
+int sdr_server_unstage(sdr_server* sv, const uint8_t* d_ptr)
+{
+  clear_error();
+  if (!sv) return fail(SDR_E_INVALID_ARG, "server is NULL");
+  auto it = sv->staged.find(d_ptr);
+  if (it == sv->staged.end()) return fail(SDR_E_NOT_FOUND, "pointer was not staged by this server");
+  for (auto& sl : sv->slots)
+    if (sl.busy) HIP_TRY(hipStreamSynchronize(sl.s));  // in-flight work may read it
+  HIP_TRY(hipFree(const_cast<uint8_t*>(d_ptr)));
+  sv->resident -= it->second;
+  sv->staged.erase(it);
+  return SDR_OK;
+}
+
+int sdr_server_submit(sdr_server* sv, const sdr_request* r)
+{
+  clear_error();
+  if (!sv || !r) return fail(SDR_E_INVALID_ARG, "NULL argument");
+  if (sv->reqs.count(r->tag)) return fail(SDR_E_INVALID_ARG, "tag %llu is pending", static_cast<unsigned long long>(r->tag));
+  if (r->n_data && !r->data) return fail(SDR_E_INVALID_ARG, "data is NULL");
+  auto q = std::make_unique<Req>();
+  q->r = *r;
+  size_t nseg = 0;
+  for (uint32_t i = 0; i < r->n_data; ++i) nseg += r->data[i].segments ? r->data[i].n_segments : 0;
+  q->segs.reserve(nseg);
+  for (uint32_t i = 0; i < r->n_data; ++i) {
+    const sdr_run& in = r->data[i];
+    const size_t first = q->segs.size();
+    for (uint32_t k = 0; in.segments && k < in.n_segments; ++k) q->segs.push_back(in.segments[k]);
+    q->runs.push_back({in.codec, static_cast<uint32_t>(q->segs.size() - first), nullptr});
+  }
+  for (size_t i = 0, o = 0; i < q->runs.size(); ++i) {
+    q->runs[i].segments = q->segs.data() + o;
+    o += q->runs[i].n_segments;
+  }
+  q->r.data = q->runs.data();
+  // validate with placeholder outputs (validation never dereferences them)
+  uint8_t* const placeholder = reinterpret_cast<uint8_t*>(uintptr_t{256});
+  int rc;
+  if (r->is_string) {
+    q->chars_cap = r->chars_capacity ? r->chars_capacity : sdr_strings_upper_bound(q->runs.data(), r->n_data);
+    if (q->chars_cap == UINT64_MAX)
+      return fail(SDR_E_INVALID_ARG, "VARCHAR request without chars_capacity and with an unknown max_string_length");
+    auto& s = q->scol;
+    s.total_rows = r->total_rows;
+    s.data       = q->runs.data();
+    s.n_data     = r->n_data;
+    s.d_offsets  = reinterpret_cast<int32_t*>(placeholder);
+    rc           = validate_string_column(&s);
+    q->out_bytes = (((uint64_t{r->total_rows} + 1u) * 4u + 255u) & ~uint64_t{255}) + q->chars_cap;
+  } else {
+    auto& c = q->col;
+    c.type_size  = r->type_size;
+    c.total_rows = r->total_rows;
+    c.data       = q->runs.data();
+    c.n_data     = r->n_data;
+    c.d_out      = placeholder;
+    rc           = validate_fixed_columns(&c, 1);
+    q->out_bytes = uint64_t{r->total_rows} * r->type_size;
+  }
+  if (rc) return rc;
+  q->t_submit = sv->now_ms();
+  sv->queue.push_back(q.get());
+  sv->reqs[r->tag] = std::move(q);
+  return SDR_OK;
+}
+
+int sdr_server_progress(sdr_server* sv, uint32_t* n_launched)
+{
+  clear_error();
+  if (!sv) return fail(SDR_E_INVALID_ARG, "server is NULL");
+  if (n_launched) *n_launched = 0;
+  int rc = collect(sv);
+  if (rc) return rc;
+  admit(sv);
+  for (auto& sl : sv->slots) {
+    if (sv->admitted.empty()) break;
+    if (sl.busy) continue;
+    rc = launch(sv, sl, n_launched);
+    if (rc) return rc;
+  }
+  return SDR_OK;
+}
+
+int sdr_server_poll(sdr_server* sv, sdr_completion* out, uint32_t cap, uint32_t* n)
+{
+  clear_error();
+  if (!sv || !n || (cap && !out)) return fail(SDR_E_INVALID_ARG, "NULL argument");
+  const uint32_t m = static_cast<uint32_t>(std::min<size_t>(cap, sv->done.size()));
+  std::copy(sv->done.begin(), sv->done.begin() + m, out);
+  sv->done.erase(sv->done.begin(), sv->done.begin() + m);
+  *n = m;
+  return SDR_OK;
+}
+
+int sdr_server_release(sdr_server* sv, uint64_t tag, sdr_stream_t stream)
+{
+  clear_error();
+  if (!sv) return fail(SDR_E_INVALID_ARG, "server is NULL");
+  auto it = sv->reqs.find(tag);
+  if (it == sv->reqs.end()) return fail(SDR_E_NOT_FOUND, "unknown tag %llu", static_cast<unsigned long long>(tag));
+  Req* q = it->second.get();
+  if (!q->done) return fail(SDR_E_STATE, "request %llu has not completed", static_cast<unsigned long long>(tag));
+  if (q->d_alloc) HIP_TRY(hipFreeAsync(q->d_alloc, static_cast<hipStream_t>(stream)));
+  sv->outputs_held -= q->out_bytes;
+  sv->reqs.erase(it);
+  return SDR_OK;
+}
+
+int sdr_server_pending(const sdr_server* sv)
+{
+  if (!sv) return SDR_E_INVALID_ARG;
+  size_t inflight = 0;
+  for (auto& sl : sv->slots) inflight += sl.reqs.size();
+  return static_cast<int>(sv->queue.size() + sv->admitted.size() + inflight);
+}
+
+uint64_t sdr_server_held_bytes(const sdr_server* sv) { return sv ? held(sv) : 0; }
+
+double sdr_server_now_ms(const sdr_server* sv) { return sv ? sv->now_ms() : 0.0; }
+
+int sdr_server_destroy(sdr_server* sv)
+{
+  delete sv;
+  return SDR_OK;
+}
+
+}  // extern "C"

@@ -23,6 +23,9 @@ DECLARED = [
     "sdr_batch_create", "sdr_batch_set_option", "sdr_batch_add_table", "sdr_batch_add_strings",
     "sdr_batch_num_entries", "sdr_batch_launch", "sdr_batch_finish", "sdr_batch_clear", "sdr_batch_destroy",
     "sdr_tune_set", "sdr_tune_get", "sdr_tune_load",
+    "sdr_server_create", "sdr_server_stage", "sdr_server_unstage", "sdr_server_submit", "sdr_server_progress",
+    "sdr_server_poll", "sdr_server_release", "sdr_server_pending", "sdr_server_held_bytes", "sdr_server_now_ms",
+    "sdr_server_destroy",
 ]
 
 _lib = None
@@ -77,6 +80,28 @@ def load() -> C.CDLL:
     lib.sdr_plan_null_counts.restype = i32
     lib.sdr_plan_destroy.argtypes = [vp]
     lib.sdr_plan_destroy.restype = i32
+    lib.sdr_server_create.argtypes = [P(_abi.ServerConfig), P(vp)]
+    lib.sdr_server_create.restype = i32
+    lib.sdr_server_stage.argtypes = [vp, vp, C.c_uint64, P(vp)]
+    lib.sdr_server_stage.restype = i32
+    lib.sdr_server_unstage.argtypes = [vp, vp]
+    lib.sdr_server_unstage.restype = i32
+    lib.sdr_server_submit.argtypes = [vp, P(_abi.Request)]
+    lib.sdr_server_submit.restype = i32
+    lib.sdr_server_progress.argtypes = [vp, P(u32)]
+    lib.sdr_server_progress.restype = i32
+    lib.sdr_server_poll.argtypes = [vp, P(_abi.Completion), u32, P(u32)]
+    lib.sdr_server_poll.restype = i32
+    lib.sdr_server_release.argtypes = [vp, C.c_uint64, vp]
+    lib.sdr_server_release.restype = i32
+    lib.sdr_server_pending.argtypes = [vp]
+    lib.sdr_server_pending.restype = i32
+    lib.sdr_server_held_bytes.argtypes = [vp]
+    lib.sdr_server_held_bytes.restype = C.c_uint64
+    lib.sdr_server_now_ms.argtypes = [vp]
+    lib.sdr_server_now_ms.restype = C.c_double
+    lib.sdr_server_destroy.argtypes = [vp]
+    lib.sdr_server_destroy.restype = i32
     lib.sdr_tune_set.argtypes = [u32, C.c_uint64]
     lib.sdr_tune_set.restype = i32
     lib.sdr_tune_get.argtypes = [u32]

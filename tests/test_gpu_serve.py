@@ -1,0 +1,95 @@
+"""The decode server (sdr_server_*): every served output is bit-exact to the
+pool's reference decode, whether a request is coalesced with others into a
+persistent launch or decoded alone; admission keeps held bytes within the
+HBM budget; the baseline modes serve the same outputs."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import sirius_amd._abi as A
+from sirius_amd import lib as L
+from sirius_amd import serve as S
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+@pytest.fixture(scope="module")
+def setup():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    lib = L.load()
+    cfg = A.ServerConfig(0, 2, A.LAUNCH_PERSISTENT, 0, 0)
+    server = C.c_void_p()
+    L.check(lib.sdr_server_create(C.byref(cfg), C.byref(server)))
+    pool = S.Pool(lib, server, max_rows=1 << 21, n_tails=8, seed=77)
+    yield lib, server, pool
+    torch.cuda.synchronize()
+    lib.sdr_server_destroy(server)
+
+
+def test_mixed_and_alone_are_identical(setup):
+    lib, server, pool = setup
+    specs = S.make_specs(pool, 60, seed=5, max_rows=1 << 21)
+    assert {s.kind for s in specs} == set(S.MIX)
+    S.poisson_arrivals(specs, 0)  # all at once: coalesced into few launches
+    mixed = S.run(lib, pool, specs, A.LAUNCH_PERSISTENT, verify_every=1, server=server)
+    assert mixed["errors"] == 0 and mixed["verified"] == 60 and mixed["verified_ok"] == 60
+    assert mixed["launches"] < 60
+    # one request per launch, one at a time
+    alone = S.run(lib, pool, specs[:24], A.LAUNCH_PERSISTENT, max_inflight=1, max_batch_requests=1,
+                  verify_every=1)
+    assert alone["errors"] == 0 and alone["verified_ok"] == 24 and alone["launches"] == 24
+
+
+def test_poisson_stream_overlaps_launches(setup):
+    lib, server, pool = setup
+    specs = S.make_specs(pool, 80, seed=9, max_rows=1 << 21)
+    S.poisson_arrivals(specs, 2000.0)
+    r = S.run(lib, pool, specs, A.LAUNCH_PERSISTENT, verify_every=1, server=server)
+    assert r["errors"] == 0 and r["verified_ok"] == 80
+    assert r["p99_ms"] >= r["p50_ms"] > 0
+
+
+@pytest.mark.parametrize("mode", [A.LAUNCH_BATCHED, A.LAUNCH_PER_SEGMENT], ids=["per_request", "per_segment"])
+def test_baseline_modes(setup, mode):
+    lib, server, pool = setup
+    specs = S.make_specs(pool, 20, seed=13, max_rows=1 << 20)
+    S.poisson_arrivals(specs, 0)
+    r = S.run(lib, pool, specs, mode, verify_every=1)
+    assert r["errors"] == 0 and r["verified_ok"] == 20 and r["launches"] == 20
+
+
+def test_admission_holds_budget(setup):
+    lib, server, pool = setup
+    specs = [s for s in S.make_specs(pool, 200, seed=21, max_rows=1 << 21) if not pool.src[s.kind].is_string][:30]
+    need = max(s.rows * pool.src[s.kind].enc.type_size for s in specs)
+    budget = 2 * need + 1
+    cfg = A.ServerConfig(budget, 4, A.LAUNCH_PERSISTENT, 0, 0)
+    sv = C.c_void_p()
+    L.check(lib.sdr_server_create(C.byref(cfg), C.byref(sv)))
+    try:
+        reqs = [s.request(pool) for s in specs]
+        for r in reqs:
+            L.check(lib.sdr_server_submit(sv, C.byref(r)))
+        comps = (A.Completion * 64)()
+        n = C.c_uint32()
+        got, peak = 0, 0
+        stream = torch.cuda.current_stream()
+        while got < len(specs):
+            L.check(lib.sdr_server_progress(sv, None))
+            peak = max(peak, lib.sdr_server_held_bytes(sv))
+            L.check(lib.sdr_server_poll(sv, comps, 64, C.byref(n)))
+            for k in range(n.value):
+                c = comps[k]
+                spec = next(s for s in specs if s.tag == c.tag)
+                assert c.status == 0 and S.verify(pool, spec, c, stream)
+                L.check(lib.sdr_server_release(sv, c.tag, stream.cuda_stream))
+                got += 1
+        assert peak <= budget
+        assert lib.sdr_server_pending(sv) == 0 and lib.sdr_server_held_bytes(sv) == 0
+    finally:
+        torch.cuda.synchronize()
+        lib.sdr_server_destroy(sv)
