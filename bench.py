@@ -293,6 +293,8 @@ def main() -> int:
                     help="batched: the drop-in steps and the per-kernel loop only (for profiler runs)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the other workloads")
+    ap.add_argument("--served-requests", type=int, default=10000,
+                    help="requests of the served mixed-codec stream (0: skip it)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported: the decode path runs on a "
@@ -341,6 +343,10 @@ def main() -> int:
                 f"bit-exact={er['bit_exact']}")
             del w
 
+    served_r = None
+    if args.served_requests and args.workload == "int_date_mix_64M":
+        served_r = served(lib, args.served_requests)
+
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(work)
@@ -378,6 +384,7 @@ def main() -> int:
         "bit_exact": r["bit_exact"] and r["plain"]["bit_exact"] and r["persistent"]["bit_exact"],
         "tuning": tuning,
         "workloads": {k: rounded({kk: vv for kk, vv in v.items() if kk != "kernels"}) for k, v in extra.items()},
+        "served": served_r,
         "roofline": None if dom is None else {
             "bound": "hbm", "kernel": dom_name, "achieved": round(dom["gbs"], 1), "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": round(dom["gbs"] / HBM_PEAK_GBS, 4), "traffic": traffic,
@@ -386,6 +393,71 @@ def main() -> int:
     }
     print(json.dumps(line), flush=True)
     return 0
+
+
+def served(lib, n_requests):
+    """The served mixed-codec stream: requests of log-uniform size
+    (2 Ki..16 Mi rows), 40% BITPACKING INT32 / 20% RLE INT64 / 25% ALP DOUBLE
+    / 15% FSST VARCHAR over segments resident in the server's pool, Poisson
+    arrivals at 0.8x the measured capacity of the persistent path, 200 GB
+    admission budget.  Baselines (one drop-in launch per request, one
+    launch per segment) serve the first requests of the same stream at the
+    same rate."""
+    import copy
+
+    import torch
+
+    import sirius_amd._abi as A
+    from sirius_amd import lib as L
+    from sirius_amd import serve as S
+
+    t = time.time()
+    cfg = A.ServerConfig(200 * 10 ** 9, 2, A.LAUNCH_PERSISTENT, 0, 0)
+    server = C.c_void_p()
+    L.check(lib.sdr_server_create(C.byref(cfg), C.byref(server)))
+    try:
+        pool = S.Pool(lib, server, max_rows=16 << 20, n_tails=48, log=log)
+        specs = S.make_specs(pool, n_requests)
+        log(f"[bench] served: pool staged ({pool.staged_bytes / 1e9:.2f} GB resident) in {time.time() - t:.1f}s")
+        # capacity: a prefix of the stream, everything queued at once
+        m = min(len(specs), 1000)
+        cap_specs = copy.deepcopy(specs[:m])
+        S.poisson_arrivals(cap_specs, 0)
+        cap = S.run(lib, pool, cap_specs, A.LAUNCH_PERSISTENT, server=server, verify_every=16)
+        cap_rps = m / (cap["span_ms"] * 1e-3)
+        mean_bytes = cap["decoded_bytes"] / m
+        rate = 0.8 * cap_rps
+        # bound the timed stream to about a minute of arrivals
+        n = min(len(specs), max(m, int(60.0 * rate)))
+        run_specs = specs[:n]
+        S.poisson_arrivals(run_specs, rate)
+        log(f"[bench] served: capacity {cap_rps:.1f} req/s ({cap['decoded_gbs']:.0f} GB/s); "
+            f"serving {n} requests at {rate:.1f} req/s")
+        main = S.run(lib, pool, run_specs, A.LAUNCH_PERSISTENT, server=server, verify_every=8)
+        log(f"[bench] served: {main['decoded_gbs']:.0f} GB/s, p50 {main['p50_ms']:.2f} ms, "
+            f"p99 {main['p99_ms']:.2f} ms, verified {main['verified_ok']}/{main['verified']}")
+        base = {}
+        for name, mode, nb in (("one_launch_per_request", A.LAUNCH_BATCHED, 300),
+                               ("one_launch_per_segment", A.LAUNCH_PER_SEGMENT, 60)):
+            bs = copy.deepcopy(run_specs[:nb])
+            S.poisson_arrivals(bs, rate)
+            base[name] = S.run(lib, pool, bs, mode, verify_every=8)
+            log(f"[bench] served baseline {name}: {base[name]['decoded_gbs']:.0f} GB/s, "
+                f"p50 {base[name]['p50_ms']:.1f} ms, p99 {base[name]['p99_ms']:.1f} ms")
+        torch.cuda.synchronize()
+    finally:
+        lib.sdr_server_destroy(server)
+    return {
+        "workload": "served mixed-codec stream (40% BP INT32, 20% RLE INT64, 25% ALP DOUBLE, 15% FSST VARCHAR; "
+                    "rows log-uniform 2Ki..16Mi; Poisson arrivals; 200 GB budget)",
+        "capacity_req_per_s": round(cap_rps, 2), "capacity_gbs": round(cap["decoded_gbs"], 2),
+        "mean_request_bytes": round(mean_bytes), "rate_req_per_s": round(rate, 2),
+        "persistent": rounded(main),
+        "frac_hbm": round(main["achieved_gbs"] / HBM_PEAK_GBS, 4),
+        "baselines": {k: rounded(v) for k, v in base.items()},
+        "vs_one_launch_per_request": round(main["decoded_gbs"] / base["one_launch_per_request"]["decoded_gbs"], 3),
+        "vs_one_launch_per_segment": round(main["decoded_gbs"] / base["one_launch_per_segment"]["decoded_gbs"], 3),
+    }
 
 
 def cpu_baseline(work):

@@ -118,12 +118,13 @@ struct ByteReader {
   const uint32_t* w;
   uint32_t cur;
   uint32_t boff;
+  // pointer arithmetic (not an integer round trip) keeps an LDS pointer's
+  // address space visible to the compiler, so LDS rows read with ds_read
   __device__ __forceinline__ explicit ByteReader(const uint8_t* p)
   {
-    uintptr_t a = reinterpret_cast<uintptr_t>(p);
-    boff        = static_cast<uint32_t>(a & 3u);
-    w           = reinterpret_cast<const uint32_t*>(a - boff);
-    cur         = *w;
+    boff = static_cast<uint32_t>(reinterpret_cast<uintptr_t>(p) & 3u);
+    w    = reinterpret_cast<const uint32_t*>(p - boff);
+    cur  = *w;
   }
   __device__ __forceinline__ uint32_t next()
   {
@@ -278,10 +279,11 @@ __device__ __forceinline__ void fsst_chunk(const FsstChunk ch, uint32_t c, uint3
   }
 
   // ---- compressed lengths (fsst.cu:132-188): the sum of the segment's
-  // fields before the chunk, and this thread's own fields ----
+  // fields before the chunk, and this thread's own fields.  Thread t owns
+  // rows t, t + 256, t + 512 and t + 768 of the chunk, so every wave has
+  // rows in every stretch of the output the emit below walks. ----
   const uint32_t rows = ch.rows;
-  const uint32_t r0   = tid * ROWS_PER_THR;
-  const uint32_t mine = r0 < rows ? (rows - r0 < ROWS_PER_THR ? rows - r0 : ROWS_PER_THR) : 0u;
+  auto row_of         = [&](uint32_t k) { return k * FSST_THREADS + tid; };
   uint32_t clen[ROWS_PER_THR];
 #pragma unroll
   for (uint32_t k = 0; k < ROWS_PER_THR; ++k) clen[k] = 0;
@@ -304,8 +306,8 @@ __device__ __forceinline__ void fsst_chunk(const FsstChunk ch, uint32_t c, uint3
       for (uint32_t f = f0 + tid; f < pe; f += FSST_THREADS) part += lds_bits32(lw, bit0 + (f - f0) * width, width);
 #pragma unroll
       for (uint32_t k = 0; k < ROWS_PER_THR; ++k) {
-        const uint32_t f = ch.row_begin + r0 + k;
-        if (k < mine && f >= f0 && f < f1) clen[k] = lds_bits32(lw, bit0 + (f - f0) * width, width);
+        const uint32_t f = ch.row_begin + row_of(k);
+        if (row_of(k) < rows && f >= f0 && f < f1) clen[k] = lds_bits32(lw, bit0 + (f - f0) * width, width);
       }
     }
   } else if (ok) {
@@ -318,21 +320,19 @@ __device__ __forceinline__ void fsst_chunk(const FsstChunk ch, uint32_t c, uint3
     }
 #pragma unroll
     for (uint32_t k = 0; k < ROWS_PER_THR; ++k)
-      if (k < mine) clen[k] = stream_bits32(seg + 16, end, uint64_t{ch.row_begin + r0 + k} * width, width);
+      if (row_of(k) < rows) clen[k] = stream_bits32(seg + 16, end, uint64_t{ch.row_begin + row_of(k)} * width, width);
   }
-  uint32_t before = 0, total = 0;
+  uint32_t before = 0;
   block_excl_scan(part, sm.wave, &before);
-  uint32_t acc = 0;
-#pragma unroll
-  for (uint32_t k = 0; k < ROWS_PER_THR; ++k) acc += clen[k];
-  const uint32_t excl = block_excl_scan(acc, sm.wave, &total);
   if (tid == 0) sm.comp[0] = before;
   {
-    uint32_t run = before + excl;
+    uint32_t run = before;  // rows before the current 256-row stride
 #pragma unroll
     for (uint32_t k = 0; k < ROWS_PER_THR; ++k) {
-      run += clen[k];
-      if (r0 + k < rows) sm.comp[r0 + k + 1] = run;
+      uint32_t tot    = 0;
+      const uint32_t ex = block_excl_scan(clen[k], sm.wave, &tot);
+      if (row_of(k) < rows) sm.comp[row_of(k) + 1] = run + ex + clen[k];
+      run += tot;
     }
   }
   __syncthreads();
@@ -347,10 +347,9 @@ __device__ __forceinline__ void fsst_chunk(const FsstChunk ch, uint32_t c, uint3
 
   // ---- decoded lengths (fsst.cu:194-286, serial escape semantics) ----
   uint32_t dlen[ROWS_PER_THR];
-  acc = 0;
 #pragma unroll
   for (uint32_t k = 0; k < ROWS_PER_THR; ++k) {
-    const uint32_t r = r0 + k;
+    const uint32_t r = row_of(k);
     uint32_t d = 0;
     if (ok && r < rows) {
       const uint32_t cum = sm.comp[r + 1], prev = sm.comp[r];
@@ -359,8 +358,8 @@ __device__ __forceinline__ void fsst_chunk(const FsstChunk ch, uint32_t c, uint3
         auto lit = [&](uint32_t) { ++d; return true; };
         auto sym = [&](uint32_t b) { d += sm.len[b]; return true; };
         if (staged && prev >= c_lo && cum <= c_hi) {
-          const uint8_t* lb = sm.cst + c_h + (c_hi - cum);
-          fsst_walk(n, [&](uint32_t i) -> uint32_t { return lb[i]; }, lit, sym);
+          ByteReader lr(sm.cst + c_h + (c_hi - cum));  // aligned LDS words, 4 bytes per read
+          fsst_walk(n, [&](uint32_t) -> uint32_t { return lr.next(); }, lit, sym);
         } else {
           ByteReader br(seg + dict_end - cum);
           fsst_walk(n, [&](uint32_t) -> uint32_t { return br.next(); }, lit, sym);
@@ -368,17 +367,14 @@ __device__ __forceinline__ void fsst_chunk(const FsstChunk ch, uint32_t c, uint3
       }
     }
     dlen[k] = d;
-    acc += d;
   }
   uint32_t T = 0;
-  {
-    const uint32_t dex = block_excl_scan(acc, sm.wave, &T);
-    uint32_t run       = dex;
 #pragma unroll
-    for (uint32_t k = 0; k < ROWS_PER_THR; ++k) {
-      if (r0 + k < rows) sm.dec[r0 + k] = run;
-      run += dlen[k];
-    }
+  for (uint32_t k = 0; k < ROWS_PER_THR; ++k) {
+    uint32_t tot    = 0;
+    const uint32_t ex = block_excl_scan(dlen[k], sm.wave, &tot);
+    if (row_of(k) < rows) sm.dec[row_of(k)] = T + ex;
+    T += tot;
   }
   if (tid == 0) sm.dec[rows] = T;
 
@@ -445,27 +441,38 @@ __device__ __forceinline__ void fsst_chunk(const FsstChunk ch, uint32_t c, uint3
     const uintptr_t pb = pa + FSST_PIECE;
 #pragma unroll
     for (uint32_t k = 0; k < ROWS_PER_THR; ++k) {
-      const uint32_t r = r0 + k;
+      const uint32_t r = row_of(k);
       if (r >= rows || dlen[k] == 0) continue;
       const uintptr_t o0 = g0 + sm.dec[r];
       if (o0 + dlen[k] <= pa || o0 >= pb) continue;
       const uint32_t cum = sm.comp[r + 1], prev = sm.comp[r];
-      uintptr_t o = o0;
+      // o: output position relative to the piece (may start before it)
+      int32_t o = static_cast<int32_t>(static_cast<intptr_t>(o0 - pa));
+      constexpr int32_t PIECE = static_cast<int32_t>(FSST_PIECE);
       auto lit = [&](uint32_t b) {
-        if (o >= pa) sm.out[o - pa] = static_cast<uint8_t>(b);
+        if (o >= 0) sm.out[o] = static_cast<uint8_t>(b);
         ++o;
-        return o < pb;
+        return o < PIECE;
       };
       auto sym = [&](uint32_t b) {
-        const uint32_t l   = sm.len[b];
-        const uint64_t val = static_cast<uint64_t>(sm.lo[b]) | (static_cast<uint64_t>(sm.hi[b]) << 32);
-        for (uint32_t j = 0; j < l; ++j, ++o)
-          if (o >= pa && o < pb) sm.out[o - pa] = static_cast<uint8_t>(val >> (8u * j));
-        return o < pb;
+        const uint32_t l  = sm.len[b];
+        const uint32_t lo = sm.lo[b], hi = sm.hi[b];
+        if (o >= 0 && o + static_cast<int32_t>(l) <= PIECE) {  // the whole symbol lands in the piece
+#pragma unroll
+          for (uint32_t j = 0; j < 8; ++j)
+            if (j < l) sm.out[o + static_cast<int32_t>(j)] = static_cast<uint8_t>((j < 4 ? lo >> (8u * j) : hi >> (8u * (j - 4))));
+        } else {
+          for (uint32_t j = 0; j < l; ++j) {
+            const int32_t q = o + static_cast<int32_t>(j);
+            if (q >= 0 && q < PIECE) sm.out[q] = static_cast<uint8_t>((j < 4 ? lo >> (8u * j) : hi >> (8u * (j - 4))));
+          }
+        }
+        o += static_cast<int32_t>(l);
+        return o < PIECE;
       };
       if (staged && prev >= c_lo && cum <= c_hi) {
-        const uint8_t* lb = sm.cst + c_h + (c_hi - cum);
-        fsst_walk(cum - prev, [&](uint32_t i) -> uint32_t { return lb[i]; }, lit, sym);
+        ByteReader lr(sm.cst + c_h + (c_hi - cum));
+        fsst_walk(cum - prev, [&](uint32_t) -> uint32_t { return lr.next(); }, lit, sym);
       } else {
         ByteReader br(seg + dict_end - cum);
         fsst_walk(cum - prev, [&](uint32_t) -> uint32_t { return br.next(); }, lit, sym);

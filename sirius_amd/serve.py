@@ -161,6 +161,7 @@ class Spec:
     n_src: int      # source segments
     tail: int       # tail column index or -1
     arrival_ms: float = 0.0
+    comp_bytes: int = 0
     keep: list = field(default_factory=list)
 
     def request(self, pool: Pool) -> A.Request:
@@ -178,6 +179,7 @@ class Spec:
                 row += seg.row_count
         runs, keep = A.make_runs([(src.enc.codec, segs)])
         self.keep = [runs, keep]
+        self.comp_bytes = sum(int(g.bytes_size) for g in segs)
         r = A.Request()
         r.tag, r.is_string, r.type_size, r.total_rows = self.tag, int(src.is_string), src.enc.type_size, row
         r.n_data, r.data = 1, runs
@@ -287,7 +289,7 @@ def run(lib, pool: Pool, specs: list, mode: int, max_inflight: int = 2, budget: 
     by_tag = {s.tag: s for s in specs}
     order = sorted(specs, key=lambda s: s.arrival_ms)
     t_base = lib.sdr_server_now_ms(server)
-    lat, done_ms, bytes_out, statuses = [], [], 0, []
+    lat, done_ms, bytes_out, bytes_in, statuses = [], [], 0, 0, []
     n_verified = n_ok = 0
     n_launches = 0
     i = 0
@@ -307,6 +309,7 @@ def run(lib, pool: Pool, specs: list, mode: int, max_inflight: int = 2, budget: 
                 lat.append(c.done_ms - t_base - s.arrival_ms)
                 done_ms.append(c.done_ms - t_base)
                 bytes_out += (s.rows + 1) * 4 + c.chars_used if pool.src[s.kind].is_string else c.out_bytes
+                bytes_in += s.comp_bytes
                 if verify_every and s.tag % verify_every == 0:
                     n_verified += 1
                     n_ok += verify(pool, s, c, stream)
@@ -324,7 +327,8 @@ def run(lib, pool: Pool, specs: list, mode: int, max_inflight: int = 2, budget: 
     span_ms = max(done_ms) - first
     lat = np.asarray(lat)
     return {"requests": len(specs), "launches": n_launches, "decoded_bytes": int(bytes_out),
-            "span_ms": span_ms, "decoded_gbs": bytes_out / (span_ms * 1e-3) / 1e9,
+            "compressed_bytes": int(bytes_in), "span_ms": span_ms, "decoded_gbs": bytes_out / (span_ms * 1e-3) / 1e9,
+            "achieved_gbs": (bytes_out + bytes_in) / (span_ms * 1e-3) / 1e9,
             "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
             "mean_ms": float(lat.mean()), "errors": int(sum(1 for x in statuses if x != 0)),
             "verified": n_verified, "verified_ok": n_ok}
