@@ -436,9 +436,11 @@ def served(lib, n_requests):
         main = S.run(lib, pool, run_specs, A.LAUNCH_PERSISTENT, server=server, verify_every=8)
         log(f"[bench] served: {main['decoded_gbs']:.0f} GB/s, p50 {main['p50_ms']:.2f} ms, "
             f"p99 {main['p99_ms']:.2f} ms, verified {main['verified_ok']}/{main['verified']}")
+        for f in main["failed"]:
+            log(f"[bench] served MISMATCH {f}")
         base = {}
-        for name, mode, nb in (("one_launch_per_request", A.LAUNCH_BATCHED, 300),
-                               ("one_launch_per_segment", A.LAUNCH_PER_SEGMENT, 60)):
+        for name, mode, nb in (("one_launch_per_request", A.LAUNCH_BATCHED, n),
+                               ("one_launch_per_segment", A.LAUNCH_PER_SEGMENT, n)):
             bs = copy.deepcopy(run_specs[:nb])
             S.poisson_arrivals(bs, rate)
             base[name] = S.run(lib, pool, bs, mode, verify_every=8)

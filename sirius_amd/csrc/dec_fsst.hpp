@@ -162,8 +162,8 @@ constexpr uint32_t FSST_SYMSTAGE = 17u + 8u * FSST_NSYM;  // the largest symbol 
 constexpr uint32_t FSST_CSTAGE   = 16384;                  // chunk compressed bytes staged in LDS
 
 struct FsstSmem {
+  uint64_t sym[256];  // symbol bytes, zero above its length
   uint8_t len[256];
-  uint32_t lo[256], hi[256];
   uint32_t comp[FSST_CHUNK_ROWS + 1];  // segment-relative inclusive compressed prefix
   uint32_t dec[FSST_CHUNK_ROWS + 1];   // chunk-relative exclusive decoded offset
   // emit pieces; before that, pieces of the compressed-length stream
@@ -270,12 +270,10 @@ __device__ __forceinline__ void fsst_chunk(const FsstChunk ch, uint32_t c, uint3
       }
     }
     sm.len[tid] = static_cast<uint8_t>(len);
-    sm.lo[tid]  = static_cast<uint32_t>(sym);
-    sm.hi[tid]  = static_cast<uint32_t>(sym >> 32);
+    sm.sym[tid] = sym;
   } else {
     sm.len[tid] = 0;
-    sm.lo[tid]  = 0;
-    sm.hi[tid]  = 0;
+    sm.sym[tid] = 0;
   }
 
   // ---- compressed lengths (fsst.cu:132-188): the sum of the segment's
@@ -449,33 +447,56 @@ __device__ __forceinline__ void fsst_chunk(const FsstChunk ch, uint32_t c, uint3
       // o: output position relative to the piece (may start before it)
       int32_t o = static_cast<int32_t>(static_cast<intptr_t>(o0 - pa));
       constexpr int32_t PIECE = static_cast<int32_t>(FSST_PIECE);
-      auto lit = [&](uint32_t b) {
-        if (o >= 0) sm.out[o] = static_cast<uint8_t>(b);
-        ++o;
-        return o < PIECE;
-      };
-      auto sym = [&](uint32_t b) {
-        const uint32_t l  = sm.len[b];
-        const uint32_t lo = sm.lo[b], hi = sm.hi[b];
-        if (o >= 0 && o + static_cast<int32_t>(l) <= PIECE) {  // the whole symbol lands in the piece
-#pragma unroll
-          for (uint32_t j = 0; j < 8; ++j)
-            if (j < l) sm.out[o + static_cast<int32_t>(j)] = static_cast<uint8_t>((j < 4 ? lo >> (8u * j) : hi >> (8u * (j - 4))));
+      const bool inside = o >= 0 && o + static_cast<int32_t>(dlen[k]) <= PIECE;
+      auto walk = [&](auto&& put) {
+        auto lit = [&](uint32_t b) { return put(b, 1u); };
+        auto sym = [&](uint32_t b) { return put(sm.sym[b], sm.len[b]); };
+        if (staged && prev >= c_lo && cum <= c_hi) {
+          ByteReader lr(sm.cst + c_h + (c_hi - cum));
+          fsst_walk(cum - prev, [&](uint32_t) -> uint32_t { return lr.next(); }, lit, sym);
         } else {
-          for (uint32_t j = 0; j < l; ++j) {
-            const int32_t q = o + static_cast<int32_t>(j);
-            if (q >= 0 && q < PIECE) sm.out[q] = static_cast<uint8_t>((j < 4 ? lo >> (8u * j) : hi >> (8u * (j - 4))));
-          }
+          ByteReader br(seg + dict_end - cum);
+          fsst_walk(cum - prev, [&](uint32_t) -> uint32_t { return br.next(); }, lit, sym);
         }
-        o += static_cast<int32_t>(l);
-        return o < PIECE;
       };
-      if (staged && prev >= c_lo && cum <= c_hi) {
-        ByteReader lr(sm.cst + c_h + (c_hi - cum));
-        fsst_walk(cum - prev, [&](uint32_t) -> uint32_t { return lr.next(); }, lit, sym);
+      if (inside) {
+        // The row lies inside the piece: its first bytes up to a dword
+        // boundary and its last < 4 bytes are written singly (neighbouring
+        // rows share those dwords); everything between leaves a 16-byte
+        // register window as aligned dword stores.
+        uint64_t wa = 0, wb = 0;  // pending bytes, starting at dword-aligned oa
+        uint32_t na = 0;
+        int32_t oa  = o;
+        auto put = [&](uint64_t v, uint32_t l) {
+          while (na == 0 && (oa & 3) && l) {
+            sm.out[oa++] = static_cast<uint8_t>(v);
+            v >>= 8;
+            --l;
+          }
+          if (l) {
+            wa |= v << (8u * na);
+            wb |= na ? v >> (64u - 8u * na) : 0ull;
+            na += l;
+            while (na >= 4u) {
+              *reinterpret_cast<uint32_t*>(&sm.out[oa]) = static_cast<uint32_t>(wa);
+              wa = (wa >> 32) | (wb << 32);
+              wb >>= 32;
+              oa += 4;
+              na -= 4u;
+            }
+          }
+          return true;
+        };
+        walk(put);
+        for (uint32_t j = 0; j < na; ++j) sm.out[oa + static_cast<int32_t>(j)] = static_cast<uint8_t>(wa >> (8u * j));
       } else {
-        ByteReader br(seg + dict_end - cum);
-        fsst_walk(cum - prev, [&](uint32_t) -> uint32_t { return br.next(); }, lit, sym);
+        // a row crossing a piece edge: bytes outside the piece are skipped
+        auto put = [&](uint64_t v, uint32_t l) {
+          for (uint32_t j = 0; j < l; ++j, ++o)
+            if (o >= 0 && o < PIECE) sm.out[o] = static_cast<uint8_t>(v >> (8u * j));
+          return o < PIECE;
+        };
+        walk(put);
       }
     }
     __syncthreads();

@@ -26,6 +26,7 @@ struct Req {
   uint64_t out_bytes  = 0;
   uint64_t chars_cap  = 0;
   uint8_t* d_alloc    = nullptr;
+  hipStream_t stream  = nullptr;  // launch stream the outputs were allocated on
   double t_submit     = 0;
   int status          = SDR_OK;
   bool done           = false;
@@ -104,6 +105,7 @@ uint64_t held(const sdr_server* sv) { return sv->resident + sv->outputs_held; }
 int allocate_outputs(sdr_server* sv, Req* q, hipStream_t s)
 {
   HIP_TRY(hipMallocFromPoolAsync(reinterpret_cast<void**>(&q->d_alloc), std::max<uint64_t>(q->out_bytes, 256), sv->pool, s));
+  q->stream = s;
   if (q->r.is_string) {
     const uint64_t off_bytes = (uint64_t{q->r.total_rows} + 1u) * 4u;
     q->scol.d_offsets        = reinterpret_cast<int32_t*>(q->d_alloc);
@@ -237,6 +239,12 @@ int sdr_server_create(const sdr_server_config* config, sdr_server** out)
   HIP_TRY(hipMemPoolCreate(&sv->pool, &props));
   uint64_t keep = UINT64_MAX;  // a warm pool never returns memory to the driver
   HIP_TRY(hipMemPoolSetAttribute(sv->pool, hipMemPoolAttrReleaseThreshold, &keep));
+  // Outputs are freed on the launch stream they were allocated on (after a
+  // wait on the consumer's stream); memory is reused only in that stream's
+  // order, never opportunistically by another launch stream.
+  int no = 0;
+  HIP_TRY(hipMemPoolSetAttribute(sv->pool, hipMemPoolReuseAllowOpportunistic, &no));
+  HIP_TRY(hipMemPoolSetAttribute(sv->pool, hipMemPoolReuseAllowInternalDependencies, &no));
   HIP_TRY(hipStreamCreateWithFlags(&sv->aux, hipStreamNonBlocking));
   sv->slots.resize(sv->cfg.max_inflight);
   for (auto& sl : sv->slots) {
@@ -386,7 +394,19 @@ int sdr_server_release(sdr_server* sv, uint64_t tag, sdr_stream_t stream)
   if (it == sv->reqs.end()) return fail(SDR_E_NOT_FOUND, "unknown tag %llu", static_cast<unsigned long long>(tag));
   Req* q = it->second.get();
   if (!q->done) return fail(SDR_E_STATE, "request %llu has not completed", static_cast<unsigned long long>(tag));
-  if (q->d_alloc) HIP_TRY(hipFreeAsync(q->d_alloc, static_cast<hipStream_t>(stream)));
+  if (q->d_alloc) {
+    // free in the launch stream's order, after the consumer's work so far
+    hipStream_t consumer = static_cast<hipStream_t>(stream);
+    if (consumer != q->stream) {
+      hipEvent_t ev = nullptr;
+      HIP_TRY(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+      hipError_t e = hipEventRecord(ev, consumer);
+      if (e == hipSuccess) e = hipStreamWaitEvent(q->stream, ev, 0);
+      (void)hipEventDestroy(ev);  // released once it has completed
+      if (e != hipSuccess) return fail(SDR_E_HIP, "release: %s", hipGetErrorString(e));
+    }
+    HIP_TRY(hipFreeAsync(q->d_alloc, q->stream));
+  }
   sv->outputs_held -= q->out_bytes;
   sv->reqs.erase(it);
   return SDR_OK;

@@ -73,6 +73,7 @@ class Column:
     enc: E.Encoded
     d_base: int  # device address of the staged blob
     ref: object = None  # torch uint8 (fixed) or (offsets int32, chars uint8)
+    ref_offs: object = None  # VARCHAR: host int64 copy of the reference offsets
 
     @property
     def is_string(self) -> bool:
@@ -138,6 +139,7 @@ class Pool:
                   and torch.equal(d_offs.cpu(), torch.from_numpy(offs.astype(np.int32)))
                   and torch.equal(d_chars[: len(chars)].cpu(), torch.from_numpy(chars)))
             col.ref = (d_offs, d_chars)
+            col.ref_offs = offs.astype(np.int64)  # host copy: slicing bounds without a device sync
         else:
             out = torch.empty(enc.n_rows * enc.type_size, dtype=torch.uint8, device=dev)
             c = A.Column()
@@ -229,52 +231,86 @@ def poisson_arrivals(specs: list, rate_per_s: float, seed: int = 0xC0DE0005, t0_
         s.arrival_ms = float(a)
 
 
-def verify(pool: Pool, spec: Spec, comp: A.Completion, stream) -> bool:
-    """Copy the request's output to torch buffers and compare it with the
-    pool's reference decode, on the device."""
+class _DeviceView:
+    """Zero-copy torch view of device bytes through __cuda_array_interface__."""
+
+    def __init__(self, ptr: int, nbytes: int):
+        self.__cuda_array_interface__ = {"shape": (nbytes,), "typestr": "|u1", "data": (ptr, False),
+                                         "version": 3, "strides": None}
+
+
+_ZERO_COPY = True
+
+
+def _view(ptr: int, nbytes: int):
+    """uint8 tensor over device bytes: a zero-copy view when torch accepts
+    the array interface, else a stream-ordered copy."""
+    global _ZERO_COPY
     import torch
 
-    hip = _hip()
+    if nbytes == 0:
+        return torch.empty(0, dtype=torch.uint8, device="cuda:0")
+    if _ZERO_COPY:
+        try:
+            return torch.as_tensor(_DeviceView(ptr, nbytes), device="cuda:0")
+        except (TypeError, RuntimeError, ValueError):
+            _ZERO_COPY = False
+    t = torch.empty(nbytes, dtype=torch.uint8, device="cuda:0")
+    if _hip().hipMemcpyAsync(t.data_ptr(), ptr, nbytes, 3, torch.cuda.current_stream().cuda_stream) != 0:
+        raise RuntimeError("hipMemcpyAsync failed")
+    return t
+
+
+def mismatches(pool: Pool, spec: Spec, comp: A.Completion):
+    """Device tensor counting the bytes of the request's output that differ
+    from the pool's reference decode (enqueued on torch's current stream,
+    no host synchronisation)."""
+    import torch
+
     src = pool.src[spec.kind]
-    dev = torch.device("cuda:0")
-    sh = stream.cuda_stream
-
-    def grab(ptr, nbytes):
-        t = torch.empty(max(1, nbytes), dtype=torch.uint8, device=dev)
-        if nbytes and hip.hipMemcpyAsync(t.data_ptr(), ptr, nbytes, 3, sh) != 0:
-            raise RuntimeError("hipMemcpyAsync failed")
-        return t[:nbytes]
-
     cum = src.cum
     r0 = int(cum[spec.src_first])
     ns = int(cum[spec.src_first + spec.n_src]) - r0
     if src.is_string:
-        offs = grab(comp.d_out, (spec.rows + 1) * 4).view(torch.int32)
-        chars = grab(comp.d_chars, comp.chars_used)
+        ro_h = src.ref_offs
         ro, rc = src.ref
-        c0 = int(ro[r0].item())
+        c0, c1 = int(ro_h[r0]), int(ro_h[r0 + ns])
         exp_o = [ro[r0:r0 + ns + 1] - c0]
-        exp_c = [rc[c0:int(ro[r0 + ns].item())]]
-        base = int(exp_o[0][-1].item()) if ns else 0
+        exp_c = [rc[c0:c1]]
         if spec.tail >= 0:
-            to, tc = pool.tails[spec.kind][spec.tail].ref
-            exp_o.append(to[1:] + base)
-            exp_c.append(tc[: int(to[-1].item())])
-        eo = torch.cat(exp_o) if ns else torch.cat([torch.zeros(1, dtype=torch.int32, device=dev)] + exp_o[1:])
+            tcol = pool.tails[spec.kind][spec.tail]
+            to, tc = tcol.ref
+            exp_o.append(to[1:] + (c1 - c0))
+            exp_c.append(tc[: int(tcol.ref_offs[-1])])
+        eo = torch.cat(exp_o)
         ec = torch.cat(exp_c)
-        return bool(torch.equal(offs, eo) and comp.chars_used == ec.numel() and torch.equal(chars, ec))
+        offs = _view(comp.d_out, (spec.rows + 1) * 4).view(torch.int32)
+        bad = (offs != eo).sum()
+        if comp.chars_used != ec.numel():
+            return bad + 1
+        return bad + (_view(comp.d_chars, comp.chars_used) != ec).sum()
     ts = src.enc.type_size
-    out = grab(comp.d_out, spec.rows * ts)
     parts = [src.ref[r0 * ts:(r0 + ns) * ts]]
     if spec.tail >= 0:
         parts.append(pool.tails[spec.kind][spec.tail].ref)
-    return bool(torch.equal(out, torch.cat(parts)))
+    return (_view(comp.d_out, spec.rows * ts) != torch.cat(parts)).sum()
+
+
+def verify(pool: Pool, spec: Spec, comp: A.Completion, stream=None) -> bool:
+    """True when the request's output equals the pool's reference decode."""
+    return int(mismatches(pool, spec, comp).item()) == 0
 
 
 def run(lib, pool: Pool, specs: list, mode: int, max_inflight: int = 2, budget: int = 0,
         max_batch_requests: int = 0, verify_every: int = 8, server=None, log=None) -> dict:
     """Serve `specs` (arrival_ms relative to the start of the run) and return
-    throughput, latency percentiles and verification counts."""
+    throughput, latency percentiles and verification counts.
+
+    Request descriptors are built before the clock starts; each request is
+    submitted when its arrival time has passed (all at once when every
+    arrival is 0, which measures capacity).  Sampled outputs are compared
+    with the reference on torch's stream without host synchronisation and
+    released after the comparison in stream order."""
     import torch
 
     own = server is None
@@ -283,21 +319,32 @@ def run(lib, pool: Pool, specs: list, mode: int, max_inflight: int = 2, budget: 
         server = C.c_void_p()
         L.check(lib.sdr_server_create(C.byref(cfg), C.byref(server)))
     stream = torch.cuda.current_stream()
+    sh = stream.cuda_stream
     comps = (A.Completion * 256)()
     n_out = C.c_uint32()
     launched = C.c_uint32()
     by_tag = {s.tag: s for s in specs}
     order = sorted(specs, key=lambda s: s.arrival_ms)
-    t_base = lib.sdr_server_now_ms(server)
-    lat, done_ms, bytes_out, bytes_in, statuses = [], [], 0, 0, []
-    n_verified = n_ok = 0
+    reqs = [s.request(pool) for s in order]
+    is_str = {k: pool.src[k].is_string for k in pool.src}
+    lat, done_ms, statuses = [], [], []
+    bytes_out = bytes_in = 0
+    n_verified = 0
+    bad = torch.zeros((), dtype=torch.int64, device="cuda:0")
+    checks = []  # (tag, mismatch count on the device) per verified request
     n_launches = 0
     i = 0
     try:
+        t_base = lib.sdr_server_now_ms(server)
+        if all(s.arrival_ms <= 0 for s in order):  # capacity: queue everything first
+            for r in reqs:
+                L.check(lib.sdr_server_submit(server, C.byref(r)))
+            i = len(order)
+            t_base = lib.sdr_server_now_ms(server)
         while len(lat) < len(specs):
             now = lib.sdr_server_now_ms(server) - t_base
             while i < len(order) and order[i].arrival_ms <= now:
-                L.check(lib.sdr_server_submit(server, C.byref(order[i].request(pool))))
+                L.check(lib.sdr_server_submit(server, C.byref(reqs[i])))
                 i += 1
             L.check(lib.sdr_server_progress(server, C.byref(launched)))
             n_launches += launched.value
@@ -306,15 +353,16 @@ def run(lib, pool: Pool, specs: list, mode: int, max_inflight: int = 2, budget: 
                 c = comps[k]
                 s = by_tag[c.tag]
                 statuses.append(c.status)
-                lat.append(c.done_ms - t_base - s.arrival_ms)
+                lat.append(c.done_ms - t_base - max(s.arrival_ms, 0.0))
                 done_ms.append(c.done_ms - t_base)
-                bytes_out += (s.rows + 1) * 4 + c.chars_used if pool.src[s.kind].is_string else c.out_bytes
+                bytes_out += (s.rows + 1) * 4 + c.chars_used if is_str[s.kind] else c.out_bytes
                 bytes_in += s.comp_bytes
-                if verify_every and s.tag % verify_every == 0:
+                if verify_every and s.tag % verify_every == 0 and c.status == 0:
                     n_verified += 1
-                    n_ok += verify(pool, s, c, stream)
-                L.check(lib.sdr_server_release(server, c.tag, stream.cuda_stream))
-                s.keep = []
+                    mm = mismatches(pool, s, c)
+                    bad += mm
+                    checks.append((s.tag, mm))
+                L.check(lib.sdr_server_release(server, c.tag, sh))
             if n_out.value == 0 and launched.value == 0:
                 nxt = order[i].arrival_ms if i < len(order) else None
                 if nxt is not None and nxt - now > 0.2:
@@ -323,12 +371,23 @@ def run(lib, pool: Pool, specs: list, mode: int, max_inflight: int = 2, budget: 
         torch.cuda.synchronize()
         if own:
             lib.sdr_server_destroy(server)
-    first = min(s.arrival_ms for s in specs)
+    for s in specs:
+        s.keep = []
+    first = max(min(s.arrival_ms for s in specs), 0.0)
     span_ms = max(done_ms) - first
     lat = np.asarray(lat)
-    return {"requests": len(specs), "launches": n_launches, "decoded_bytes": int(bytes_out),
+    n_bad = int(bad.item())
+    failed = []
+    if n_bad:
+        counts = torch.stack([m for _, m in checks]).cpu().numpy()
+        for (tag, _), cnt in zip(checks, counts):
+            if cnt:
+                sp = by_tag[tag]
+                failed.append({"tag": tag, "kind": sp.kind, "rows": sp.rows, "src_first": sp.src_first,
+                               "n_src": sp.n_src, "tail": sp.tail, "bad_bytes": int(cnt)})
+    return {"failed": failed[:20], "requests": len(specs), "launches": n_launches, "decoded_bytes": int(bytes_out),
             "compressed_bytes": int(bytes_in), "span_ms": span_ms, "decoded_gbs": bytes_out / (span_ms * 1e-3) / 1e9,
             "achieved_gbs": (bytes_out + bytes_in) / (span_ms * 1e-3) / 1e9,
             "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
             "mean_ms": float(lat.mean()), "errors": int(sum(1 for x in statuses if x != 0)),
-            "verified": n_verified, "verified_ok": n_ok}
+            "verified": n_verified, "verified_ok": n_verified if n_bad == 0 else -1, "mismatched_bytes": n_bad}
