@@ -194,6 +194,35 @@ __device__ __forceinline__ void fsst_walk(uint32_t n, Fetch fetch, Lit lit, Sym 
   }
 }
 
+// Walk the n compressed bytes of one row at p (LDS or global memory) a
+// word at a time: emit(value, length) for every byte -- a symbol, the
+// literal after an escape (length 1), or nothing (length 0) for an escape.
+// The body has no data-dependent branch, so lanes of a wave stay together
+// except for the row length.  Reads only the aligned words holding the row.
+template <typename Emit>
+__device__ __forceinline__ void fsst_row(const uint8_t* p, uint32_t n, const FsstSmem& sm, Emit emit)
+{
+  const uint32_t boff = static_cast<uint32_t>(reinterpret_cast<uintptr_t>(p) & 3u);
+  const uint32_t* w   = reinterpret_cast<const uint32_t*>(p - boff);
+  const uint32_t end  = boff + n;
+  bool esc            = false;
+  for (uint32_t q = 0; q * 4u < end; ++q) {
+    const uint32_t word = w[q];
+#pragma unroll
+    for (uint32_t j = 0; j < 4; ++j) {
+      const uint32_t idx = q * 4u + j;
+      const uint32_t b   = (word >> (8u * j)) & 0xFFu;
+      const bool lit     = esc;
+      const uint32_t l   = lit ? 1u : sm.len[b];  // len[255] == 0: an escape emits nothing
+      const uint64_t v   = lit ? uint64_t{b} : sm.sym[b];
+      if (idx >= boff && idx < end) {
+        emit(v, l);
+        esc = !lit && b == FSST_ESC;
+      }
+    }
+  }
+}
+
 // Decode chunk `c` (its position in the column's chunk order) with the
 // calling 256-thread workgroup.  Every thread must call it.
 //
@@ -353,15 +382,9 @@ __device__ __forceinline__ void fsst_chunk(const FsstChunk ch, uint32_t c, uint3
       const uint32_t cum = sm.comp[r + 1], prev = sm.comp[r];
       if (cum <= dict_end && prev <= cum && cum != prev) {
         const uint32_t n = cum - prev;
-        auto lit = [&](uint32_t) { ++d; return true; };
-        auto sym = [&](uint32_t b) { d += sm.len[b]; return true; };
-        if (staged && prev >= c_lo && cum <= c_hi) {
-          ByteReader lr(sm.cst + c_h + (c_hi - cum));  // aligned LDS words, 4 bytes per read
-          fsst_walk(n, [&](uint32_t) -> uint32_t { return lr.next(); }, lit, sym);
-        } else {
-          ByteReader br(seg + dict_end - cum);
-          fsst_walk(n, [&](uint32_t) -> uint32_t { return br.next(); }, lit, sym);
-        }
+        auto add = [&](uint64_t, uint32_t l) { d += l; };
+        if (staged && prev >= c_lo && cum <= c_hi) fsst_row(sm.cst + c_h + (c_hi - cum), n, sm, add);
+        else fsst_row(seg + dict_end - cum, n, sm, add);
       }
     }
     dlen[k] = d;
@@ -463,21 +486,25 @@ __device__ __forceinline__ void fsst_chunk(const FsstChunk ch, uint32_t c, uint3
         // The row lies inside the piece: its first bytes up to a dword
         // boundary and its last < 4 bytes are written singly (neighbouring
         // rows share those dwords); everything between leaves a 16-byte
-        // register window as aligned dword stores.
+        // register window as aligned dword stores.  Fixed-trip loops only.
         uint64_t wa = 0, wb = 0;  // pending bytes, starting at dword-aligned oa
         uint32_t na = 0;
         int32_t oa  = o;
         auto put = [&](uint64_t v, uint32_t l) {
-          while (na == 0 && (oa & 3) && l) {
-            sm.out[oa++] = static_cast<uint8_t>(v);
-            v >>= 8;
-            --l;
+#pragma unroll
+          for (uint32_t h = 0; h < 3; ++h) {
+            if (na == 0 && (oa & 3) && l) {
+              sm.out[oa++] = static_cast<uint8_t>(v);
+              v >>= 8;
+              --l;
+            }
           }
-          if (l) {
-            wa |= v << (8u * na);
-            wb |= na ? v >> (64u - 8u * na) : 0ull;
-            na += l;
-            while (na >= 4u) {
+          wa |= l ? v << (8u * na) : 0ull;
+          wb |= (l && na) ? v >> (64u - 8u * na) : 0ull;
+          na += l;
+#pragma unroll
+          for (uint32_t f = 0; f < 2; ++f) {
+            if (na >= 4u) {
               *reinterpret_cast<uint32_t*>(&sm.out[oa]) = static_cast<uint32_t>(wa);
               wa = (wa >> 32) | (wb << 32);
               wb >>= 32;
@@ -485,10 +512,12 @@ __device__ __forceinline__ void fsst_chunk(const FsstChunk ch, uint32_t c, uint3
               na -= 4u;
             }
           }
-          return true;
         };
-        walk(put);
-        for (uint32_t j = 0; j < na; ++j) sm.out[oa + static_cast<int32_t>(j)] = static_cast<uint8_t>(wa >> (8u * j));
+        if (staged && prev >= c_lo && cum <= c_hi) fsst_row(sm.cst + c_h + (c_hi - cum), cum - prev, sm, put);
+        else fsst_row(seg + dict_end - cum, cum - prev, sm, put);
+#pragma unroll
+        for (uint32_t j = 0; j < 3; ++j)
+          if (j < na) sm.out[oa + static_cast<int32_t>(j)] = static_cast<uint8_t>(wa >> (8u * j));
       } else {
         // a row crossing a piece edge: bytes outside the piece are skipped
         auto put = [&](uint64_t v, uint32_t l) {
