@@ -15,8 +15,18 @@
 // ever waits on chunks already running: each chunk publishes its
 // decoded byte count, looks back over its predecessors for the column-wide
 // exclusive prefix (a decoupled look-back), writes its rows' int32 offsets
-// and then its chars.  The compressed bytes are read from HBM once: the
-// second walk (emit) hits the L1/L2 lines the first walk (lengths) loaded.
+// and then its chars.
+//
+// The chunk's compressed bytes are staged in LDS once and walked twice, in
+// storage order, by spans: thread t takes the t-th 16-byte-aligned slice of
+// the staged bytes whatever rows it cuts, so every lane of a wave does the
+// same number of byte steps however the row lengths vary.  The first walk
+// sums decoded lengths and records, at every 16-byte grid point, the walk
+// state (decoded prefix, escape flag, row); the second walk restarts from
+// those states and ORs each symbol into a zeroed LDS copy of the chunk's
+// output, which leaves in 16-byte stores.  Chunks whose compressed bytes do
+// not fit the stage, malformed ones and rows longer than the output window
+// take a row-per-thread path that reads global memory.
 #pragma once
 
 #include "sdr_device.hpp"
@@ -27,7 +37,7 @@ namespace sdr {
 constexpr uint32_t FSST_THREADS   = 256;
 constexpr uint32_t FSST_WAVES     = FSST_THREADS / WAVE;
 constexpr uint32_t ROWS_PER_THR   = FSST_CHUNK_ROWS / FSST_THREADS;
-constexpr uint32_t FSST_PIECE     = 8192;  // LDS output staging bytes
+constexpr uint32_t FSST_PIECE     = 8192;  // LDS output piece of the row-wise path
 constexpr uint32_t FSST_ESC       = 255;
 constexpr uint32_t FSST_NSYM      = 255;
 constexpr uint64_t FSST_VERSION   = 20190218ull;
@@ -118,8 +128,6 @@ struct ByteReader {
   const uint32_t* w;
   uint32_t cur;
   uint32_t boff;
-  // pointer arithmetic (not an integer round trip) keeps an LDS pointer's
-  // address space visible to the compiler, so LDS rows read with ds_read
   __device__ __forceinline__ explicit ByteReader(const uint8_t* p)
   {
     boff = static_cast<uint32_t>(reinterpret_cast<uintptr_t>(p) & 3u);
@@ -156,21 +164,36 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* s_wave
   return before + incl - v;
 }
 
-
 // Shared memory of one chunk decode.
 constexpr uint32_t FSST_SYMSTAGE = 17u + 8u * FSST_NSYM;  // the largest symbol table (2057 bytes)
 constexpr uint32_t FSST_CSTAGE   = 16384;                  // chunk compressed bytes staged in LDS
+constexpr uint32_t FSST_NGRID    = (FSST_CSTAGE + 16u) / 16u + 2u;
+constexpr uint32_t FSST_OUT_COLUMN = 32768;  // output window of the per-column kernel
+constexpr uint32_t FSST_OUT_BATCH  = 16384;  // ... of the persistent batch kernel
 
-struct FsstSmem {
+// Walk state at a grid point: decoded prefix (bits 0-19), escape flag (bit
+// 20), row (bits 21-31).  A staged chunk decodes to at most 8 * FSST_CSTAGE
+// bytes and holds at most 1024 rows, so the fields never overflow.
+constexpr uint32_t GS_ESC   = 1u << 20;
+constexpr uint32_t GS_ROW   = 21;
+constexpr uint32_t GS_S     = GS_ESC - 1u;
+constexpr uint32_t ROW_HUGE = 1u << 20;  // a staged row is shorter than this
+
+template <uint32_t OUT_CAP>
+struct FsstSmemT {
   uint64_t sym[256];  // symbol bytes, zero above its length
   uint8_t len[256];
   uint32_t comp[FSST_CHUNK_ROWS + 1];  // segment-relative inclusive compressed prefix
   uint32_t dec[FSST_CHUNK_ROWS + 1];   // chunk-relative exclusive decoded offset
-  // emit pieces; before that, pieces of the compressed-length stream
-  __attribute__((aligned(16))) uint8_t out[FSST_PIECE + 48];
+  uint32_t gst[FSST_NGRID];            // walk state at each 16-byte grid point of the stage
+  uint32_t part[FSST_THREADS];         // decoded bytes before each thread's span
+  // output window; before that, pieces of the compressed-length stream
+  __attribute__((aligned(16))) uint8_t out[(OUT_CAP > FSST_PIECE ? OUT_CAP : FSST_PIECE) + 48];
   // the symbol table, then the chunk's compressed bytes
   __attribute__((aligned(16))) uint8_t cst[FSST_CSTAGE + 48];
   uint32_t wave[FSST_WAVES];
+  uint32_t win;
+  uint32_t flag;
   uint64_t base;
 };
 
@@ -194,33 +217,413 @@ __device__ __forceinline__ void fsst_walk(uint32_t n, Fetch fetch, Lit lit, Sym 
   }
 }
 
-// Walk the n compressed bytes of one row at p (LDS or global memory) a
-// word at a time: emit(value, length) for every byte -- a symbol, the
-// literal after an escape (length 1), or nothing (length 0) for an escape.
-// The body has no data-dependent branch, so lanes of a wave stay together
-// except for the row length.  Reads only the aligned words holding the row.
-template <typename Emit>
-__device__ __forceinline__ void fsst_row(const uint8_t* p, uint32_t n, const FsstSmem& sm, Emit emit)
+// Decoded length of the n compressed bytes of one row at p, read a word at
+// a time (only the aligned words holding the row).
+template <class SM>
+__device__ __forceinline__ uint32_t fsst_row_len(const uint8_t* p, uint32_t n, const SM& sm)
 {
   const uint32_t boff = static_cast<uint32_t>(reinterpret_cast<uintptr_t>(p) & 3u);
   const uint32_t* w   = reinterpret_cast<const uint32_t*>(p - boff);
   const uint32_t end  = boff + n;
   bool esc            = false;
+  uint32_t d          = 0;
   for (uint32_t q = 0; q * 4u < end; ++q) {
     const uint32_t word = w[q];
 #pragma unroll
     for (uint32_t j = 0; j < 4; ++j) {
       const uint32_t idx = q * 4u + j;
       const uint32_t b   = (word >> (8u * j)) & 0xFFu;
-      const bool lit     = esc;
-      const uint32_t l   = lit ? 1u : sm.len[b];  // len[255] == 0: an escape emits nothing
-      const uint64_t v   = lit ? uint64_t{b} : sm.sym[b];
       if (idx >= boff && idx < end) {
-        emit(v, l);
-        esc = !lit && b == FSST_ESC;
+        d += esc ? 1u : sm.len[b];  // len[255] == 0: an escape emits nothing
+        esc = !esc && b == FSST_ESC;
       }
     }
   }
+  return d;
+}
+
+// ---- row-per-thread path (unstaged or malformed chunks) ----
+
+// dec[] and the decoded total of a chunk, one row per thread at a time,
+// reading the compressed bytes from global memory.  Rows whose compressed
+// range leaves [0, dict_end) decode to nothing.
+template <class SM>
+__device__ __attribute__((noinline)) uint32_t fsst_lengths_rows(SM& sm, bool ok, const uint8_t* seg,
+                                                                uint32_t dict_end, uint32_t rows)
+{
+  uint32_t T = 0;
+#pragma unroll 1
+  for (uint32_t k = 0; k < ROWS_PER_THR; ++k) {
+    const uint32_t r = k * FSST_THREADS + threadIdx.x;
+    uint32_t d       = 0;
+    if (ok && r < rows) {
+      const uint32_t cum = sm.comp[r + 1], prev = sm.comp[r];
+      if (cum <= dict_end && prev <= cum && cum != prev) d = fsst_row_len(seg + dict_end - cum, cum - prev, sm);
+    }
+    uint32_t tot      = 0;
+    const uint32_t ex = block_excl_scan(d, sm.wave, &tot);
+    if (r < rows) sm.dec[r] = T + ex;
+    T += tot;
+  }
+  if (threadIdx.x == 0) sm.dec[rows] = T;
+  __syncthreads();
+  return T;
+}
+
+// Chars of a chunk from dec[], one row per thread, in 8 KiB LDS pieces
+// aligned to the absolute chars address: whole 16-byte blocks leave with
+// vector stores, the partial first/last blocks byte-wise (they share 16
+// bytes with the neighbours).
+template <class SM>
+__device__ __attribute__((noinline)) void fsst_emit_rows(SM& sm, const uint8_t* seg, uint32_t dict_end,
+                                                        uint32_t rows, uint32_t T, uint64_t base, uint8_t* chars,
+                                                        uint64_t chars_cap)
+{
+  const uint32_t tid = threadIdx.x;
+  const uintptr_t cb = reinterpret_cast<uintptr_t>(chars);
+  const uintptr_t g0 = cb + base;
+  const uintptr_t g1 = cb + ((base + T < chars_cap) ? base + T : chars_cap);
+  constexpr int32_t PIECE = static_cast<int32_t>(FSST_PIECE);
+  for (uintptr_t pa = g0 & ~uintptr_t{15}; pa < g1; pa += FSST_PIECE) {
+    const uintptr_t pb = pa + FSST_PIECE;
+#pragma unroll 1
+    for (uint32_t k = 0; k < ROWS_PER_THR; ++k) {
+      const uint32_t r = k * FSST_THREADS + tid;
+      if (r >= rows) continue;
+      const uint32_t d0 = sm.dec[r], dl = sm.dec[r + 1] - d0;
+      const uintptr_t o0 = g0 + d0;
+      if (dl == 0 || o0 + dl <= pa || o0 >= pb) continue;
+      const uint32_t cum = sm.comp[r + 1], prev = sm.comp[r];
+      int32_t o = static_cast<int32_t>(static_cast<intptr_t>(o0 - pa));  // may start before the piece
+      auto put = [&](uint64_t v, uint32_t l) {
+        for (uint32_t j = 0; j < l; ++j, ++o)
+          if (o >= 0 && o < PIECE) sm.out[o] = static_cast<uint8_t>(v >> (8u * j));
+        return o < PIECE;
+      };
+      ByteReader br(seg + dict_end - cum);
+      fsst_walk(cum - prev, [&](uint32_t) -> uint32_t { return br.next(); },
+                [&](uint32_t b) { return put(b, 1u); }, [&](uint32_t b) { return put(sm.sym[b], sm.len[b]); });
+    }
+    __syncthreads();
+    const uintptr_t lo = pa > g0 ? pa : g0;
+    const uintptr_t hi = pb < g1 ? pb : g1;
+    for (uintptr_t blk = pa + uintptr_t{tid} * 16u; blk < hi; blk += uintptr_t{FSST_THREADS} * 16u) {
+      if (blk >= lo && blk + 16u <= hi) {
+        const u32x4 v = *reinterpret_cast<const u32x4*>(&sm.out[blk - pa]);
+        __builtin_nontemporal_store(v, reinterpret_cast<u32x4*>(blk));
+      } else {
+        for (uintptr_t x = blk; x < blk + 16u; ++x)
+          if (x >= lo && x < hi) *reinterpret_cast<uint8_t*>(x) = sm.out[x - pa];
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// ---- span path (staged chunks) ----
+//
+// Stage coordinates: the chunk's compressed bytes sit at cst[c_h, c_h + C);
+// row r occupies [top - comp[r + 1], top - comp[r]) with top = c_h + c_hi
+// (rows are stored back to front, so row indices fall as the position
+// rises).  A walk's row variable r is the row holding the byte it is about
+// to read; `rend` is that row's end, and at rend the walk steps to the next
+// lower row (several, over empty rows).
+
+// Span of thread t over [lo, hi), cut on a 16-byte grid starting at g0.
+__device__ __forceinline__ uint32_t span_bytes(uint32_t g0, uint32_t hi)
+{
+  return ((hi - g0 + 16u * FSST_THREADS - 1u) / (16u * FSST_THREADS)) * 16u;
+}
+
+// Byte j (0..15) of a 16-byte group.
+__device__ __forceinline__ uint32_t group_byte(const u32x4& q, uint32_t j)
+{
+  return (q[j >> 2] >> (8u * (j & 3u))) & 0xFFu;
+}
+
+// First walk: decoded lengths.  Records the walk state at each grid point
+// and, per row, the decoded prefix at its start; returns the chunk's decoded
+// total and leaves dec[] as the chunk-relative exclusive decoded offsets.
+template <class SM>
+__device__ __forceinline__ uint32_t fsst_lengths_staged(SM& sm, uint32_t rows, uint32_t c_h, uint32_t c_hi,
+                                                        uint32_t C)
+{
+  const uint32_t tid  = threadIdx.x;
+  const uint32_t y_hi = c_h + C;
+  const uint32_t top  = c_h + c_hi;
+  const uint32_t span = span_bytes(0, y_hi);
+  const uint32_t ya   = tid * span > c_h ? tid * span : c_h;
+  const uint32_t yb   = tid * span + span < y_hi ? tid * span + span : y_hi;
+  uint32_t acc        = 0;
+  if (ya < yb) {
+    // the row holding byte ya: the largest r <= rows with comp[r] <= top - ya
+    // (r == rows, a row above the first, when ya is the first byte)
+    const uint32_t u0 = top - ya;
+    uint32_t r        = 0;
+#pragma unroll
+    for (uint32_t s = 1024; s; s >>= 1)
+      if (r + s <= rows && sm.comp[r + s] <= u0) r += s;
+    uint32_t rend  = top - sm.comp[r];
+    uint32_t rnext = r ? top - sm.comp[r - 1] : rend;  // end of row r - 1
+    bool esc       = false;
+    if (r < rows) {  // ya inside row r: an odd run of 255s before it escapes it
+      const uint32_t rs = top - sm.comp[r + 1];
+      for (uint32_t y = ya; y > rs && sm.cst[y - 1] == FSST_ESC; --y) esc = !esc;
+    }
+#pragma unroll 1
+    for (uint32_t g = ya & ~15u; g < yb; g += 16) {
+      const u32x4 q = *reinterpret_cast<const u32x4*>(&sm.cst[g]);
+      uint32_t L[16];
+#pragma unroll
+      for (uint32_t j = 0; j < 16; ++j) L[j] = sm.len[group_byte(q, j)];
+      const uint32_t lo = ya > g ? ya - g : 0u;  // the group's bytes of this span: [lo, hi)
+      const uint32_t hi = yb - g < 16u ? yb - g : 16u;
+#pragma unroll
+      for (uint32_t j = 0; j < 16; ++j) {
+        const uint32_t y = g + j;
+        if (y == rend && r) {  // rows starting at y
+          do {
+            --r;
+            sm.dec[r] = acc;
+            rend      = rnext;
+            rnext     = r ? top - sm.comp[r - 1] : rend;
+          } while (y == rend && r);
+          esc = false;
+        }
+        if (j == 0 && lo == 0) sm.gst[g >> 4] = acc | (esc ? GS_ESC : 0u) | (r << GS_ROW);
+        const uint32_t b  = group_byte(q, j);
+        const bool valid  = j >= lo && j < hi;
+        acc += valid ? (esc ? 1u : L[j]) : 0u;
+        esc = valid ? (!esc && b == FSST_ESC) : esc;
+      }
+    }
+    // rows starting at the last byte's end (empty rows below the others)
+    if (yb == y_hi)
+      while (rend == y_hi && r) {
+        --r;
+        sm.dec[r] = acc;
+        rend      = rnext;
+        rnext     = r ? top - sm.comp[r - 1] : rend;
+      }
+  }
+  uint32_t T         = 0;
+  const uint32_t pre = block_excl_scan(acc, sm.wave, &T);
+  sm.part[tid]       = pre;
+  if (ya < yb)
+    for (uint32_t g = (ya + 15u) & ~15u; g < yb; g += 16) sm.gst[g >> 4] += pre;
+  __syncthreads();
+  // dec[r + 1] = T - (decoded prefix at row r's start); the prefix was
+  // recorded by the thread whose span holds that start
+  uint32_t dv[ROWS_PER_THR];
+#pragma unroll
+  for (uint32_t k = 0; k < ROWS_PER_THR; ++k) {
+    const uint32_t r = k * FSST_THREADS + tid;
+    if (r < rows) {
+      const uint32_t p = top - sm.comp[r + 1];
+      dv[k]            = T - (sm.dec[r] + sm.part[(p < y_hi ? p : y_hi - 1u) / span]);
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (uint32_t k = 0; k < ROWS_PER_THR; ++k) {
+    const uint32_t r = k * FSST_THREADS + tid;
+    if (r < rows) sm.dec[r + 1] = dv[k];
+  }
+  if (tid == 0) sm.dec[0] = 0;
+  __syncthreads();
+  return T;
+}
+
+// Second walk: chars.  Rows are emitted in windows of whole rows whose
+// output fits OUT_CAP bytes; each window's output is built in LDS (zeroed,
+// then every symbol ORed in at its byte position, so neighbouring spans
+// may share words) and leaves in 16-byte stores.  Returns false, having
+// written nothing more, when a single row's output exceeds OUT_CAP.
+template <uint32_t OUT_CAP, class SM>
+__device__ __forceinline__ bool fsst_emit_staged(SM& sm, uint32_t rows, uint32_t c_h, uint32_t c_hi, uint32_t T,
+                                                 uint64_t base, uint8_t* chars, uint64_t chars_cap)
+{
+  const uint32_t tid = threadIdx.x;
+  const uint32_t top = c_h + c_hi;
+  const uintptr_t cb = reinterpret_cast<uintptr_t>(chars);
+  uint32_t r_hi      = rows - 1;  // highest row of the next window
+  for (;;) {
+    if (tid == 0) {
+      // lowest r_lo with dec[r_hi + 1] - dec[r_lo] <= OUT_CAP
+      const uint32_t d_top = sm.dec[r_hi + 1];
+      uint32_t r_lo;
+      if (d_top <= OUT_CAP) {
+        r_lo = 0;
+      } else if (d_top - sm.dec[r_hi] > OUT_CAP) {
+        r_lo = ~0u;
+      } else {
+        uint32_t a = 0, b = r_hi;  // fails at a, holds at b
+        while (b - a > 1u) {
+          const uint32_t m = (a + b) >> 1;
+          if (d_top - sm.dec[m] <= OUT_CAP) b = m;
+          else a = m;
+        }
+        r_lo = b;
+      }
+      sm.win = r_lo;
+    }
+    __syncthreads();
+    const uint32_t r_lo = sm.win;
+    if (r_lo == ~0u) return false;
+    const uint32_t wlo    = sm.dec[r_lo];
+    const uint32_t whi    = sm.dec[r_hi + 1];
+    const uintptr_t g_lo  = cb + base + wlo;
+    const uintptr_t g_hi  = cb + (base + whi < chars_cap ? base + whi : chars_cap);
+    if (g_lo < g_hi) {
+      const uint32_t pad  = static_cast<uint32_t>(g_lo & 15u);
+      const uintptr_t pa  = g_lo - pad;
+      const uint32_t nout = pad + (whi - wlo);
+      for (uint32_t i = tid * 16u; i < nout + 16u; i += FSST_THREADS * 16u)
+        *reinterpret_cast<u32x4*>(&sm.out[i]) = u32x4{0u, 0u, 0u, 0u};
+      __syncthreads();
+      const uint32_t ya   = top - sm.comp[r_hi + 1];
+      const uint32_t yb   = top - sm.comp[r_lo];
+      const uint32_t g0   = ya & ~15u;
+      const uint32_t span = span_bytes(g0, yb);
+      const uint32_t s0   = g0 + tid * span;
+      const uint32_t ys   = tid ? s0 : ya;
+      const uint32_t ye   = s0 + span < yb ? s0 + span : yb;
+      if (ys < ye) {
+        // thread 0 starts above the window's highest row and steps into it
+        // at ya; the others resume a grid-point state of the first walk
+        uint32_t r = r_hi + 1, o = 0;
+        bool esc   = false;
+        if (tid) {
+          const uint32_t gs = sm.gst[ys >> 4];
+          r                 = gs >> GS_ROW;
+          esc               = (gs & GS_ESC) != 0;
+          o = sm.dec[r] + (gs & GS_S) - (T - sm.dec[r + 1]) - wlo + pad;
+        }
+        uint32_t rend  = top - sm.comp[r];
+        uint32_t rnext = r ? top - sm.comp[r - 1] : rend;
+        uint32_t onext = r ? sm.dec[r - 1] : 0u;
+#pragma unroll 1
+        for (uint32_t g = ys & ~15u; g < ye; g += 16) {
+          const u32x4 q = *reinterpret_cast<const u32x4*>(&sm.cst[g]);
+          uint32_t L[16];
+          uint64_t V[16];
+#pragma unroll
+          for (uint32_t j = 0; j < 16; ++j) {
+            const uint32_t b = group_byte(q, j);
+            L[j]             = sm.len[b];
+            V[j]             = sm.sym[b];
+          }
+          const uint32_t lo = ys > g ? ys - g : 0u;
+          const uint32_t hi = ye - g < 16u ? ye - g : 16u;
+#pragma unroll
+          for (uint32_t j = 0; j < 16; ++j) {
+            const uint32_t y = g + j;
+            if (y == rend && y < ye) {  // rows starting at y (r > r_lo here)
+              do {
+                --r;
+                o     = onext - wlo + pad;
+                rend  = rnext;
+                rnext = r ? top - sm.comp[r - 1] : rend;
+                onext = r ? sm.dec[r - 1] : 0u;
+              } while (y == rend);
+              esc = false;
+            }
+            const uint32_t b  = group_byte(q, j);
+            const bool valid  = j >= lo && j < hi;
+            const bool lit    = esc;
+            const uint32_t l  = valid ? (lit ? 1u : L[j]) : 0u;
+            const uint64_t v  = valid ? (lit ? uint64_t{b} : V[j]) : 0ull;
+            esc               = valid ? (!lit && b == FSST_ESC) : esc;
+            const uint32_t sh = 8u * (o & 3u);
+            const uint64_t w  = v << sh;
+            const uint32_t w2 = sh ? static_cast<uint32_t>(v >> (64u - sh)) : 0u;
+            uint32_t* d       = reinterpret_cast<uint32_t*>(&sm.out[o & ~3u]);
+            __hip_atomic_fetch_or(d, static_cast<uint32_t>(w), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            __hip_atomic_fetch_or(d + 1, static_cast<uint32_t>(w >> 32), __ATOMIC_RELAXED,
+                                  __HIP_MEMORY_SCOPE_WORKGROUP);
+            __hip_atomic_fetch_or(d + 2, w2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            o += l;
+          }
+        }
+      }
+      __syncthreads();
+      for (uintptr_t blk = pa + uintptr_t{tid} * 16u; blk < g_hi; blk += uintptr_t{FSST_THREADS} * 16u) {
+        if (blk >= g_lo && blk + 16u <= g_hi) {
+          const u32x4 v = *reinterpret_cast<const u32x4*>(&sm.out[blk - pa]);
+          __builtin_nontemporal_store(v, reinterpret_cast<u32x4*>(blk));
+        } else {
+          for (uintptr_t x = blk; x < blk + 16u; ++x)
+            if (x >= g_lo && x < g_hi) *reinterpret_cast<uint8_t*>(x) = sm.out[x - pa];
+        }
+      }
+    }
+    __syncthreads();
+    if (r_lo == 0) return true;
+    r_hi = r_lo - 1;
+  }
+}
+
+// Decoupled look-back for the column-wide chars offset of chunk c, by wave
+// 0: publish this chunk's aggregate T, then sum predecessors 64 at a time
+// back to the nearest published inclusive prefix.  A status word carries
+// its flag and value together and is the only datum exchanged between
+// chunks, so device-scope relaxed atomics (coherent across the XCDs' L2s)
+// suffice; release/acquire orderings would add an L2 write-back /
+// invalidate per access.  Returns the exclusive prefix to every thread.
+template <class SM>
+__device__ __forceinline__ uint64_t fsst_lookback(uint32_t c, uint32_t n_chunks, uint32_t T, FsstState* st,
+                                                  uint32_t spin_limit, SM& sm)
+{
+  if (threadIdx.x < WAVE) {
+    const uint32_t lane = threadIdx.x;
+    uint64_t* status    = st->status;
+    uint64_t excl64     = 0;
+    if (c == 0) {
+      if (lane == 0) __hip_atomic_store(&status[0], ST_INC | T, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+      if (lane == 0) __hip_atomic_store(&status[c], ST_AGG | T, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      uint32_t p = c - 1, spins = 0;
+      for (;;) {
+        const bool valid = lane <= p;
+        const uint64_t s =
+          valid ? __hip_atomic_load(&status[p - lane], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0ull;
+        const uint64_t flag = s & ~ST_VAL;
+        const uint64_t incm = ballot(valid && flag == ST_INC);
+        const uint32_t stop = incm ? static_cast<uint32_t>(__builtin_ctzll(incm)) : (p < WAVE - 1 ? p : WAVE - 1);
+        const uint64_t need = stop == WAVE - 1 ? ~0ull : ((2ull << stop) - 1ull);
+        if (ballot(valid && flag == 0) & need) {  // a predecessor has not published yet
+          if (++spins > spin_limit) {              // never expected: give up rather than hang
+            if (lane == 0) atomicOr(&st->error, 1u);
+            break;
+          }
+          __builtin_amdgcn_s_sleep(1);
+          continue;
+        }
+        excl64 += readlane_u64(wave_incl_sum_u64(lane <= stop ? (s & ST_VAL) : 0ull), WAVE - 1);
+        if (incm || p < WAVE) break;  // chunk 0 always publishes an inclusive prefix
+        p -= WAVE;
+      }
+      if (lane == 0)
+        __hip_atomic_store(&status[c], ST_INC | ((excl64 + T) & ST_VAL), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    if (lane == 0) {
+      sm.base = excl64;
+      if (c == n_chunks - 1) st->total = excl64 + T;
+    }
+  }
+  __syncthreads();
+  return sm.base;
+}
+
+// Column offsets of the chunk's rows (and the column's closing offset).
+template <class SM>
+__device__ __forceinline__ void fsst_offsets(const FsstChunk& ch, uint32_t c, uint32_t n_chunks, uint32_t rows,
+                                             uint32_t T, uint64_t base, int32_t* offsets, uint32_t total_rows,
+                                             const SM& sm)
+{
+  for (uint32_t i = threadIdx.x; i < rows; i += FSST_THREADS)
+    offsets[ch.col_row + i] = static_cast<int32_t>(static_cast<uint32_t>(base + sm.dec[i]));
+  if (c == n_chunks - 1 && threadIdx.x == 0) offsets[total_rows] = static_cast<int32_t>(static_cast<uint32_t>(base + T));
 }
 
 // Decode chunk `c` (its position in the column's chunk order) with the
@@ -229,14 +632,14 @@ __device__ __forceinline__ void fsst_row(const uint8_t* p, uint32_t n, const Fss
 // Global traffic is staged through LDS with LDS-direct loads, so each stage
 // costs one memory round trip: the symbol table, the compressed-length
 // stream up to the chunk's last row (in 8 KiB pieces), and the chunk's
-// compressed bytes (when they fit in 16 KiB; rows outside the staged range
-// read global memory).  The column-wide chars offset comes from a
-// decoupled look-back done by one wavefront over 64 predecessors at a time.
+// compressed bytes.  The column-wide chars offset comes from a decoupled
+// look-back done by one wavefront over 64 predecessors at a time.
+template <uint32_t OUT_CAP>
 __device__ __forceinline__ void fsst_chunk(const FsstChunk ch, uint32_t c, uint32_t n_chunks,
                                            const FsstSeg* __restrict__ segs, FsstState* __restrict__ st,
                                            int32_t* __restrict__ offsets, uint8_t* __restrict__ chars,
                                            uint64_t chars_cap, uint32_t total_rows, uint32_t spin_limit,
-                                           FsstSmem& sm)
+                                           FsstSmemT<OUT_CAP>& sm)
 {
   const uint32_t tid = threadIdx.x;
   const bool gap     = ch.seg == FSST_GAP;
@@ -257,6 +660,7 @@ __device__ __forceinline__ void fsst_chunk(const FsstChunk ch, uint32_t c, uint3
   // ---- symbol table import, one code per thread (detail/fsst.cuh:50-96) ----
   // Bytes past the segment read as zero; no parse reads past byte 2057.
   __syncthreads();  // the previous user of sm is done
+  if (tid == 0) sm.flag = 0;
   uint32_t sym_n = 0, sym_h = 0;
   if (ok) {
     sym_n = sg.bytes_size - symoff;
@@ -307,8 +711,7 @@ __device__ __forceinline__ void fsst_chunk(const FsstChunk ch, uint32_t c, uint3
 
   // ---- compressed lengths (fsst.cu:132-188): the sum of the segment's
   // fields before the chunk, and this thread's own fields.  Thread t owns
-  // rows t, t + 256, t + 512 and t + 768 of the chunk, so every wave has
-  // rows in every stretch of the output the emit below walks. ----
+  // rows t, t + 256, t + 512 and t + 768 of the chunk. ----
   const uint32_t rows = ch.rows;
   auto row_of         = [&](uint32_t k) { return k * FSST_THREADS + tid; };
   uint32_t clen[ROWS_PER_THR];
@@ -353,194 +756,50 @@ __device__ __forceinline__ void fsst_chunk(const FsstChunk ch, uint32_t c, uint3
   block_excl_scan(part, sm.wave, &before);
   if (tid == 0) sm.comp[0] = before;
   {
-    uint32_t run = before;  // rows before the current 256-row stride
+    uint32_t run  = before;  // rows before the current 256-row stride
+    bool huge_row = false;   // a row this long cannot be staged; sums could wrap
 #pragma unroll
     for (uint32_t k = 0; k < ROWS_PER_THR; ++k) {
       uint32_t tot    = 0;
       const uint32_t ex = block_excl_scan(clen[k], sm.wave, &tot);
       if (row_of(k) < rows) sm.comp[row_of(k) + 1] = run + ex + clen[k];
+      huge_row |= clen[k] >= ROW_HUGE;
       run += tot;
     }
+    if (huge_row) sm.flag = 1;
   }
   __syncthreads();
 
   // ---- stage the chunk's compressed bytes.  Rows are stored back to front
   // ending at dict_end: row r is [dict_end - comp[r+1], dict_end - comp[r]).
+  // Every row shorter than 1 MiB keeps the prefix sums from wrapping, so
+  // comp[] rises from c_lo to c_hi. ----
   const uint32_t c_lo = sm.comp[0], c_hi = sm.comp[rows];
-  const bool staged   = ok && rows && c_hi >= c_lo && c_hi <= dict_end && c_hi - c_lo <= FSST_CSTAGE;
-  uint32_t c_h = 0;
-  if (staged) c_h = block_stage<FSST_THREADS>(reinterpret_cast<uint32_t*>(sm.cst), seg + dict_end - c_hi, c_hi - c_lo);
-  __syncthreads();
-
-  // ---- decoded lengths (fsst.cu:194-286, serial escape semantics) ----
-  uint32_t dlen[ROWS_PER_THR];
-#pragma unroll
-  for (uint32_t k = 0; k < ROWS_PER_THR; ++k) {
-    const uint32_t r = row_of(k);
-    uint32_t d = 0;
-    if (ok && r < rows) {
-      const uint32_t cum = sm.comp[r + 1], prev = sm.comp[r];
-      if (cum <= dict_end && prev <= cum && cum != prev) {
-        const uint32_t n = cum - prev;
-        auto add = [&](uint64_t, uint32_t l) { d += l; };
-        if (staged && prev >= c_lo && cum <= c_hi) fsst_row(sm.cst + c_h + (c_hi - cum), n, sm, add);
-        else fsst_row(seg + dict_end - cum, n, sm, add);
-      }
-    }
-    dlen[k] = d;
-  }
-  uint32_t T = 0;
-#pragma unroll
-  for (uint32_t k = 0; k < ROWS_PER_THR; ++k) {
-    uint32_t tot    = 0;
-    const uint32_t ex = block_excl_scan(dlen[k], sm.wave, &tot);
-    if (row_of(k) < rows) sm.dec[row_of(k)] = T + ex;
-    T += tot;
-  }
-  if (tid == 0) sm.dec[rows] = T;
-
-  // ---- decoupled look-back for the column-wide chars offset, by wave 0:
-  // publish this chunk's aggregate, then sum predecessors 64 at a time back
-  // to the nearest published inclusive prefix.  A status word carries its
-  // flag and value together and is the only datum exchanged between chunks,
-  // so device-scope relaxed atomics (coherent across the XCDs' L2s) suffice;
-  // release/acquire orderings would add an L2 write-back / invalidate per
-  // access. ----
-  if (tid < WAVE) {
-    const uint32_t lane = tid;
-    uint64_t* status    = st->status;
-    uint64_t excl64     = 0;
-    if (c == 0) {
-      if (lane == 0) __hip_atomic_store(&status[0], ST_INC | T, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  const bool staged   = ok && rows && sm.flag == 0 && c_hi >= c_lo && c_hi <= dict_end && c_hi - c_lo <= FSST_CSTAGE;
+  uint32_t T          = 0;
+  if (staged) {
+    const uint32_t C   = c_hi - c_lo;
+    const uint32_t c_h = block_stage<FSST_THREADS>(reinterpret_cast<uint32_t*>(sm.cst), seg + dict_end - c_hi, C);
+    __syncthreads();
+    if (C) {
+      T = fsst_lengths_staged(sm, rows, c_h, c_hi, C);
     } else {
-      if (lane == 0) __hip_atomic_store(&status[c], ST_AGG | T, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      uint32_t p = c - 1, spins = 0;
-      for (;;) {
-        const bool valid = lane <= p;
-        const uint64_t s =
-          valid ? __hip_atomic_load(&status[p - lane], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0ull;
-        const uint64_t flag = s & ~ST_VAL;
-        const uint64_t incm = ballot(valid && flag == ST_INC);
-        const uint32_t stop = incm ? static_cast<uint32_t>(__builtin_ctzll(incm)) : (p < WAVE - 1 ? p : WAVE - 1);
-        const uint64_t need = stop == WAVE - 1 ? ~0ull : ((2ull << stop) - 1ull);
-        if (ballot(valid && flag == 0) & need) {  // a predecessor has not published yet
-          if (++spins > spin_limit) {              // never expected: give up rather than hang
-            if (lane == 0) atomicOr(&st->error, 1u);
-            break;
-          }
-          __builtin_amdgcn_s_sleep(1);
-          continue;
-        }
-        excl64 += readlane_u64(wave_incl_sum_u64(lane <= stop ? (s & ST_VAL) : 0ull), WAVE - 1);
-        if (incm || p < WAVE) break;  // chunk 0 always publishes an inclusive prefix
-        p -= WAVE;
-      }
-      if (lane == 0)
-        __hip_atomic_store(&status[c], ST_INC | ((excl64 + T) & ST_VAL), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      for (uint32_t i = tid; i <= rows; i += FSST_THREADS) sm.dec[i] = 0;
+      __syncthreads();
     }
-    if (lane == 0) {
-      sm.base = excl64;
-      if (c == n_chunks - 1) st->total = excl64 + T;
-    }
-  }
-  __syncthreads();
-  const uint64_t base = sm.base;
 
-  // ---- offsets ----
-  for (uint32_t i = tid; i < rows; i += FSST_THREADS)
-    offsets[ch.col_row + i] = static_cast<int32_t>(static_cast<uint32_t>(base + sm.dec[i]));
-  if (c == n_chunks - 1 && tid == 0) offsets[total_rows] = static_cast<int32_t>(static_cast<uint32_t>(base + T));
-  if (T == 0 || base >= chars_cap) return;
-
-  // ---- chars: decode into LDS pieces aligned to the absolute chars
-  // address, flush whole 16-byte blocks with vector stores and the partial
-  // first/last blocks byte-wise (they share 16 bytes with the neighbours) ----
-  const uintptr_t cb = reinterpret_cast<uintptr_t>(chars);
-  const uintptr_t g0 = cb + base;
-  const uintptr_t g1 = cb + ((base + T < chars_cap) ? base + T : chars_cap);
-  for (uintptr_t pa = g0 & ~uintptr_t{15}; pa < g1; pa += FSST_PIECE) {
-    const uintptr_t pb = pa + FSST_PIECE;
-#pragma unroll
-    for (uint32_t k = 0; k < ROWS_PER_THR; ++k) {
-      const uint32_t r = row_of(k);
-      if (r >= rows || dlen[k] == 0) continue;
-      const uintptr_t o0 = g0 + sm.dec[r];
-      if (o0 + dlen[k] <= pa || o0 >= pb) continue;
-      const uint32_t cum = sm.comp[r + 1], prev = sm.comp[r];
-      // o: output position relative to the piece (may start before it)
-      int32_t o = static_cast<int32_t>(static_cast<intptr_t>(o0 - pa));
-      constexpr int32_t PIECE = static_cast<int32_t>(FSST_PIECE);
-      const bool inside = o >= 0 && o + static_cast<int32_t>(dlen[k]) <= PIECE;
-      auto walk = [&](auto&& put) {
-        auto lit = [&](uint32_t b) { return put(b, 1u); };
-        auto sym = [&](uint32_t b) { return put(sm.sym[b], sm.len[b]); };
-        if (staged && prev >= c_lo && cum <= c_hi) {
-          ByteReader lr(sm.cst + c_h + (c_hi - cum));
-          fsst_walk(cum - prev, [&](uint32_t) -> uint32_t { return lr.next(); }, lit, sym);
-        } else {
-          ByteReader br(seg + dict_end - cum);
-          fsst_walk(cum - prev, [&](uint32_t) -> uint32_t { return br.next(); }, lit, sym);
-        }
-      };
-      if (inside) {
-        // The row lies inside the piece: its first bytes up to a dword
-        // boundary and its last < 4 bytes are written singly (neighbouring
-        // rows share those dwords); everything between leaves a 16-byte
-        // register window as aligned dword stores.  Fixed-trip loops only.
-        uint64_t wa = 0, wb = 0;  // pending bytes, starting at dword-aligned oa
-        uint32_t na = 0;
-        int32_t oa  = o;
-        auto put = [&](uint64_t v, uint32_t l) {
-#pragma unroll
-          for (uint32_t h = 0; h < 3; ++h) {
-            if (na == 0 && (oa & 3) && l) {
-              sm.out[oa++] = static_cast<uint8_t>(v);
-              v >>= 8;
-              --l;
-            }
-          }
-          wa |= l ? v << (8u * na) : 0ull;
-          wb |= (l && na) ? v >> (64u - 8u * na) : 0ull;
-          na += l;
-#pragma unroll
-          for (uint32_t f = 0; f < 2; ++f) {
-            if (na >= 4u) {
-              *reinterpret_cast<uint32_t*>(&sm.out[oa]) = static_cast<uint32_t>(wa);
-              wa = (wa >> 32) | (wb << 32);
-              wb >>= 32;
-              oa += 4;
-              na -= 4u;
-            }
-          }
-        };
-        if (staged && prev >= c_lo && cum <= c_hi) fsst_row(sm.cst + c_h + (c_hi - cum), cum - prev, sm, put);
-        else fsst_row(seg + dict_end - cum, cum - prev, sm, put);
-#pragma unroll
-        for (uint32_t j = 0; j < 3; ++j)
-          if (j < na) sm.out[oa + static_cast<int32_t>(j)] = static_cast<uint8_t>(wa >> (8u * j));
-      } else {
-        // a row crossing a piece edge: bytes outside the piece are skipped
-        auto put = [&](uint64_t v, uint32_t l) {
-          for (uint32_t j = 0; j < l; ++j, ++o)
-            if (o >= 0 && o < PIECE) sm.out[o] = static_cast<uint8_t>(v >> (8u * j));
-          return o < PIECE;
-        };
-        walk(put);
-      }
-    }
-    __syncthreads();
-    const uintptr_t lo = pa > g0 ? pa : g0;
-    const uintptr_t hi = pb < g1 ? pb : g1;
-    for (uintptr_t blk = pa + uintptr_t{tid} * 16u; blk < hi; blk += uintptr_t{FSST_THREADS} * 16u) {
-      if (blk >= lo && blk + 16u <= hi) {
-        const u32x4 v = *reinterpret_cast<const u32x4*>(&sm.out[blk - pa]);
-        __builtin_nontemporal_store(v, reinterpret_cast<u32x4*>(blk));
-      } else {
-        for (uintptr_t x = blk; x < blk + 16u; ++x)
-          if (x >= lo && x < hi) *reinterpret_cast<uint8_t*>(x) = sm.out[x - pa];
-      }
-    }
-    __syncthreads();
+    // ---- look-back and offsets, then the second walk ----
+    const uint64_t base = fsst_lookback(c, n_chunks, T, st, spin_limit, sm);
+    fsst_offsets(ch, c, n_chunks, rows, T, base, offsets, total_rows, sm);
+    if (T == 0 || base >= chars_cap) return;
+    if (fsst_emit_staged<OUT_CAP>(sm, rows, c_h, c_hi, T, base, chars, chars_cap)) return;
+    fsst_emit_rows(sm, seg, dict_end, rows, T, base, chars, chars_cap);
+  } else {
+    T                   = fsst_lengths_rows(sm, ok, seg, dict_end, rows);
+    const uint64_t base = fsst_lookback(c, n_chunks, T, st, spin_limit, sm);
+    fsst_offsets(ch, c, n_chunks, rows, T, base, offsets, total_rows, sm);
+    if (T == 0 || base >= chars_cap) return;
+    fsst_emit_rows(sm, seg, dict_end, rows, T, base, chars, chars_cap);
   }
 }
 

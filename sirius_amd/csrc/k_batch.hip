@@ -21,7 +21,7 @@ constexpr uint32_t BATCH_SPIN_LIMIT = FSST_SPIN_LIMIT;
 union __attribute__((aligned(16))) BatchSmem {
   uint32_t fixed[WAVES_PER_BLOCK][ALP_STAGE_WORDS];
   RleSmem rle;
-  FsstSmem fsst;
+  FsstSmemT<FSST_OUT_BATCH> fsst;
 };
 
 template <typename T>
@@ -58,7 +58,7 @@ __global__ __launch_bounds__(BLOCK_THREADS) void k_batch(const BatchItem* __rest
     switch (it.kind) {
       case BK_FSST: {
         const FsstColumn col = *it.column;
-        fsst_chunk(col.chunks[it.chunk], it.chunk, col.n_chunks, col.segs, col.state, col.offsets, col.chars,
+        fsst_chunk<FSST_OUT_BATCH>(col.chunks[it.chunk], it.chunk, col.n_chunks, col.segs, col.state, col.offsets, col.chars,
                    col.chars_cap, col.total_rows, BATCH_SPIN_LIMIT, sm.fsst);
         break;
       }

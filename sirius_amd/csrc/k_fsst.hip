@@ -9,13 +9,13 @@ __global__ __launch_bounds__(FSST_THREADS) void k_fsst(const FsstChunk* __restri
                                                        int32_t* __restrict__ offsets, uint8_t* __restrict__ chars,
                                                        uint64_t chars_cap, uint32_t total_rows, uint32_t spin_limit)
 {
-  __shared__ FsstSmem sm;
+  __shared__ FsstSmemT<FSST_OUT_COLUMN> sm;
   __shared__ uint32_t s_ticket;
   if (threadIdx.x == 0) s_ticket = atomicAdd(&st->ticket, 1u);
   __syncthreads();
   const uint32_t c = s_ticket;
   if (c >= n_chunks) return;  // grid size equals chunk count; never taken
-  fsst_chunk(chunks[c], c, n_chunks, segs, st, offsets, chars, chars_cap, total_rows, spin_limit, sm);
+  fsst_chunk<FSST_OUT_COLUMN>(chunks[c], c, n_chunks, segs, st, offsets, chars, chars_cap, total_rows, spin_limit, sm);
 }
 
 __global__ void k_fsst_reset(FsstState* st, uint32_t n_chunks)

@@ -148,3 +148,54 @@ def test_config4_full_size():
     assert col.chars_used == len(chars)
     assert np.array_equal(g_offs.astype(np.uint64), offs)
     assert np.array_equal(g_chars[: col.chars_used], chars)
+
+
+def _shape(name, seed):
+    """String columns that steer chunks onto each decode path: short rows
+    (one staged walk), long incompressible rows (compressed bytes larger than
+    the stage), long repetitive rows (output larger than one window), one
+    row longer than the output window, mostly-empty and all-empty rows."""
+    rng = np.random.default_rng(seed)
+    if name == "long_random":
+        lens = rng.integers(60, 300, 6_000)
+    elif name == "repetitive":
+        lens = rng.integers(40, 120, 12_000)
+    elif name == "huge_row":
+        lens = rng.integers(0, 30, 5_000)
+        lens[1234] = 70_000
+    elif name == "mostly_empty":
+        lens = np.where(rng.random(40_000) < 0.7, 0, rng.integers(1, 40, 40_000))
+    elif name == "all_empty":
+        lens = np.zeros(9_000, np.int64)
+    else:
+        raise ValueError(name)
+    offs = np.zeros(len(lens) + 1, np.uint64)
+    offs[1:] = np.cumsum(lens)
+    total = int(offs[-1])
+    if name == "long_random":
+        chars = rng.integers(0, 256, total, dtype=np.uint8)
+    else:
+        words = [b"abcdefgh", b"ijklmnop", b"qrstuvwx", b"yz012345", b"\xff\xfe\x00\x01"]
+        pick = rng.integers(0, len(words), total // 4 + 2)
+        chars = np.frombuffer(b"".join(words[i] for i in pick), np.uint8)[:total].copy()
+    return chars, offs
+
+
+SHAPES = ["long_random", "repetitive", "huge_row", "mostly_empty", "all_empty"]
+
+
+@pytest.mark.parametrize("mode", [A.LAUNCH_BATCHED, A.LAUNCH_PER_SEGMENT, A.LAUNCH_PERSISTENT],
+                         ids=["batched", "plain", "persistent"])
+@pytest.mark.parametrize("shape", SHAPES)
+def test_string_shapes_match_oracle(shape, mode):
+    chars, offs = _shape(shape, 0x5A17)
+    enc = E.encode_fsst(chars, offs)
+    o_offs, o_chars = O.decode_strings(enc)
+    np.testing.assert_array_equal(o_chars, chars)
+    d, base = upload(enc.blob, 3)
+    rc, col, g_offs, g_chars = decode_strings_gpu([(A.FSST, enc.segments(base))], enc.n_rows, len(chars) + 64,
+                                                  mode, 5)
+    L.check(rc)
+    np.testing.assert_array_equal(g_offs, o_offs)
+    assert col.chars_used == len(o_chars)
+    np.testing.assert_array_equal(g_chars[: col.chars_used], o_chars)
