@@ -168,7 +168,7 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* s_wave
 constexpr uint32_t FSST_SYMSTAGE = 17u + 8u * FSST_NSYM;  // the largest symbol table (2057 bytes)
 constexpr uint32_t FSST_CSTAGE   = 16384;                  // chunk compressed bytes staged in LDS
 constexpr uint32_t FSST_NGRID    = (FSST_CSTAGE + 16u) / 16u + 2u;
-constexpr uint32_t FSST_OUT_COLUMN = 32768;  // output window of the per-column kernel
+constexpr uint32_t FSST_OUT_COLUMN = 8192;   // output window of the per-column kernel (4 blocks per CU)
 constexpr uint32_t FSST_OUT_BATCH  = 16384;  // ... of the persistent batch kernel
 
 // Walk state at a grid point: decoded prefix (bits 0-19), escape flag (bit
@@ -181,7 +181,8 @@ constexpr uint32_t ROW_HUGE = 1u << 20;  // a staged row is shorter than this
 
 template <uint32_t OUT_CAP>
 struct FsstSmemT {
-  uint64_t sym[256];  // symbol bytes, zero above its length
+  // the segment's FsstSymTab, staged whole
+  __attribute__((aligned(16))) uint64_t sym[256];  // symbol bytes, zero above its length
   uint8_t len[256];
   uint32_t comp[FSST_CHUNK_ROWS + 1];  // segment-relative inclusive compressed prefix
   uint32_t dec[FSST_CHUNK_ROWS + 1];   // chunk-relative exclusive decoded offset
@@ -193,7 +194,6 @@ struct FsstSmemT {
   __attribute__((aligned(16))) uint8_t cst[FSST_CSTAGE + 48];
   uint32_t wave[FSST_WAVES];
   uint32_t win;
-  uint32_t flag;
   uint64_t base;
 };
 
@@ -626,55 +626,76 @@ __device__ __forceinline__ void fsst_offsets(const FsstChunk& ch, uint32_t c, ui
   if (c == n_chunks - 1 && threadIdx.x == 0) offsets[total_rows] = static_cast<int32_t>(static_cast<uint32_t>(base + T));
 }
 
-// Decode chunk `c` (its position in the column's chunk order) with the
-// calling 256-thread workgroup.  Every thread must call it.
-//
-// Global traffic is staged through LDS with LDS-direct loads, so each stage
-// costs one memory round trip: the symbol table, the compressed-length
-// stream up to the chunk's last row (in 8 KiB pieces), and the chunk's
-// compressed bytes.  The column-wide chars offset comes from a decoupled
-// look-back done by one wavefront over 64 predecessors at a time.
-template <uint32_t OUT_CAP>
-__device__ __forceinline__ void fsst_chunk(const FsstChunk ch, uint32_t c, uint32_t n_chunks,
-                                           const FsstSeg* __restrict__ segs, FsstState* __restrict__ st,
-                                           int32_t* __restrict__ offsets, uint8_t* __restrict__ chars,
-                                           uint64_t chars_cap, uint32_t total_rows, uint32_t spin_limit,
-                                           FsstSmemT<OUT_CAP>& sm)
+// ---- segment pre-pass ----
+
+struct FsstPrepSmem {
+  __attribute__((aligned(16))) uint8_t st[FSST_SYMSTAGE + 48];
+  uint32_t sum[FSST_THREADS];
+  uint32_t huge[FSST_THREADS];
+  uint32_t wave[FSST_WAVES];
+};
+
+// Sum of the w-bit fields [f0, f1) of the length stream at `base`, and
+// whether one of them reaches ROW_HUGE.  Fields past `end` read as zero.
+__device__ __forceinline__ uint32_t fields_sum(const uint8_t* base, const uint8_t* end, uint32_t f0, uint32_t f1,
+                                               uint32_t w, bool* huge)
+{
+  uint32_t s = 0, mx = 0;
+  if (f0 < f1 && w) {
+    if (base + ((uint64_t{f1} * w + 7u) >> 3) <= end) {
+      BitCursor bc(base, uint64_t{f0} * w);
+      for (uint32_t f = f0; f < f1; ++f) {
+        const uint32_t v = bc.next(w);
+        s += v;
+        mx = v > mx ? v : mx;
+      }
+    } else {
+      for (uint32_t f = f0; f < f1; ++f) {
+        const uint32_t v = stream_bits32(base, end, uint64_t{f} * w, w);
+        s += v;
+        mx = v > mx ? v : mx;
+      }
+    }
+  }
+  *huge = mx >= ROW_HUGE;
+  return s;
+}
+
+// One segment with the calling 256-thread workgroup: header checks
+// (fsst.cu:79-87), the symbol table import, one code per thread
+// (detail/fsst.cuh:50-96), into `tab`, and for each chunk k of the segment
+// its compressed-length prefix bounds (fsst.cu:132-188) and flags into
+// info[first_chunk + k].  Bytes past the segment read as zero; no symbol
+// table parse reads past byte 2057.
+__device__ __forceinline__ void fsst_prep_segment(const FsstSeg sg, FsstChunkInfo* __restrict__ info,
+                                                  FsstSymTab* __restrict__ tab, FsstPrepSmem& sm)
 {
   const uint32_t tid = threadIdx.x;
-  const bool gap     = ch.seg == FSST_GAP;
-  const FsstSeg sg   = gap ? FsstSeg{nullptr, 0, 0} : segs[ch.seg];
   const uint8_t* seg = sg.bytes;
   const uint8_t* end = seg + sg.bytes_size;
-
-  // ---- header (fsst.cu:79-87) ----
-  bool ok = false;
+  bool ok            = false;
   uint32_t dict_end = 0, width = 0, symoff = 0;
-  if (!gap && sg.bytes_size >= 16u) {
+  if (sg.bytes_size >= 16u) {
     dict_end = su32(seg + 4);
     width    = su32(seg + 8);
     symoff   = su32(seg + 12);
     ok       = dict_end <= sg.bytes_size && symoff < dict_end && width <= 32u;
   }
-
-  // ---- symbol table import, one code per thread (detail/fsst.cuh:50-96) ----
-  // Bytes past the segment read as zero; no parse reads past byte 2057.
-  __syncthreads();  // the previous user of sm is done
-  if (tid == 0) sm.flag = 0;
+  __syncthreads();  // the previous segment's use of sm is done
   uint32_t sym_n = 0, sym_h = 0;
   if (ok) {
     sym_n = sg.bytes_size - symoff;
     if (sym_n > FSST_SYMSTAGE) sym_n = FSST_SYMSTAGE;
-    sym_h = block_stage<FSST_THREADS>(reinterpret_cast<uint32_t*>(sm.cst), seg + symoff, sym_n);
+    sym_h = block_stage<FSST_THREADS>(reinterpret_cast<uint32_t*>(sm.st), seg + symoff, sym_n);
   }
   __syncthreads();
+  uint32_t len = 0;
+  uint64_t sym = 0;
   if (ok) {
-    auto sb = [&](uint32_t i) -> uint32_t { return i < sym_n ? sm.cst[sym_h + i] : 0u; };
+    auto sb = [&](uint32_t i) -> uint32_t { return i < sym_n ? sm.st[sym_h + i] : 0u; };
     uint64_t version = 0;
     for (uint32_t i = 0; i < 8; ++i) version |= static_cast<uint64_t>(sb(i)) << (8u * i);
     const uint32_t code = tid;
-    uint32_t len = 0;
-    uint64_t sym = 0;
     if ((version >> 32) == FSST_VERSION && code < FSST_NSYM) {
       const uint32_t zt = sb(8) & 1u;
       uint32_t next_code = zt, pos = 17, found = 0;
@@ -689,11 +710,11 @@ __device__ __forceinline__ void fsst_chunk(const FsstChunk ch, uint32_t c, uint3
       for (uint32_t l = 1; l <= 8 && !found; ++l) {
         const uint32_t h  = l & 7u;
         const uint32_t sl = h + 1u;
-        uint32_t cnt = sb(9 + h);
+        uint32_t cnt      = sb(9 + h);
         if (h == 0 && zt) cnt = (cnt - 1u) & 0xFFu;
         if (code >= next_code && code < next_code + cnt) {
           const uint32_t at = pos + (code - next_code) * sl;
-          sym = 0;
+          sym               = 0;
           for (uint32_t j = 0; j < sl; ++j) sym |= static_cast<uint64_t>(sb(at + j)) << (8u * j);
           len   = sl;
           found = 1;
@@ -702,100 +723,156 @@ __device__ __forceinline__ void fsst_chunk(const FsstChunk ch, uint32_t c, uint3
         pos += cnt * sl;
       }
     }
-    sm.len[tid] = static_cast<uint8_t>(len);
-    sm.sym[tid] = sym;
-  } else {
-    sm.len[tid] = 0;
-    sm.sym[tid] = 0;
   }
+  tab->sym[tid] = sym;
+  tab->len[tid] = static_cast<uint8_t>(len);
 
-  // ---- compressed lengths (fsst.cu:132-188): the sum of the segment's
-  // fields before the chunk, and this thread's own fields.  Thread t owns
-  // rows t, t + 256, t + 512 and t + 768 of the chunk. ----
-  const uint32_t rows = ch.rows;
-  auto row_of         = [&](uint32_t k) { return k * FSST_THREADS + tid; };
-  uint32_t clen[ROWS_PER_THR];
-#pragma unroll
-  for (uint32_t k = 0; k < ROWS_PER_THR; ++k) clen[k] = 0;
-  uint32_t part = 0;
-  const uint32_t nf = ch.row_begin + rows;
-  const bool lfast  = ok && width && seg + 16 + ((uint64_t{nf} * width + 7u) >> 3) <= end;
-  if (lfast) {
-    const uint32_t per = ((FSST_PIECE - 16u) * 8u) / width;  // fields per staged piece
-    for (uint32_t f0 = 0; f0 < nf; f0 += per) {
-      const uint32_t f1 = nf - f0 < per ? nf : f0 + per;
-      const uint64_t b0 = (uint64_t{f0} * width) >> 3;
-      const uint64_t b1 = (uint64_t{f1} * width + 7u) >> 3;
-      __syncthreads();  // the previous piece is consumed
-      const uint32_t h = block_stage<FSST_THREADS>(reinterpret_cast<uint32_t*>(sm.out), seg + 16 + b0,
-                                                   static_cast<uint32_t>(b1 - b0));
-      __syncthreads();
-      const uint32_t bit0 = h * 8u + static_cast<uint32_t>((uint64_t{f0} * width) & 7u);
-      const uint32_t* lw  = reinterpret_cast<const uint32_t*>(sm.out);
-      const uint32_t pe   = f1 < ch.row_begin ? f1 : ch.row_begin;
-      for (uint32_t f = f0 + tid; f < pe; f += FSST_THREADS) part += lds_bits32(lw, bit0 + (f - f0) * width, width);
-#pragma unroll
-      for (uint32_t k = 0; k < ROWS_PER_THR; ++k) {
-        const uint32_t f = ch.row_begin + row_of(k);
-        if (row_of(k) < rows && f >= f0 && f < f1) clen[k] = lds_bits32(lw, bit0 + (f - f0) * width, width);
+  // chunk bounds: wave w sums chunks w, w + 4, ..., 16 fields per lane
+  const uint32_t rows = sg.row_count;
+  const uint32_t nch  = (rows + FSST_CHUNK_ROWS - 1u) / FSST_CHUNK_ROWS;
+  const uint32_t wv   = tid / WAVE;
+  const uint32_t lane = lane_id();
+  uint32_t carry      = 0;
+  for (uint32_t k0 = 0; k0 < nch; k0 += FSST_THREADS) {
+    const uint32_t cnt = nch - k0 < FSST_THREADS ? nch - k0 : FSST_THREADS;
+    if (ok)
+      for (uint32_t k = k0 + wv; k < k0 + cnt; k += FSST_WAVES) {
+        const uint32_t r0 = k * FSST_CHUNK_ROWS;
+        const uint32_t r1 = rows - r0 < FSST_CHUNK_ROWS ? rows : r0 + FSST_CHUNK_ROWS;
+        const uint32_t f0 = r0 + 16u * lane < r1 ? r0 + 16u * lane : r1;
+        const uint32_t f1 = f0 + 16u < r1 ? f0 + 16u : r1;
+        bool hg           = false;
+        const uint32_t s  = fields_sum(seg + 16, end, f0, f1, width, &hg);
+        const uint32_t tt = __builtin_amdgcn_readlane(wave_incl_sum_u32(s), WAVE - 1);
+        const bool any_hg = ballot(hg) != 0;
+        if (lane == 0) {
+          sm.sum[k - k0]  = tt;
+          sm.huge[k - k0] = any_hg;
+        }
       }
+    __syncthreads();
+    const uint32_t v  = (ok && tid < cnt) ? sm.sum[tid] : 0u;
+    uint32_t T        = 0;
+    const uint32_t ex = block_excl_scan(v, sm.wave, &T);
+    if (tid < cnt) {
+      const uint32_t k  = k0 + tid;
+      const uint32_t r1 = rows - k * FSST_CHUNK_ROWS < FSST_CHUNK_ROWS ? rows : (k + 1u) * FSST_CHUNK_ROWS;
+      uint32_t flags    = 0;
+      if (ok) {
+        flags = width | FI_OK;
+        if (seg + 16 + ((uint64_t{r1} * width + 7u) >> 3) <= end) flags |= FI_LFAST;
+        if (sm.huge[tid]) flags |= FI_HUGE;
+      }
+      FsstChunkInfo ci;
+      ci.bytes      = seg;
+      ci.c_lo       = carry + ex;
+      ci.c_hi       = carry + ex + v;
+      ci.dict_end   = dict_end;
+      ci.flags      = flags;
+      ci.bytes_size = sg.bytes_size;
+      ci.pad        = 0;
+      info[sg.first_chunk + k] = ci;
     }
-  } else if (ok) {
-    // a length stream running past the segment (malformed): bounded reads
-    if (ch.row_begin) {
-      const uint32_t q  = (ch.row_begin + FSST_THREADS - 1) / FSST_THREADS;
-      const uint32_t i0 = tid * q < ch.row_begin ? tid * q : ch.row_begin;
-      const uint32_t i1 = i0 + q < ch.row_begin ? i0 + q : ch.row_begin;
-      part = sum_fields(seg + 16, end, i0, i1, width);
-    }
-#pragma unroll
-    for (uint32_t k = 0; k < ROWS_PER_THR; ++k)
-      if (row_of(k) < rows) clen[k] = stream_bits32(seg + 16, end, uint64_t{ch.row_begin + row_of(k)} * width, width);
+    carry += T;
+    __syncthreads();  // sum[] and huge[] are read before the next batch refills them
   }
-  uint32_t before = 0;
-  block_excl_scan(part, sm.wave, &before);
-  if (tid == 0) sm.comp[0] = before;
+}
+
+// ---- chunk decode ----
+
+// Decode chunk `c` (its position in the column's chunk order) with the
+// calling 256-thread workgroup.  Every thread must call it.  `ci` and `tab`
+// are the segment pre-pass's output for the chunk and its segment (unused
+// for a chunk of uncovered rows).
+//
+// One memory round trip stages, with LDS-direct loads, the symbol table,
+// the chunk's compressed-length fields and its compressed bytes; a second
+// is the look-back for the column-wide chars offset.
+template <uint32_t OUT_CAP>
+__device__ __forceinline__ void fsst_chunk(const FsstChunk ch, const FsstChunkInfo ci, const FsstSymTab* tab,
+                                           uint32_t c, uint32_t n_chunks, FsstState* __restrict__ st,
+                                           int32_t* __restrict__ offsets, uint8_t* __restrict__ chars,
+                                           uint64_t chars_cap, uint32_t total_rows, uint32_t spin_limit,
+                                           FsstSmemT<OUT_CAP>& sm)
+{
+  const uint32_t tid      = threadIdx.x;
+  const bool ok           = ch.seg != FSST_GAP && (ci.flags & FI_OK);
+  const uint8_t* seg      = ok ? ci.bytes : nullptr;
+  const uint8_t* end      = ok ? ci.bytes + ci.bytes_size : nullptr;
+  const uint32_t dict_end = ok ? ci.dict_end : 0u;
+  const uint32_t width    = ok ? (ci.flags & FI_WIDTH) : 0u;
+  const uint32_t rows     = ch.rows;
+  const uint32_t c_lo     = ok ? ci.c_lo : 0u;
+  const uint32_t c_hi     = ok ? ci.c_hi : 0u;
+  const bool lfast        = ok && width && (ci.flags & FI_LFAST);
+  // Rows shorter than 1 MiB keep the prefix sums from wrapping, so comp[]
+  // rises from c_lo to c_hi.
+  const bool staged = ok && rows && !(ci.flags & FI_HUGE) && c_hi >= c_lo && c_hi <= dict_end &&
+                      c_hi - c_lo <= FSST_CSTAGE;
+
+  __syncthreads();  // the previous user of sm is done
+  if (ok) block_issue<FSST_THREADS>(reinterpret_cast<uint32_t*>(sm.sym), reinterpret_cast<const uint8_t*>(tab),
+                                    sizeof(FsstSymTab));
+  uint32_t bit0 = 0;
+  if (lfast) {
+    const uint64_t b0 = (uint64_t{ch.row_begin} * width) >> 3;
+    const uint64_t b1 = (uint64_t{ch.row_begin + rows} * width + 7u) >> 3;
+    const uint32_t h  = block_issue<FSST_THREADS>(reinterpret_cast<uint32_t*>(sm.out), seg + 16 + b0,
+                                                 static_cast<uint32_t>(b1 - b0));
+    bit0 = h * 8u + static_cast<uint32_t>((uint64_t{ch.row_begin} * width) & 7u);
+  }
+  // Rows are stored back to front ending at dict_end: row r is
+  // [dict_end - comp[r+1], dict_end - comp[r]).
+  uint32_t c_h = 0;
+  if (staged)
+    c_h = block_issue<FSST_THREADS>(reinterpret_cast<uint32_t*>(sm.cst), seg + dict_end - c_hi, c_hi - c_lo);
+  __builtin_amdgcn_s_waitcnt(WAITCNT_VMEM_DONE);
+  __syncthreads();
+
+  // ---- compressed prefix comp[] (fsst.cu:132-188); thread t owns rows
+  // 4t .. 4t + 3 here ----
   {
-    uint32_t run  = before;  // rows before the current 256-row stride
-    bool huge_row = false;   // a row this long cannot be staged; sums could wrap
+    const uint32_t* lw = reinterpret_cast<const uint32_t*>(sm.out);
+    uint32_t cl[ROWS_PER_THR];
+    uint32_t sum = 0;
 #pragma unroll
-    for (uint32_t k = 0; k < ROWS_PER_THR; ++k) {
-      uint32_t tot    = 0;
-      const uint32_t ex = block_excl_scan(clen[k], sm.wave, &tot);
-      if (row_of(k) < rows) sm.comp[row_of(k) + 1] = run + ex + clen[k];
-      huge_row |= clen[k] >= ROW_HUGE;
-      run += tot;
+    for (uint32_t i = 0; i < ROWS_PER_THR; ++i) {
+      const uint32_t r = ROWS_PER_THR * tid + i;
+      uint32_t v       = 0;
+      if (width && r < rows)
+        v = lfast ? lds_bits32(lw, bit0 + r * width, width)
+                  : stream_bits32(seg + 16, end, uint64_t{ch.row_begin + r} * width, width);
+      cl[i] = v;
+      sum += v;
     }
-    if (huge_row) sm.flag = 1;
+    uint32_t tot     = 0;
+    uint32_t run     = c_lo + block_excl_scan(sum, sm.wave, &tot);
+    if (tid == 0) sm.comp[0] = c_lo;
+#pragma unroll
+    for (uint32_t i = 0; i < ROWS_PER_THR; ++i) {
+      const uint32_t r = ROWS_PER_THR * tid + i;
+      run += cl[i];
+      if (r < rows) sm.comp[r + 1] = run;
+    }
   }
   __syncthreads();
 
-  // ---- stage the chunk's compressed bytes.  Rows are stored back to front
-  // ending at dict_end: row r is [dict_end - comp[r+1], dict_end - comp[r]).
-  // Every row shorter than 1 MiB keeps the prefix sums from wrapping, so
-  // comp[] rises from c_lo to c_hi. ----
-  const uint32_t c_lo = sm.comp[0], c_hi = sm.comp[rows];
-  const bool staged   = ok && rows && sm.flag == 0 && c_hi >= c_lo && c_hi <= dict_end && c_hi - c_lo <= FSST_CSTAGE;
-  uint32_t T          = 0;
   if (staged) {
-    const uint32_t C   = c_hi - c_lo;
-    const uint32_t c_h = block_stage<FSST_THREADS>(reinterpret_cast<uint32_t*>(sm.cst), seg + dict_end - c_hi, C);
-    __syncthreads();
+    const uint32_t C = c_hi - c_lo;
+    uint32_t T       = 0;
     if (C) {
       T = fsst_lengths_staged(sm, rows, c_h, c_hi, C);
     } else {
       for (uint32_t i = tid; i <= rows; i += FSST_THREADS) sm.dec[i] = 0;
       __syncthreads();
     }
-
-    // ---- look-back and offsets, then the second walk ----
     const uint64_t base = fsst_lookback(c, n_chunks, T, st, spin_limit, sm);
     fsst_offsets(ch, c, n_chunks, rows, T, base, offsets, total_rows, sm);
     if (T == 0 || base >= chars_cap) return;
     if (fsst_emit_staged<OUT_CAP>(sm, rows, c_h, c_hi, T, base, chars, chars_cap)) return;
     fsst_emit_rows(sm, seg, dict_end, rows, T, base, chars, chars_cap);
   } else {
-    T                   = fsst_lengths_rows(sm, ok, seg, dict_end, rows);
+    const uint32_t T    = fsst_lengths_rows(sm, ok, seg, dict_end, rows);
     const uint64_t base = fsst_lookback(c, n_chunks, T, st, spin_limit, sm);
     fsst_offsets(ch, c, n_chunks, rows, T, base, offsets, total_rows, sm);
     if (T == 0 || base >= chars_cap) return;

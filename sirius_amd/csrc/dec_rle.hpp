@@ -107,10 +107,10 @@ __device__ __forceinline__ uint32_t rle_pass(const uint8_t* counts, uint32_t t0,
       const uint32_t c0 = w & 0xFFFFu, c1 = w >> 16;
       run += c0;
       hits |= (run == rows ? 1u : 0u) << (2u * k);
-      bads |= ((c0 == 0u) | (run > rows) ? 1u : 0u) << (2u * k);
+      bads |= (((c0 == 0u) | (run > rows)) ? 1u : 0u) << (2u * k);
       run += c1;
       hits |= (run == rows ? 1u : 0u) << (2u * k + 1u);
-      bads |= ((c1 == 0u) | (run > rows) ? 1u : 0u) << (2u * k + 1u);
+      bads |= (((c1 == 0u) | (run > rows)) ? 1u : 0u) << (2u * k + 1u);
     }
     const uint32_t live = nlive >= RLE_PER_THREAD ? ~0u : (1u << nlive) - 1u;
     hits &= live;

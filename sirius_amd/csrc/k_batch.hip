@@ -58,8 +58,10 @@ __global__ __launch_bounds__(BLOCK_THREADS) void k_batch(const BatchItem* __rest
     switch (it.kind) {
       case BK_FSST: {
         const FsstColumn col = *it.column;
-        fsst_chunk<FSST_OUT_BATCH>(col.chunks[it.chunk], it.chunk, col.n_chunks, col.segs, col.state, col.offsets, col.chars,
-                   col.chars_cap, col.total_rows, BATCH_SPIN_LIMIT, sm.fsst);
+        const FsstChunk ch   = col.chunks[it.chunk];
+        fsst_chunk<FSST_OUT_BATCH>(ch, col.info[it.chunk], col.tabs + (ch.seg == FSST_GAP ? 0u : ch.seg), it.chunk,
+                                   col.n_chunks, col.state, col.offsets, col.chars, col.chars_cap, col.total_rows,
+                                   BATCH_SPIN_LIMIT, sm.fsst);
         break;
       }
       case BK_RLE:
@@ -102,6 +104,19 @@ __global__ void k_batch_reset(BatchQueue* q)
     q->claimed = 0;
     q->error   = 0;
   }
+}
+
+__global__ __launch_bounds__(FSST_THREADS) void k_batch_fsst_prep(const BatchQueue* __restrict__ q)
+{
+  __shared__ FsstPrepSmem sm;
+  const uint32_t n = q->n_fsst_segs;
+  for (uint32_t i = blockIdx.x; i < n; i += gridDim.x) fsst_prep_segment(q->fsst_segs[i], q->fsst_info, q->fsst_tabs + i, sm);
+}
+
+hipError_t launch_batch_fsst_prep(BatchQueue* q, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_batch_fsst_prep, dim3(512), dim3(FSST_THREADS), 0, s, q);
+  return hipGetLastError();
 }
 
 hipError_t launch_batch_reset(BatchQueue* q, hipStream_t s)

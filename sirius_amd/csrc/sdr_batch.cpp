@@ -171,7 +171,7 @@ int batch_add_strings(sdr_batch* b, sdr_string_column* col)
   uint32_t cursor = 0;
   for (const sdr_segment* g : segs) {
     if (g->row_offset > cursor) add_range(FSST_GAP, cursor, g->row_offset - cursor);
-    bc.segs.push_back({g->d_bytes, g->bytes_size, g->row_count});
+    bc.segs.push_back({g->d_bytes, g->bytes_size, g->row_count, static_cast<uint32_t>(bc.chunks.size()), 0});
     add_range(static_cast<uint32_t>(bc.segs.size() - 1), g->row_offset, g->row_count);
     cursor = g->row_offset + g->row_count;
   }
@@ -215,6 +215,12 @@ int batch_upload(sdr_batch* b, hipStream_t s)
   off                         = al(off + state_words * sizeof(uint64_t));
   b->off_nulls                = off;
   off                         = al(off + std::max<size_t>(1, b->masks.size()) * sizeof(unsigned long long));
+  const size_t readback_end   = off;
+  // written on the device by the FSST segment pre-pass: neither uploaded nor read back
+  const size_t off_info       = off;
+  off                         = al(off + n_chunks * sizeof(FsstChunkInfo));
+  const size_t off_tabs       = off;
+  off                         = al(off + n_segs * sizeof(FsstSymTab));
   const size_t upload_bytes   = b->off_state;  // the state and null counters are cleared on the device
   b->used_bytes               = off;
   if (off > b->arena_bytes) {
@@ -231,7 +237,7 @@ int batch_upload(sdr_batch* b, hipStream_t s)
     HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&b->h_stage), upload_bytes + upload_bytes / 4, hipHostMallocDefault));
     b->stage_bytes = upload_bytes + upload_bytes / 4;
   }
-  b->readback_bytes = off - b->off_state;  // state words and null counters
+  b->readback_bytes = readback_end - b->off_state;  // state words and null counters
   if (b->readback_bytes > b->readback_cap) {
     if (b->h_readback) HIP_TRY(hipHostFree(b->h_readback));
     b->h_readback   = nullptr;
@@ -245,6 +251,10 @@ int batch_upload(sdr_batch* b, hipStream_t s)
   q.n_items       = static_cast<uint32_t>(b->items.size());
   q.n_fsst_status = static_cast<uint32_t>(state_words);
   q.fsst_status   = reinterpret_cast<uint64_t*>(d + b->off_state);
+  q.n_fsst_segs   = static_cast<uint32_t>(n_segs);
+  q.fsst_segs     = reinterpret_cast<const FsstSeg*>(d + b->off_segs);
+  q.fsst_info     = reinterpret_cast<FsstChunkInfo*>(d + off_info);
+  q.fsst_tabs     = reinterpret_cast<FsstSymTab*>(d + off_tabs);
   std::memcpy(h, &q, sizeof(q));
   // FSST columns, chunks, segments, state
   size_t ck = 0, sg = 0, st = 0;
@@ -254,6 +264,8 @@ int batch_upload(sdr_batch* b, hipStream_t s)
     FsstColumn fc{};
     fc.chunks     = reinterpret_cast<const FsstChunk*>(d + b->off_chunks) + ck;
     fc.segs       = reinterpret_cast<const FsstSeg*>(d + b->off_segs) + sg;
+    fc.info       = reinterpret_cast<const FsstChunkInfo*>(d + off_info) + ck;
+    fc.tabs       = reinterpret_cast<const FsstSymTab*>(d + off_tabs) + sg;
     fc.state      = reinterpret_cast<FsstState*>(reinterpret_cast<uint64_t*>(d + b->off_state) + st);
     fc.offsets    = c.offsets;
     fc.chars      = c.chars;
@@ -262,8 +274,11 @@ int batch_upload(sdr_batch* b, hipStream_t s)
     fc.n_chunks   = static_cast<uint32_t>(c.chunks.size());
     dcols[i]      = fc;
     std::memcpy(h + b->off_chunks + ck * sizeof(FsstChunk), c.chunks.data(), c.chunks.size() * sizeof(FsstChunk));
-    if (!c.segs.empty())
-      std::memcpy(h + b->off_segs + sg * sizeof(FsstSeg), c.segs.data(), c.segs.size() * sizeof(FsstSeg));
+    auto* hs = reinterpret_cast<FsstSeg*>(h + b->off_segs) + sg;
+    for (size_t k = 0; k < c.segs.size(); ++k) {  // first_chunk: an index into the batch's chunk array
+      hs[k] = c.segs[k];
+      hs[k].first_chunk += static_cast<uint32_t>(ck);
+    }
     ck += c.chunks.size();
     sg += c.segs.size();
     st += 2 + c.chunks.size();
@@ -310,6 +325,7 @@ int batch_launch(sdr_batch* b, hipStream_t s)
       HIP_TRY(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
       hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
       if (e == hipSuccess) e = launch_batch_reset(q, cs);
+      if (e == hipSuccess) e = launch_batch_fsst_prep(q, cs);
       if (e == hipSuccess) e = launch_batch(items, q, grid, cs);
       hipGraph_t g = nullptr;
       hipError_t e2 = hipStreamEndCapture(cs, &g);
@@ -326,6 +342,7 @@ int batch_launch(sdr_batch* b, hipStream_t s)
     HIP_TRY(hipGraphLaunch(b->gexec, s));
   } else {
     HIP_TRY(launch_batch_reset(q, s));
+    HIP_TRY(launch_batch_fsst_prep(q, s));
     HIP_TRY(launch_batch(items, q, grid, s));
   }
   // validity masks (UNCOMPRESSED overlays) after the data decode

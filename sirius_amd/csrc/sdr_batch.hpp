@@ -25,6 +25,8 @@ constexpr uint32_t COPY_UNIT_BYTES = 16384;
 struct FsstColumn {
   const FsstChunk* chunks;
   const FsstSeg* segs;
+  const FsstChunkInfo* info;  // per chunk, from the segment pre-pass
+  const FsstSymTab* tabs;     // per segment, from the segment pre-pass
   FsstState* state;
   int32_t* offsets;
   uint8_t* chars;
@@ -60,9 +62,17 @@ struct BatchQueue {
   uint32_t n_fsst_status;  // status words to clear in `fsst_status`
   uint32_t error;
   uint64_t* fsst_status;   // contiguous FsstState storage of all FSST columns
+  uint32_t n_fsst_segs;    // segments of all FSST columns, for the pre-pass
+  uint32_t pad;
+  const FsstSeg* fsst_segs;      // their first_chunk index fsst_info
+  FsstChunkInfo* fsst_info;
+  FsstSymTab* fsst_tabs;
 };
 
 hipError_t launch_batch_reset(BatchQueue* q, hipStream_t s);
+// FSST segment pre-pass over the queue's segments (a fixed grid that loops,
+// so one recorded launch serves every batch).
+hipError_t launch_batch_fsst_prep(BatchQueue* q, hipStream_t s);
 hipError_t launch_batch(const BatchItem* items, BatchQueue* q, uint32_t grid, hipStream_t s);
 // Blocks per CU the persistent kernel can keep resident.
 int batch_occupancy_blocks_per_cu();

@@ -239,10 +239,10 @@ __device__ __forceinline__ uint32_t stage_to_lds(uint32_t* lds, const uint8_t* p
 
 // Block-wide copy of [p, p + nbytes) into LDS as 16-byte blocks by NT
 // threads (LDS byte 0 holds align_down(p, 16)); returns the byte skew of p.
-// Each wave waits for its own loads; the caller's barrier makes the whole
-// stage visible to the block.
+// Issues the loads only: several stages can be in flight together, and the
+// caller waits (s_waitcnt vmcnt(0)) and then synchronises the block.
 template <uint32_t NT>
-__device__ __forceinline__ uint32_t block_stage(uint32_t* lds, const uint8_t* p, uint32_t nbytes)
+__device__ __forceinline__ uint32_t block_issue(uint32_t* lds, const uint8_t* p, uint32_t nbytes)
 {
   const uintptr_t a   = reinterpret_cast<uintptr_t>(p);
   const uintptr_t a16 = a & ~uintptr_t{15};
@@ -252,6 +252,15 @@ __device__ __forceinline__ uint32_t block_stage(uint32_t* lds, const uint8_t* p,
   const uint8_t* src  = reinterpret_cast<const uint8_t*>(a16) + 16u * threadIdx.x;
   for (uint32_t k = 0; k * NT < nblk; ++k)
     if (k * NT + threadIdx.x < nblk) lds_direct_16(src + 16u * NT * k, lds + 4u * (NT * k + wb));
+  return head;
+}
+
+// block_issue, then this wave's wait; the caller's barrier makes the whole
+// stage visible to the block.
+template <uint32_t NT>
+__device__ __forceinline__ uint32_t block_stage(uint32_t* lds, const uint8_t* p, uint32_t nbytes)
+{
+  const uint32_t head = block_issue<NT>(lds, p, nbytes);
   __builtin_amdgcn_s_waitcnt(WAITCNT_VMEM_DONE);
   return head;
 }

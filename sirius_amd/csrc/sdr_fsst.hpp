@@ -1,4 +1,4 @@
-// sdr_fsst.hpp — descriptors shared by the FSST kernel and its host driver.
+// sdr_fsst.hpp — descriptors shared by the FSST kernels and their host drivers.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -10,10 +10,14 @@ constexpr uint32_t FSST_CHUNK_ROWS = 1024;        // rows per workgroup
 constexpr uint32_t FSST_GAP        = 0xFFFFFFFFu;  // chunk of rows no segment covers
 constexpr uint32_t FSST_SPIN_LIMIT = 1u << 24;     // look-back wait bound (never reached)
 
+// A segment and the run of chunks that covers it (first_chunk indexes the
+// chunk list the segment list is laid out with).
 struct FsstSeg {
   const uint8_t* bytes;
   uint32_t bytes_size;
   uint32_t row_count;
+  uint32_t first_chunk;
+  uint32_t pad;
 };
 
 // A chunk: `rows` rows starting at segment row `row_begin` (column row
@@ -25,6 +29,33 @@ struct FsstChunk {
   uint32_t col_row;
 };
 
+// What the segment pre-pass (k_fsst_prep) leaves for each chunk: the
+// segment's header fields and the chunk's compressed-length prefix bounds,
+// so a chunk decode issues all of its loads at once.
+constexpr uint32_t FI_WIDTH = 0xFFu;      // bits 0-7: length-stream bit width
+constexpr uint32_t FI_OK    = 1u << 8;    // header valid, symbol table imported
+constexpr uint32_t FI_LFAST = 1u << 9;    // the chunk's length fields lie inside the segment
+constexpr uint32_t FI_HUGE  = 1u << 10;   // a row of the chunk has >= 1 MiB of compressed bytes
+
+struct FsstChunkInfo {
+  const uint8_t* bytes;  // the segment
+  uint32_t c_lo;         // compressed bytes of the segment's rows before the chunk
+  uint32_t c_hi;         // ... up to and including the chunk's last row
+  uint32_t dict_end;
+  uint32_t flags;        // width and FI_* bits
+  uint32_t bytes_size;
+  uint32_t pad;
+};
+static_assert(sizeof(FsstChunkInfo) == 32, "FsstChunkInfo layout");
+
+// A segment's symbol table as the decoder reads it: symbol bytes (zero
+// above the length) and lengths, 256 codes (255, the escape, has length 0).
+struct FsstSymTab {
+  uint64_t sym[256];
+  uint8_t len[256];
+};
+static_assert(sizeof(FsstSymTab) == 2304, "FsstSymTab layout");
+
 // Device state of one column decode: ticket counter, look-back status per
 // chunk, error flag and the decoded chars total.
 struct FsstState {
@@ -35,8 +66,13 @@ struct FsstState {
 };
 
 hipError_t launch_fsst_reset(FsstState* d_state, uint32_t n_chunks, hipStream_t s);
+// Segment pre-pass over n_segs segments (or *d_n_segs when given, for a
+// launch recorded before the count is known).
+hipError_t launch_fsst_prep(const FsstSeg* d_segs, uint32_t n_segs, const uint32_t* d_n_segs, uint32_t grid,
+                            FsstChunkInfo* d_info, FsstSymTab* d_tabs, hipStream_t s);
 hipError_t launch_fsst(const FsstChunk* d_chunks, uint32_t first, uint32_t n, uint32_t n_chunks_total,
-                       const FsstSeg* d_segs, FsstState* d_state, int32_t* d_offsets, uint8_t* d_chars,
-                       uint64_t chars_cap, uint32_t total_rows, hipStream_t s);
+                       const FsstChunkInfo* d_info, const FsstSymTab* d_tabs, FsstState* d_state,
+                       int32_t* d_offsets, uint8_t* d_chars, uint64_t chars_cap, uint32_t total_rows,
+                       hipStream_t s);
 
 }  // namespace sdr

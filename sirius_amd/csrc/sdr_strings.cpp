@@ -3,7 +3,8 @@
 // Mirrors gpu_decode_strings_column (src/cuda/scan/gpu_decode_strings.cu:
 // 100-402): per-run validation, one column-wide int32 offset space, chars
 // sized by the caller, validity overlay.  The reference's length pass, CUB
-// scan and gather pass are one kernel here (k_fsst.hip).
+// scan and gather pass are one kernel here (k_fsst.hip), after a segment
+// pre-pass that imports symbol tables and bounds every chunk.
 #include "sdr_fsst.hpp"
 #include "sdr_internal.hpp"
 
@@ -71,36 +72,45 @@ int decode_strings(sdr_string_column* col, hipStream_t s, uint32_t mode)
   uint32_t cursor = 0;
   for (const sdr_segment* g : segs) {
     if (g->row_offset > cursor) add_range(FSST_GAP, cursor, g->row_offset - cursor);
-    hsegs.push_back({g->d_bytes, g->bytes_size, g->row_count});
+    hsegs.push_back({g->d_bytes, g->bytes_size, g->row_count, static_cast<uint32_t>(chunks.size()), 0});
     add_range(static_cast<uint32_t>(hsegs.size() - 1), g->row_offset, g->row_count);
     cursor = g->row_offset + g->row_count;
   }
   if (cursor < n) add_range(FSST_GAP, cursor, n - cursor);
   const uint32_t n_chunks = static_cast<uint32_t>(chunks.size());
 
-  const size_t seg_bytes   = std::max<size_t>(1, hsegs.size()) * sizeof(FsstSeg);
-  const size_t chunk_bytes = chunks.size() * sizeof(FsstChunk);
-  const size_t state_bytes = sizeof(FsstState) + sizeof(uint64_t) * n_chunks;
-  const size_t total_bytes = seg_bytes + chunk_bytes + state_bytes + sizeof(unsigned long long);
+  auto al = [](size_t v) { return (v + 255u) & ~size_t{255}; };
+  const size_t seg_bytes   = al(std::max<size_t>(1, hsegs.size()) * sizeof(FsstSeg));
+  const size_t chunk_bytes = al(chunks.size() * sizeof(FsstChunk));
+  const size_t info_bytes  = al(chunks.size() * sizeof(FsstChunkInfo));
+  const size_t tab_bytes   = al(std::max<size_t>(1, hsegs.size()) * sizeof(FsstSymTab));
+  const size_t state_bytes = al(sizeof(FsstState) + sizeof(uint64_t) * n_chunks);
+  const size_t total_bytes = seg_bytes + chunk_bytes + info_bytes + tab_bytes + state_bytes + sizeof(unsigned long long);
   uint8_t* d = nullptr;
   rc = device_alloc(reinterpret_cast<void**>(&d), total_bytes, s);
   if (rc) return rc;
   auto* d_segs   = reinterpret_cast<FsstSeg*>(d);
   auto* d_chunks = reinterpret_cast<FsstChunk*>(d + seg_bytes);
-  auto* d_state  = reinterpret_cast<FsstState*>(d + seg_bytes + chunk_bytes);
-  auto* d_nulls  = reinterpret_cast<unsigned long long*>(d + seg_bytes + chunk_bytes + state_bytes);
+  auto* d_info   = reinterpret_cast<FsstChunkInfo*>(d + seg_bytes + chunk_bytes);
+  auto* d_tabs   = reinterpret_cast<FsstSymTab*>(d + seg_bytes + chunk_bytes + info_bytes);
+  uint8_t* d_rest = d + seg_bytes + chunk_bytes + info_bytes + tab_bytes;
+  auto* d_state  = reinterpret_cast<FsstState*>(d_rest);
+  auto* d_nulls  = reinterpret_cast<unsigned long long*>(d_rest + state_bytes);
 
   hipError_t e = hipSuccess;
   if (!hsegs.empty()) e = hipMemcpyAsync(d_segs, hsegs.data(), hsegs.size() * sizeof(FsstSeg), hipMemcpyHostToDevice, s);
   if (e == hipSuccess) e = hipMemcpyAsync(d_chunks, chunks.data(), chunk_bytes, hipMemcpyHostToDevice, s);
   if (e == hipSuccess) e = launch_fsst_reset(d_state, n_chunks, s);
+  if (e == hipSuccess && !hsegs.empty())
+    e = launch_fsst_prep(d_segs, static_cast<uint32_t>(hsegs.size()), nullptr,
+                         std::min<uint32_t>(static_cast<uint32_t>(hsegs.size()), 4096u), d_info, d_tabs, s);
   if (e == hipSuccess) {
     if (mode == SDR_LAUNCH_PER_SEGMENT) {
       for (size_t i = 0; e == hipSuccess && i < groups.size(); ++i)
-        e = launch_fsst(d_chunks, groups[i].first, groups[i].second, n_chunks, d_segs, d_state, col->d_offsets,
-                        col->d_chars, col->d_chars ? col->chars_capacity : 0, n, s);
+        e = launch_fsst(d_chunks, groups[i].first, groups[i].second, n_chunks, d_info, d_tabs, d_state,
+                        col->d_offsets, col->d_chars, col->d_chars ? col->chars_capacity : 0, n, s);
     } else {
-      e = launch_fsst(d_chunks, 0, n_chunks, n_chunks, d_segs, d_state, col->d_offsets, col->d_chars,
+      e = launch_fsst(d_chunks, 0, n_chunks, n_chunks, d_info, d_tabs, d_state, col->d_offsets, col->d_chars,
                       col->d_chars ? col->chars_capacity : 0, n, s);
     }
   }
