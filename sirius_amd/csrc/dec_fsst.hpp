@@ -661,19 +661,28 @@ __device__ __forceinline__ uint32_t fields_sum(const uint8_t* base, const uint8_
   return s;
 }
 
-// One segment with the calling 256-thread workgroup: header checks
+// Range `si` with the calling 256-thread workgroup: its chunk descriptors
+// into chunks[first_chunk + k]; for a segment, header checks
 // (fsst.cu:79-87), the symbol table import, one code per thread
-// (detail/fsst.cuh:50-96), into `tab`, and for each chunk k of the segment
-// its compressed-length prefix bounds (fsst.cu:132-188) and flags into
+// (detail/fsst.cuh:50-96), into `tab`, and for each chunk k its
+// compressed-length prefix bounds (fsst.cu:132-188) and flags into
 // info[first_chunk + k].  Bytes past the segment read as zero; no symbol
 // table parse reads past byte 2057.
-__device__ __forceinline__ void fsst_prep_segment(const FsstSeg sg, FsstChunkInfo* __restrict__ info,
-                                                  FsstSymTab* __restrict__ tab, FsstPrepSmem& sm)
+__device__ __forceinline__ void fsst_prep_segment(const FsstSeg sg, uint32_t si, FsstChunk* __restrict__ chunks,
+                                                  FsstChunkInfo* __restrict__ info, FsstSymTab* __restrict__ tab,
+                                                  FsstPrepSmem& sm)
 {
   const uint32_t tid = threadIdx.x;
   const uint8_t* seg = sg.bytes;
   const uint8_t* end = seg + sg.bytes_size;
-  bool ok            = false;
+  const bool gap     = seg == nullptr;
+  for (uint32_t k = tid; k * FSST_CHUNK_ROWS < sg.row_count; k += FSST_THREADS) {
+    const uint32_t r0 = k * FSST_CHUNK_ROWS;
+    const uint32_t n  = sg.row_count - r0 < FSST_CHUNK_ROWS ? sg.row_count - r0 : FSST_CHUNK_ROWS;
+    chunks[sg.first_chunk + k] = FsstChunk{gap ? FSST_GAP : si, gap ? 0u : r0, n, sg.col_row + r0};
+  }
+  if (gap) return;  // uniform: nothing else of a gap is read
+  bool ok = false;
   uint32_t dict_end = 0, width = 0, symoff = 0;
   if (sg.bytes_size >= 16u) {
     dict_end = su32(seg + 4);

@@ -110,7 +110,8 @@ __global__ __launch_bounds__(FSST_THREADS) void k_batch_fsst_prep(const BatchQue
 {
   __shared__ FsstPrepSmem sm;
   const uint32_t n = q->n_fsst_segs;
-  for (uint32_t i = blockIdx.x; i < n; i += gridDim.x) fsst_prep_segment(q->fsst_segs[i], q->fsst_info, q->fsst_tabs + i, sm);
+  for (uint32_t i = blockIdx.x; i < n; i += gridDim.x)
+    fsst_prep_segment(q->fsst_segs[i], i, q->fsst_chunks, q->fsst_info, q->fsst_tabs + i, sm);
 }
 
 hipError_t launch_batch_fsst_prep(BatchQueue* q, hipStream_t s)

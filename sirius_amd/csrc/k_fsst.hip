@@ -24,12 +24,13 @@ __global__ __launch_bounds__(FSST_THREADS) void k_fsst(const FsstChunk* __restri
 // Segment pre-pass: one workgroup per segment (grid-stride).
 __global__ __launch_bounds__(FSST_THREADS) void k_fsst_prep(const FsstSeg* __restrict__ segs, uint32_t n_segs,
                                                             const uint32_t* __restrict__ d_n_segs,
+                                                            FsstChunk* __restrict__ chunks,
                                                             FsstChunkInfo* __restrict__ info,
                                                             FsstSymTab* __restrict__ tabs)
 {
   __shared__ FsstPrepSmem sm;
   const uint32_t n = d_n_segs ? *d_n_segs : n_segs;
-  for (uint32_t i = blockIdx.x; i < n; i += gridDim.x) fsst_prep_segment(segs[i], info, tabs + i, sm);
+  for (uint32_t i = blockIdx.x; i < n; i += gridDim.x) fsst_prep_segment(segs[i], i, chunks, info, tabs + i, sm);
 }
 
 __global__ void k_fsst_reset(FsstState* st, uint32_t n_chunks)
@@ -53,10 +54,11 @@ hipError_t launch_fsst_reset(FsstState* d_state, uint32_t n_chunks, hipStream_t 
 }
 
 hipError_t launch_fsst_prep(const FsstSeg* d_segs, uint32_t n_segs, const uint32_t* d_n_segs, uint32_t grid,
-                            FsstChunkInfo* d_info, FsstSymTab* d_tabs, hipStream_t s)
+                            FsstChunk* d_chunks, FsstChunkInfo* d_info, FsstSymTab* d_tabs, hipStream_t s)
 {
   if (grid == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_fsst_prep, dim3(grid), dim3(FSST_THREADS), 0, s, d_segs, n_segs, d_n_segs, d_info, d_tabs);
+  hipLaunchKernelGGL(k_fsst_prep, dim3(grid), dim3(FSST_THREADS), 0, s, d_segs, n_segs, d_n_segs, d_chunks, d_info,
+                     d_tabs);
   return hipGetLastError();
 }
 

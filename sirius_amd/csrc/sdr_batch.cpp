@@ -15,8 +15,8 @@ struct sdr_batch {
   // host-side description
   std::vector<sdr::BatchItem> items;
   struct Col {
-    std::vector<sdr::FsstChunk> chunks;
-    std::vector<sdr::FsstSeg> segs;
+    uint32_t n_chunks = 0;
+    std::vector<sdr::FsstSeg> segs;  // row ranges: segments and the gaps between them
     sdr_string_column* user;  // chars_used / null_count written back at finish
     int32_t* offsets;
     uint8_t* chars;
@@ -164,21 +164,20 @@ int batch_add_strings(sdr_batch* b, sdr_string_column* col)
   for (size_t i = 1; i < segs.size(); ++i)
     if (segs[i]->row_offset < segs[i - 1]->row_offset + segs[i - 1]->row_count)
       return fail(SDR_E_BOUNDS, "string segments overlap at row %u", segs[i]->row_offset);
-  auto add_range = [&](uint32_t seg, uint32_t col_row, uint32_t rows) {
-    for (uint32_t o = 0; o < rows; o += FSST_CHUNK_ROWS)
-      bc.chunks.push_back({seg, seg == FSST_GAP ? 0u : o, std::min(FSST_CHUNK_ROWS, rows - o), col_row + o});
+  auto add_range = [&](const uint8_t* bytes, uint32_t size, uint32_t col_row, uint32_t rows) {
+    bc.segs.push_back({bytes, size, rows, bc.n_chunks, col_row});
+    bc.n_chunks += fsst_chunk_count(rows);
   };
   uint32_t cursor = 0;
   for (const sdr_segment* g : segs) {
-    if (g->row_offset > cursor) add_range(FSST_GAP, cursor, g->row_offset - cursor);
-    bc.segs.push_back({g->d_bytes, g->bytes_size, g->row_count, static_cast<uint32_t>(bc.chunks.size()), 0});
-    add_range(static_cast<uint32_t>(bc.segs.size() - 1), g->row_offset, g->row_count);
+    if (g->row_offset > cursor) add_range(nullptr, 0, cursor, g->row_offset - cursor);
+    add_range(g->d_bytes, g->bytes_size, g->row_offset, g->row_count);
     cursor = g->row_offset + g->row_count;
   }
-  if (cursor < n) add_range(FSST_GAP, cursor, n - cursor);
+  if (cursor < n) add_range(nullptr, 0, cursor, n - cursor);
   bc.item_first = static_cast<uint32_t>(b->items.size());
   const uint32_t ci = static_cast<uint32_t>(b->cols.size());
-  for (uint32_t k = 0; k < bc.chunks.size(); ++k) {
+  for (uint32_t k = 0; k < bc.n_chunks; ++k) {
     BatchItem it{};
     it.kind  = BK_FSST;
     it.chunk = k;
@@ -196,19 +195,17 @@ int batch_upload(sdr_batch* b, hipStream_t s)
   auto al = [](size_t v) { return (v + 255u) & ~size_t{255}; };
   size_t n_chunks = 0, n_segs = 0;
   for (auto& c : b->cols) {
-    n_chunks += c.chunks.size();
+    n_chunks += c.n_chunks;
     n_segs += c.segs.size();
   }
   // FsstState = {ticket, error, total, status[n]}: 16 + 8 n bytes per column
   size_t state_words = 0;
-  for (auto& c : b->cols) state_words += 2 + c.chunks.size();
+  for (auto& c : b->cols) state_words += 2 + c.n_chunks;
   size_t off                  = al(sizeof(BatchQueue));
   b->off_items                = off;
   off                         = al(off + b->items.size() * sizeof(BatchItem));
   b->off_cols                 = off;
   off                         = al(off + b->cols.size() * sizeof(FsstColumn));
-  b->off_chunks               = off;
-  off                         = al(off + n_chunks * sizeof(FsstChunk));
   b->off_segs                 = off;
   off                         = al(off + n_segs * sizeof(FsstSeg));
   b->off_state                = off;
@@ -217,6 +214,8 @@ int batch_upload(sdr_batch* b, hipStream_t s)
   off                         = al(off + std::max<size_t>(1, b->masks.size()) * sizeof(unsigned long long));
   const size_t readback_end   = off;
   // written on the device by the FSST segment pre-pass: neither uploaded nor read back
+  b->off_chunks               = off;
+  off                         = al(off + n_chunks * sizeof(FsstChunk));
   const size_t off_info       = off;
   off                         = al(off + n_chunks * sizeof(FsstChunkInfo));
   const size_t off_tabs       = off;
@@ -253,6 +252,7 @@ int batch_upload(sdr_batch* b, hipStream_t s)
   q.fsst_status   = reinterpret_cast<uint64_t*>(d + b->off_state);
   q.n_fsst_segs   = static_cast<uint32_t>(n_segs);
   q.fsst_segs     = reinterpret_cast<const FsstSeg*>(d + b->off_segs);
+  q.fsst_chunks   = reinterpret_cast<FsstChunk*>(d + b->off_chunks);
   q.fsst_info     = reinterpret_cast<FsstChunkInfo*>(d + off_info);
   q.fsst_tabs     = reinterpret_cast<FsstSymTab*>(d + off_tabs);
   std::memcpy(h, &q, sizeof(q));
@@ -265,23 +265,22 @@ int batch_upload(sdr_batch* b, hipStream_t s)
     fc.chunks     = reinterpret_cast<const FsstChunk*>(d + b->off_chunks) + ck;
     fc.segs       = reinterpret_cast<const FsstSeg*>(d + b->off_segs) + sg;
     fc.info       = reinterpret_cast<const FsstChunkInfo*>(d + off_info) + ck;
-    fc.tabs       = reinterpret_cast<const FsstSymTab*>(d + off_tabs) + sg;
+    fc.tabs       = reinterpret_cast<const FsstSymTab*>(d + off_tabs);  // chunk seg indices are batch-wide
     fc.state      = reinterpret_cast<FsstState*>(reinterpret_cast<uint64_t*>(d + b->off_state) + st);
     fc.offsets    = c.offsets;
     fc.chars      = c.chars;
     fc.chars_cap  = c.cap;
     fc.total_rows = c.total_rows;
-    fc.n_chunks   = static_cast<uint32_t>(c.chunks.size());
+    fc.n_chunks   = c.n_chunks;
     dcols[i]      = fc;
-    std::memcpy(h + b->off_chunks + ck * sizeof(FsstChunk), c.chunks.data(), c.chunks.size() * sizeof(FsstChunk));
     auto* hs = reinterpret_cast<FsstSeg*>(h + b->off_segs) + sg;
     for (size_t k = 0; k < c.segs.size(); ++k) {  // first_chunk: an index into the batch's chunk array
       hs[k] = c.segs[k];
       hs[k].first_chunk += static_cast<uint32_t>(ck);
     }
-    ck += c.chunks.size();
+    ck += c.n_chunks;
     sg += c.segs.size();
-    st += 2 + c.chunks.size();
+    st += 2 + c.n_chunks;
   }
   if (!dcols.empty()) std::memcpy(h + b->off_cols, dcols.data(), dcols.size() * sizeof(FsstColumn));
   auto* hi = reinterpret_cast<BatchItem*>(h + b->off_items);
@@ -392,7 +391,7 @@ int batch_collect(sdr_batch* b)
   for (auto& c : b->cols) {
     const uint32_t err   = static_cast<uint32_t>(st[w] >> 32);
     const uint64_t total = st[w + 1];
-    w += 2 + c.chunks.size();
+    w += 2 + c.n_chunks;
     c.user->chars_used = total;
     if (err && !rc) rc = fail(SDR_E_HIP, "string decode: offset look-back did not complete");
     if (!rc && (total > 0x7FFFFFFFull || total > c.cap))

@@ -26,7 +26,7 @@ struct FsstColumn {
   const FsstChunk* chunks;
   const FsstSeg* segs;
   const FsstChunkInfo* info;  // per chunk, from the segment pre-pass
-  const FsstSymTab* tabs;     // per segment, from the segment pre-pass
+  const FsstSymTab* tabs;     // per row range of the batch (a chunk's seg indexes it), from the pre-pass
   FsstState* state;
   int32_t* offsets;
   uint8_t* chars;
@@ -62,9 +62,10 @@ struct BatchQueue {
   uint32_t n_fsst_status;  // status words to clear in `fsst_status`
   uint32_t error;
   uint64_t* fsst_status;   // contiguous FsstState storage of all FSST columns
-  uint32_t n_fsst_segs;    // segments of all FSST columns, for the pre-pass
+  uint32_t n_fsst_segs;    // row ranges of all FSST columns, for the pre-pass
   uint32_t pad;
-  const FsstSeg* fsst_segs;      // their first_chunk index fsst_info
+  const FsstSeg* fsst_segs;      // their first_chunk index fsst_chunks and fsst_info
+  FsstChunk* fsst_chunks;
   FsstChunkInfo* fsst_info;
   FsstSymTab* fsst_tabs;
 };

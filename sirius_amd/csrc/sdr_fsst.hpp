@@ -10,15 +10,20 @@ constexpr uint32_t FSST_CHUNK_ROWS = 1024;        // rows per workgroup
 constexpr uint32_t FSST_GAP        = 0xFFFFFFFFu;  // chunk of rows no segment covers
 constexpr uint32_t FSST_SPIN_LIMIT = 1u << 24;     // look-back wait bound (never reached)
 
-// A segment and the run of chunks that covers it (first_chunk indexes the
-// chunk list the segment list is laid out with).
+// A row range of a column: a segment, or (bytes == nullptr) rows no
+// segment covers, which decode to empty strings.  Its rows are chunks
+// first_chunk, first_chunk + 1, ... of the column's chunk list, which the
+// segment pre-pass writes; the range starts at column row col_row.
 struct FsstSeg {
   const uint8_t* bytes;
   uint32_t bytes_size;
   uint32_t row_count;
   uint32_t first_chunk;
-  uint32_t pad;
+  uint32_t col_row;
 };
+
+// Chunks covering `rows` rows.
+inline uint32_t fsst_chunk_count(uint32_t rows) { return (rows + FSST_CHUNK_ROWS - 1u) / FSST_CHUNK_ROWS; }
 
 // A chunk: `rows` rows starting at segment row `row_begin` (column row
 // `col_row`).  Chunks are listed in column row order.
@@ -66,10 +71,11 @@ struct FsstState {
 };
 
 hipError_t launch_fsst_reset(FsstState* d_state, uint32_t n_chunks, hipStream_t s);
-// Segment pre-pass over n_segs segments (or *d_n_segs when given, for a
-// launch recorded before the count is known).
+// Segment pre-pass over n_segs ranges (or *d_n_segs when given, for a
+// launch recorded before the count is known): writes the chunk list, the
+// chunks' FsstChunkInfo and the segments' symbol tables.
 hipError_t launch_fsst_prep(const FsstSeg* d_segs, uint32_t n_segs, const uint32_t* d_n_segs, uint32_t grid,
-                            FsstChunkInfo* d_info, FsstSymTab* d_tabs, hipStream_t s);
+                            FsstChunk* d_chunks, FsstChunkInfo* d_info, FsstSymTab* d_tabs, hipStream_t s);
 hipError_t launch_fsst(const FsstChunk* d_chunks, uint32_t first, uint32_t n, uint32_t n_chunks_total,
                        const FsstChunkInfo* d_info, const FsstSymTab* d_tabs, FsstState* d_state,
                        int32_t* d_offsets, uint8_t* d_chars, uint64_t chars_cap, uint32_t total_rows,

@@ -60,29 +60,27 @@ int decode_strings(sdr_string_column* col, hipStream_t s, uint32_t mode)
     if (segs[i]->row_offset < segs[i - 1]->row_offset + segs[i - 1]->row_count)
       return fail(SDR_E_BOUNDS, "string segments overlap at row %u", segs[i]->row_offset);
 
+  // row ranges: the segments and the gaps between them; the pre-pass
+  // writes their chunk lists on the device
   std::vector<FsstSeg> hsegs;
-  std::vector<FsstChunk> chunks;
-  std::vector<std::pair<uint32_t, uint32_t>> groups;  // launch groups: [first, count)
-  auto add_range = [&](uint32_t seg, uint32_t col_row, uint32_t rows) {
-    const uint32_t first = static_cast<uint32_t>(chunks.size());
-    for (uint32_t o = 0; o < rows; o += FSST_CHUNK_ROWS)
-      chunks.push_back({seg, seg == FSST_GAP ? 0u : o, std::min(FSST_CHUNK_ROWS, rows - o), col_row + o});
-    groups.push_back({first, static_cast<uint32_t>(chunks.size()) - first});
+  uint32_t n_chunks = 0;
+  auto add_range = [&](const uint8_t* bytes, uint32_t size, uint32_t col_row, uint32_t rows) {
+    hsegs.push_back({bytes, size, rows, n_chunks, col_row});
+    n_chunks += fsst_chunk_count(rows);
   };
   uint32_t cursor = 0;
   for (const sdr_segment* g : segs) {
-    if (g->row_offset > cursor) add_range(FSST_GAP, cursor, g->row_offset - cursor);
-    hsegs.push_back({g->d_bytes, g->bytes_size, g->row_count, static_cast<uint32_t>(chunks.size()), 0});
-    add_range(static_cast<uint32_t>(hsegs.size() - 1), g->row_offset, g->row_count);
+    if (g->row_offset > cursor) add_range(nullptr, 0, cursor, g->row_offset - cursor);
+    add_range(g->d_bytes, g->bytes_size, g->row_offset, g->row_count);
     cursor = g->row_offset + g->row_count;
   }
-  if (cursor < n) add_range(FSST_GAP, cursor, n - cursor);
-  const uint32_t n_chunks = static_cast<uint32_t>(chunks.size());
+  if (cursor < n) add_range(nullptr, 0, cursor, n - cursor);
+  const uint32_t n_ranges = static_cast<uint32_t>(hsegs.size());
 
   auto al = [](size_t v) { return (v + 255u) & ~size_t{255}; };
   const size_t seg_bytes   = al(std::max<size_t>(1, hsegs.size()) * sizeof(FsstSeg));
-  const size_t chunk_bytes = al(chunks.size() * sizeof(FsstChunk));
-  const size_t info_bytes  = al(chunks.size() * sizeof(FsstChunkInfo));
+  const size_t chunk_bytes = al(size_t{n_chunks} * sizeof(FsstChunk));
+  const size_t info_bytes  = al(size_t{n_chunks} * sizeof(FsstChunkInfo));
   const size_t tab_bytes   = al(std::max<size_t>(1, hsegs.size()) * sizeof(FsstSymTab));
   const size_t state_bytes = al(sizeof(FsstState) + sizeof(uint64_t) * n_chunks);
   const size_t total_bytes = seg_bytes + chunk_bytes + info_bytes + tab_bytes + state_bytes + sizeof(unsigned long long);
@@ -98,17 +96,15 @@ int decode_strings(sdr_string_column* col, hipStream_t s, uint32_t mode)
   auto* d_nulls  = reinterpret_cast<unsigned long long*>(d_rest + state_bytes);
 
   hipError_t e = hipSuccess;
-  if (!hsegs.empty()) e = hipMemcpyAsync(d_segs, hsegs.data(), hsegs.size() * sizeof(FsstSeg), hipMemcpyHostToDevice, s);
-  if (e == hipSuccess) e = hipMemcpyAsync(d_chunks, chunks.data(), chunk_bytes, hipMemcpyHostToDevice, s);
+  e = hipMemcpyAsync(d_segs, hsegs.data(), hsegs.size() * sizeof(FsstSeg), hipMemcpyHostToDevice, s);
   if (e == hipSuccess) e = launch_fsst_reset(d_state, n_chunks, s);
-  if (e == hipSuccess && !hsegs.empty())
-    e = launch_fsst_prep(d_segs, static_cast<uint32_t>(hsegs.size()), nullptr,
-                         std::min<uint32_t>(static_cast<uint32_t>(hsegs.size()), 4096u), d_info, d_tabs, s);
+  if (e == hipSuccess)
+    e = launch_fsst_prep(d_segs, n_ranges, nullptr, std::min<uint32_t>(n_ranges, 4096u), d_chunks, d_info, d_tabs, s);
   if (e == hipSuccess) {
     if (mode == SDR_LAUNCH_PER_SEGMENT) {
-      for (size_t i = 0; e == hipSuccess && i < groups.size(); ++i)
-        e = launch_fsst(d_chunks, groups[i].first, groups[i].second, n_chunks, d_info, d_tabs, d_state,
-                        col->d_offsets, col->d_chars, col->d_chars ? col->chars_capacity : 0, n, s);
+      for (uint32_t i = 0; e == hipSuccess && i < n_ranges; ++i)
+        e = launch_fsst(d_chunks, hsegs[i].first_chunk, fsst_chunk_count(hsegs[i].row_count), n_chunks, d_info,
+                        d_tabs, d_state, col->d_offsets, col->d_chars, col->d_chars ? col->chars_capacity : 0, n, s);
     } else {
       e = launch_fsst(d_chunks, 0, n_chunks, n_chunks, d_info, d_tabs, d_state, col->d_offsets, col->d_chars,
                       col->d_chars ? col->chars_capacity : 0, n, s);
