@@ -434,22 +434,65 @@ __device__ __forceinline__ uint32_t fsst_lengths_staged(SM& sm, uint32_t rows, u
   return T;
 }
 
+// One byte of the second walk for a lane whose next row start is in
+// registers or LDS (the per-byte form, used when a 16-byte group holds more
+// row starts than the prefetch covers).  `y` is the byte's stage position.
+template <class SM>
+__device__ __forceinline__ void emit_byte_slow(SM& sm, uint32_t y, uint32_t b, bool valid, uint32_t top,
+                                               uint32_t wofs, uint32_t& r, uint32_t& rend, uint32_t& o, bool& esc)
+{
+  if (valid && y == rend) {  // rows starting at y
+    do {
+      --r;
+      rend = top - sm.comp[r];
+    } while (y == rend);
+    o   = sm.dec[r] - wofs;
+    esc = false;
+  }
+  const bool lit   = esc;
+  const uint32_t l = lit ? 1u : sm.len[b];
+  const uint64_t v = lit ? uint64_t{b} : sm.sym[b];
+  esc              = !lit && b == FSST_ESC;
+  if (valid) {
+    const uint32_t sh = 8u * (o & 3u);
+    const uint64_t w  = v << sh;
+    uint32_t* d       = reinterpret_cast<uint32_t*>(&sm.out[o & ~3u]);
+    __hip_atomic_fetch_or(d, static_cast<uint32_t>(w), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    __hip_atomic_fetch_or(d + 1, static_cast<uint32_t>(w >> 32), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    __hip_atomic_fetch_or(d + 2, static_cast<uint32_t>((uint64_t{static_cast<uint32_t>(v >> 32)} << sh) >> 32),
+                          __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+  }
+  o += l;
+}
+
+constexpr uint32_t EMIT_AHEAD = 4;  // row starts a lane prefetches per 16-byte group
+
 // Second walk: chars.  Rows are emitted in windows of whole rows whose
 // output fits OUT_CAP bytes; each window's output is built in LDS (zeroed,
 // then every symbol ORed in at its byte position, so neighbouring spans
 // may share words) and leaves in 16-byte stores.  Returns false, having
 // written nothing more, when a single row's output exceeds OUT_CAP.
+//
+// A lane walks its span 16 bytes at a time.  At each group it loads, with
+// the group's symbols, the ends and output starts of the next EMIT_AHEAD
+// rows, so the bytes themselves only compare and select: a row starting at
+// byte j moves the output position to its start and clears the escape
+// flag (empty rows start where the next row starts; the last match wins).
+// A lane whose group holds more row starts than that takes the group byte
+// by byte.
 template <uint32_t OUT_CAP, class SM>
 __device__ __forceinline__ bool fsst_emit_staged(SM& sm, uint32_t rows, uint32_t c_h, uint32_t c_hi, uint32_t T,
                                                  uint64_t base, uint8_t* chars, uint64_t chars_cap)
 {
-  const uint32_t tid = threadIdx.x;
-  const uint32_t top = c_h + c_hi;
-  const uintptr_t cb = reinterpret_cast<uintptr_t>(chars);
-  uint32_t r_hi      = rows - 1;  // highest row of the next window
+  const uint32_t tid  = threadIdx.x;
+  const uint32_t lane = lane_id();
+  const uint32_t top  = c_h + c_hi;
+  const uintptr_t cb  = reinterpret_cast<uintptr_t>(chars);
+  uint32_t r_hi       = rows - 1;  // highest row of the next window
   for (;;) {
-    if (tid == 0) {
-      // lowest r_lo with dec[r_hi + 1] - dec[r_lo] <= OUT_CAP
+    if (tid < WAVE) {
+      // lowest r_lo with dec[r_hi + 1] - dec[r_lo] <= OUT_CAP, by 64-way
+      // search rounds over (a, b] (fails at a, holds at b)
       const uint32_t d_top = sm.dec[r_hi + 1];
       uint32_t r_lo;
       if (d_top <= OUT_CAP) {
@@ -457,28 +500,33 @@ __device__ __forceinline__ bool fsst_emit_staged(SM& sm, uint32_t rows, uint32_t
       } else if (d_top - sm.dec[r_hi] > OUT_CAP) {
         r_lo = ~0u;
       } else {
-        uint32_t a = 0, b = r_hi;  // fails at a, holds at b
+        uint32_t a = 0, b = r_hi;
         while (b - a > 1u) {
-          const uint32_t m = (a + b) >> 1;
-          if (d_top - sm.dec[m] <= OUT_CAP) b = m;
-          else a = m;
+          const uint32_t step = (b - a + WAVE - 1u) / WAVE;
+          const uint32_t m    = a + (lane + 1u) * step < b ? a + (lane + 1u) * step : b;
+          const uint64_t hold = ballot(d_top - sm.dec[m] <= OUT_CAP);
+          const uint32_t f    = static_cast<uint32_t>(__builtin_ctzll(hold));  // lane 63 tests b, which holds
+          const uint32_t nb   = a + (f + 1u) * step < b ? a + (f + 1u) * step : b;
+          a                   = a + f * step;
+          b                   = nb;
         }
         r_lo = b;
       }
-      sm.win = r_lo;
+      if (tid == 0) sm.win = r_lo;
     }
     __syncthreads();
     const uint32_t r_lo = sm.win;
     if (r_lo == ~0u) return false;
-    const uint32_t wlo    = sm.dec[r_lo];
-    const uint32_t whi    = sm.dec[r_hi + 1];
-    const uintptr_t g_lo  = cb + base + wlo;
-    const uintptr_t g_hi  = cb + (base + whi < chars_cap ? base + whi : chars_cap);
+    const uint32_t wlo   = sm.dec[r_lo];
+    const uint32_t whi   = sm.dec[r_hi + 1];
+    const uintptr_t g_lo = cb + base + wlo;
+    const uintptr_t g_hi = cb + (base + whi < chars_cap ? base + whi : chars_cap);
     if (g_lo < g_hi) {
       const uint32_t pad  = static_cast<uint32_t>(g_lo & 15u);
       const uintptr_t pa  = g_lo - pad;
       const uint32_t nout = pad + (whi - wlo);
-      for (uint32_t i = tid * 16u; i < nout + 16u; i += FSST_THREADS * 16u)
+      const uint32_t wofs = wlo - pad;  // chunk output position of out[0]
+      for (uint32_t i = tid * 16u; i < nout + 32u; i += FSST_THREADS * 16u)
         *reinterpret_cast<u32x4*>(&sm.out[i]) = u32x4{0u, 0u, 0u, 0u};
       __syncthreads();
       const uint32_t ya   = top - sm.comp[r_hi + 1];
@@ -488,6 +536,8 @@ __device__ __forceinline__ bool fsst_emit_staged(SM& sm, uint32_t rows, uint32_t
       const uint32_t s0   = g0 + tid * span;
       const uint32_t ys   = tid ? s0 : ya;
       const uint32_t ye   = s0 + span < yb ? s0 + span : yb;
+      // bytes outside [ys, ye) OR into this word, which is never flushed
+      const uint32_t trash = (nout + 19u) & ~3u;
       if (ys < ye) {
         // thread 0 starts above the window's highest row and steps into it
         // at ya; the others resume a grid-point state of the first walk
@@ -497,11 +547,9 @@ __device__ __forceinline__ bool fsst_emit_staged(SM& sm, uint32_t rows, uint32_t
           const uint32_t gs = sm.gst[ys >> 4];
           r                 = gs >> GS_ROW;
           esc               = (gs & GS_ESC) != 0;
-          o = sm.dec[r] + (gs & GS_S) - (T - sm.dec[r + 1]) - wlo + pad;
+          o = sm.dec[r] + (gs & GS_S) - (T - sm.dec[r + 1]) - wofs;
         }
-        uint32_t rend  = top - sm.comp[r];
-        uint32_t rnext = r ? top - sm.comp[r - 1] : rend;
-        uint32_t onext = r ? sm.dec[r - 1] : 0u;
+        uint32_t rend = top - sm.comp[r];
 #pragma unroll 1
         for (uint32_t g = ys & ~15u; g < ye; g += 16) {
           const u32x4 q = *reinterpret_cast<const u32x4*>(&sm.cst[g]);
@@ -509,40 +557,63 @@ __device__ __forceinline__ bool fsst_emit_staged(SM& sm, uint32_t rows, uint32_t
           uint64_t V[16];
 #pragma unroll
           for (uint32_t j = 0; j < 16; ++j) {
-            const uint32_t b = group_byte(q, j);
-            L[j]             = sm.len[b];
-            V[j]             = sm.sym[b];
+            const uint32_t bj = group_byte(q, j);
+            L[j]              = sm.len[bj];
+            V[j]              = sm.sym[bj];
           }
-          const uint32_t lo = ys > g ? ys - g : 0u;
-          const uint32_t hi = ye - g < 16u ? ye - g : 16u;
+          // the next EMIT_AHEAD rows below r: ends e[k] and output starts
+          // d[k]; row r - k - 1 starts at e[k - 1] (e[-1] = rend)
+          uint32_t e[EMIT_AHEAD], d[EMIT_AHEAD];
 #pragma unroll
-          for (uint32_t j = 0; j < 16; ++j) {
-            const uint32_t y = g + j;
-            if (y == rend && y < ye) {  // rows starting at y (r > r_lo here)
-              do {
-                --r;
-                o     = onext - wlo + pad;
-                rend  = rnext;
-                rnext = r ? top - sm.comp[r - 1] : rend;
-                onext = r ? sm.dec[r - 1] : 0u;
-              } while (y == rend);
-              esc = false;
+          for (uint32_t k = 0; k < EMIT_AHEAD; ++k) {
+            const bool in    = r > k;
+            const uint32_t i = in ? r - k - 1u : 0u;
+            e[k]             = in ? top - sm.comp[i] : ~0u;
+            d[k]             = sm.dec[i] - wofs;
+          }
+          const uint32_t lim = ye - g < 16u ? ye - g : 16u;  // bytes of the group in the span: [lo, lim)
+          const uint32_t lo  = ys > g ? ys - g : 0u;
+          const bool more    = e[EMIT_AHEAD - 1] - g < lim;  // another row starts inside the group
+          if (!more) {
+            uint32_t st[EMIT_AHEAD];  // start offsets in the group (>= 16 when none)
+#pragma unroll
+            for (uint32_t k = 0; k < EMIT_AHEAD; ++k) st[k] = (k ? e[k - 1] : rend) - g;
+#pragma unroll
+            for (uint32_t j = 0; j < 16; ++j) {
+              bool start = false;
+#pragma unroll
+              for (uint32_t k = 0; k < EMIT_AHEAD; ++k) {
+                const bool s = st[k] == j;
+                o            = s ? d[k] : o;
+                start |= s;
+              }
+              const uint32_t bj = group_byte(q, j);
+              const bool lit    = esc && !start;
+              const uint32_t l  = lit ? 1u : L[j];
+              const uint64_t v  = lit ? uint64_t{bj} : V[j];
+              esc               = !lit && bj == FSST_ESC;
+              const bool valid  = j >= lo && j < lim;
+              const uint32_t sh = 8u * (o & 3u);
+              const uint64_t w  = v << sh;
+              const uint32_t w2 = static_cast<uint32_t>((uint64_t{static_cast<uint32_t>(v >> 32)} << sh) >> 32);
+              uint32_t* dw      = reinterpret_cast<uint32_t*>(&sm.out[valid ? (o & ~3u) : trash]);
+              __hip_atomic_fetch_or(dw, static_cast<uint32_t>(w), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+              __hip_atomic_fetch_or(dw + 1, static_cast<uint32_t>(w >> 32), __ATOMIC_RELAXED,
+                                    __HIP_MEMORY_SCOPE_WORKGROUP);
+              __hip_atomic_fetch_or(dw + 2, w2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+              o += l;
             }
-            const uint32_t b  = group_byte(q, j);
-            const bool valid  = j >= lo && j < hi;
-            const bool lit    = esc;
-            const uint32_t l  = valid ? (lit ? 1u : L[j]) : 0u;
-            const uint64_t v  = valid ? (lit ? uint64_t{b} : V[j]) : 0ull;
-            esc               = valid ? (!lit && b == FSST_ESC) : esc;
-            const uint32_t sh = 8u * (o & 3u);
-            const uint64_t w  = v << sh;
-            const uint32_t w2 = sh ? static_cast<uint32_t>(v >> (64u - sh)) : 0u;
-            uint32_t* d       = reinterpret_cast<uint32_t*>(&sm.out[o & ~3u]);
-            __hip_atomic_fetch_or(d, static_cast<uint32_t>(w), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-            __hip_atomic_fetch_or(d + 1, static_cast<uint32_t>(w >> 32), __ATOMIC_RELAXED,
-                                  __HIP_MEMORY_SCOPE_WORKGROUP);
-            __hip_atomic_fetch_or(d + 2, w2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-            o += l;
+            // rows started in the group
+#pragma unroll
+            for (uint32_t k = 0; k < EMIT_AHEAD; ++k)
+              if (st[k] < lim) {
+                r    = r - 1u;
+                rend = e[k];
+              }
+          } else {
+#pragma unroll 1
+            for (uint32_t j = 0; j < 16; ++j)
+              emit_byte_slow(sm, g + j, group_byte(q, j), j >= lo && j < lim, top, wofs, r, rend, o, esc);
           }
         }
       }
