@@ -66,6 +66,55 @@ void device_free(void* p, hipStream_t s)
   if (p) (void)hipFreeAsync(p, s);
 }
 
+namespace {
+// Pinned staging slots for descriptor uploads, kept for the process.
+struct StageSlot {
+  void* h          = nullptr;
+  size_t cap       = 0;
+  hipEvent_t done  = nullptr;  // recorded after the slot's last copy
+  int dev          = -1;
+};
+std::mutex g_stage_mu;
+std::vector<StageSlot> g_stage;
+constexpr size_t STAGE_SLOTS_MAX = 64;
+constexpr size_t STAGE_MIN_BYTES = 64u << 10;
+}  // namespace
+
+int upload_async(void* d_dst, const void* h_src, size_t bytes, hipStream_t s)
+{
+  if (bytes == 0) return SDR_OK;
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lk(g_stage_mu);
+  StageSlot* slot = nullptr;
+  for (auto& sl : g_stage)
+    if (sl.dev == dev && sl.cap >= bytes && hipEventQuery(sl.done) == hipSuccess) {
+      slot = &sl;
+      break;
+    }
+  if (!slot && g_stage.size() < STAGE_SLOTS_MAX) {
+    StageSlot n;
+    n.dev = dev;
+    n.cap = STAGE_MIN_BYTES;
+    while (n.cap < bytes) n.cap *= 2;
+    HIP_TRY(hipHostMalloc(&n.h, n.cap, hipHostMallocDefault));
+    if (hipEventCreateWithFlags(&n.done, hipEventDisableTiming) != hipSuccess) {
+      (void)hipHostFree(n.h);
+      return fail(SDR_E_HIP, "staging event creation failed");
+    }
+    g_stage.push_back(n);
+    slot = &g_stage.back();
+  }
+  if (!slot) {  // every slot busy: a copy from the caller's memory (the runtime stages it)
+    HIP_TRY(hipMemcpyAsync(d_dst, h_src, bytes, hipMemcpyHostToDevice, s));
+    return SDR_OK;
+  }
+  std::memcpy(slot->h, h_src, bytes);
+  HIP_TRY(hipMemcpyAsync(d_dst, slot->h, bytes, hipMemcpyHostToDevice, s));
+  HIP_TRY(hipEventRecord(slot->done, s));
+  return SDR_OK;
+}
+
 bool type_ok_for(uint32_t codec, uint32_t ts)
 {
   switch (codec) {
@@ -257,8 +306,12 @@ int decode_runs(const std::vector<RunTarget>& targets, hipStream_t s, uint32_t m
   FixedSeg* d = nullptr;
   int rc = device_alloc(reinterpret_cast<void**>(&d), all.size() * sizeof(FixedSeg), s);
   if (rc) return rc;
-  hipError_t e = hipMemcpyAsync(d, all.data(), all.size() * sizeof(FixedSeg), hipMemcpyHostToDevice, s);
-  if (e == hipSuccess) e = launch_batches(batches, all, d, mode, 0, s, nullptr);
+  rc = upload_async(d, all.data(), all.size() * sizeof(FixedSeg), s);
+  if (rc) {
+    device_free(d, s);
+    return rc;
+  }
+  hipError_t e = launch_batches(batches, all, d, mode, 0, s, nullptr);
   device_free(d, s);
   if (e != hipSuccess) return fail(SDR_E_HIP, "kernel launch failed: %s", hipGetErrorString(e));
   return SDR_OK;

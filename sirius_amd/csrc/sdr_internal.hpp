@@ -23,6 +23,9 @@ const char* last_error();
 void ensure_pool(int dev);
 int device_alloc(void** p, size_t bytes, hipStream_t s);
 void device_free(void* p, hipStream_t s);
+// Host-to-device copy of a small host array (descriptors) through a pinned
+// staging slot: asynchronous for the host, ordered on s.
+int upload_async(void* d_dst, const void* h_src, size_t bytes, hipStream_t s);
 
 bool type_ok_for(uint32_t codec, uint32_t ts);
 bool codec_fixed(uint32_t codec);

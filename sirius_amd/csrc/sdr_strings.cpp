@@ -96,8 +96,12 @@ int decode_strings(sdr_string_column* col, hipStream_t s, uint32_t mode)
   auto* d_nulls  = reinterpret_cast<unsigned long long*>(d_rest + state_bytes);
 
   hipError_t e = hipSuccess;
-  e = hipMemcpyAsync(d_segs, hsegs.data(), hsegs.size() * sizeof(FsstSeg), hipMemcpyHostToDevice, s);
-  if (e == hipSuccess) e = launch_fsst_reset(d_state, n_chunks, s);
+  rc = upload_async(d_segs, hsegs.data(), hsegs.size() * sizeof(FsstSeg), s);
+  if (rc) {
+    device_free(d, s);
+    return rc;
+  }
+  e = launch_fsst_reset(d_state, n_chunks, s);
   if (e == hipSuccess)
     e = launch_fsst_prep(d_segs, n_ranges, nullptr, std::min<uint32_t>(n_ranges, 4096u), d_chunks, d_info, d_tabs, s);
   if (e == hipSuccess) {
