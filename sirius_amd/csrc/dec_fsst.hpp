@@ -169,7 +169,7 @@ constexpr uint32_t FSST_SYMSTAGE = 17u + 8u * FSST_NSYM;  // the largest symbol 
 constexpr uint32_t FSST_CSTAGE   = 16384;                  // chunk compressed bytes staged in LDS
 constexpr uint32_t FSST_NGRID    = (FSST_CSTAGE + 16u) / 16u + 2u;
 constexpr uint32_t FSST_OUT_COLUMN = 8192;   // output window of the per-column kernel (4 blocks per CU)
-constexpr uint32_t FSST_OUT_BATCH  = 16384;  // ... of the persistent batch kernel
+constexpr uint32_t FSST_OUT_BATCH  = 8192;   // ... of the persistent batch kernel
 
 // Walk state at a grid point: decoded prefix (bits 0-19), escape flag (bit
 // 20), row (bits 21-31).  A staged chunk decodes to at most 8 * FSST_CSTAGE
@@ -526,8 +526,12 @@ __device__ __forceinline__ bool fsst_emit_staged(SM& sm, uint32_t rows, uint32_t
       const uintptr_t pa  = g_lo - pad;
       const uint32_t nout = pad + (whi - wlo);
       const uint32_t wofs = wlo - pad;  // chunk output position of out[0]
-      for (uint32_t i = tid * 16u; i < nout + 32u; i += FSST_THREADS * 16u)
-        *reinterpret_cast<u32x4*>(&sm.out[i]) = u32x4{0u, 0u, 0u, 0u};
+      // Cleared with 4-byte stores.  Measured on gfx950: clearing with
+      // 16-byte stores (ds_write_b128) and then ORing from other waves
+      // after a barrier lost about 0.2% of the ORed bytes once five or more
+      // workgroups shared a CU; 4-byte stores never did.
+      for (uint32_t i = tid * 4u; i < nout + 32u; i += FSST_THREADS * 4u)
+        *reinterpret_cast<uint32_t*>(&sm.out[i]) = 0u;
       __syncthreads();
       const uint32_t ya   = top - sm.comp[r_hi + 1];
       const uint32_t yb   = top - sm.comp[r_lo];
@@ -553,14 +557,6 @@ __device__ __forceinline__ bool fsst_emit_staged(SM& sm, uint32_t rows, uint32_t
 #pragma unroll 1
         for (uint32_t g = ys & ~15u; g < ye; g += 16) {
           const u32x4 q = *reinterpret_cast<const u32x4*>(&sm.cst[g]);
-          uint32_t L[16];
-          uint64_t V[16];
-#pragma unroll
-          for (uint32_t j = 0; j < 16; ++j) {
-            const uint32_t bj = group_byte(q, j);
-            L[j]              = sm.len[bj];
-            V[j]              = sm.sym[bj];
-          }
           // the next EMIT_AHEAD rows below r: ends e[k] and output starts
           // d[k]; row r - k - 1 starts at e[k - 1] (e[-1] = rend)
           uint32_t e[EMIT_AHEAD], d[EMIT_AHEAD];
@@ -578,8 +574,20 @@ __device__ __forceinline__ bool fsst_emit_staged(SM& sm, uint32_t rows, uint32_t
             uint32_t st[EMIT_AHEAD];  // start offsets in the group (>= 16 when none)
 #pragma unroll
             for (uint32_t k = 0; k < EMIT_AHEAD; ++k) st[k] = (k ? e[k - 1] : rend) - g;
+            // symbols and lengths in two halves of eight bytes (fewer live registers)
 #pragma unroll
-            for (uint32_t j = 0; j < 16; ++j) {
+            for (uint32_t h = 0; h < 16; h += 8) {
+            uint32_t L[8];
+            uint64_t V[8];
+#pragma unroll
+            for (uint32_t i = 0; i < 8; ++i) {
+              const uint32_t bj = group_byte(q, h + i);
+              L[i]              = sm.len[bj];
+              V[i]              = sm.sym[bj];
+            }
+#pragma unroll
+            for (uint32_t i = 0; i < 8; ++i) {
+              const uint32_t j = h + i;
               bool start = false;
 #pragma unroll
               for (uint32_t k = 0; k < EMIT_AHEAD; ++k) {
@@ -589,8 +597,8 @@ __device__ __forceinline__ bool fsst_emit_staged(SM& sm, uint32_t rows, uint32_t
               }
               const uint32_t bj = group_byte(q, j);
               const bool lit    = esc && !start;
-              const uint32_t l  = lit ? 1u : L[j];
-              const uint64_t v  = lit ? uint64_t{bj} : V[j];
+              const uint32_t l  = lit ? 1u : L[i];
+              const uint64_t v  = lit ? uint64_t{bj} : V[i];
               esc               = !lit && bj == FSST_ESC;
               const bool valid  = j >= lo && j < lim;
               const uint32_t sh = 8u * (o & 3u);
@@ -602,6 +610,7 @@ __device__ __forceinline__ bool fsst_emit_staged(SM& sm, uint32_t rows, uint32_t
                                     __HIP_MEMORY_SCOPE_WORKGROUP);
               __hip_atomic_fetch_or(dw + 2, w2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
               o += l;
+            }
             }
             // rows started in the group
 #pragma unroll
