@@ -330,7 +330,7 @@ __device__ void alp_vector(const uint8_t* seg, uint32_t bytes, uint32_t v, uint3
       T x;
       if constexpr (TS == 8) x = __builtin_bit_cast(T, b);
       else x = __builtin_bit_cast(T, static_cast<uint32_t>(b));
-      out[p] = x;
+      gbl(out)[p] = x;
     });
   }
 }
@@ -430,7 +430,7 @@ __device__ void alprd_vector(const uint8_t* seg, uint32_t bytes, uint32_t v, uin
       const R right = static_cast<R>(lds_bits64(lds_r, rhead + p * rbw, rbw));
       const R bits  = full_right ? right
                                  : static_cast<R>(static_cast<R>(ld_u16_any(exc + 2u * e)) << rbw) | right;
-      out[p]        = __builtin_bit_cast(T, bits);
+      gbl(out)[p]   = __builtin_bit_cast(T, bits);
     });
   }
 }
@@ -439,15 +439,17 @@ __device__ void alprd_vector(const uint8_t* seg, uint32_t bytes, uint32_t v, uin
 // UNCOMPRESSED / CONSTANT units: bytes [b0, b1) of the segment's output,
 // written by one wavefront.
 // ===========================================================================
-__device__ __forceinline__ void copy_unit(const uint8_t* src, uint8_t* dst, uint64_t b0, uint64_t b1)
+__device__ __forceinline__ void copy_unit(const uint8_t* src_p, uint8_t* dst_p, uint64_t b0, uint64_t b1)
 {
+  const gptr<const uint8_t> src = gbl(src_p);
+  const gptr<uint8_t> dst       = gbl(dst_p);
   const bool al = ((reinterpret_cast<uintptr_t>(src) | reinterpret_cast<uintptr_t>(dst)) & 15u) == 0;
   if (al) {
     const uint64_t v0 = (b0 + 15u) / 16u, v1 = b1 / 16u;
     for (uint64_t i = b0 + lane_id(); i < v0 * 16u && i < b1; i += WAVE) dst[i] = src[i];
     for (uint64_t i = v0 + lane_id(); i < v1; i += WAVE)
-      __builtin_nontemporal_store(__builtin_nontemporal_load(reinterpret_cast<const u32x4*>(src) + i),
-                                  reinterpret_cast<u32x4*>(dst) + i);
+      __builtin_nontemporal_store(__builtin_nontemporal_load(reinterpret_cast<gptr<const u32x4>>(src) + i),
+                                  reinterpret_cast<gptr<u32x4>>(dst) + i);
     for (uint64_t i = (v1 > v0 ? v1 * 16u : b1) + lane_id(); i < b1; i += WAVE) dst[i] = src[i];
   } else {
     for (uint64_t i = b0 + lane_id(); i < b1; i += WAVE) dst[i] = src[i];
@@ -457,8 +459,8 @@ __device__ __forceinline__ void copy_unit(const uint8_t* src, uint8_t* dst, uint
 __device__ __forceinline__ void const_unit(const uint8_t* val, uint32_t ts, uint8_t* dst, uint64_t b0, uint64_t b1)
 {
   uint8_t v[16];
-  for (uint32_t i = 0; i < ts; ++i) v[i] = val[i];
-  for (uint64_t i = b0 + lane_id(); i < b1; i += WAVE) dst[i] = v[i % ts];
+  for (uint32_t i = 0; i < ts; ++i) v[i] = gbl(val)[i];
+  for (uint64_t i = b0 + lane_id(); i < b1; i += WAVE) gbl(dst)[i] = v[i % ts];
 }
 
 }  // namespace sdr

@@ -52,7 +52,7 @@ constexpr uint64_t ST_VAL         = (1ull << 62) - 1;
 __device__ __forceinline__ uint32_t byte_at(const uint8_t* p, const uint8_t* end)
 {
   if (p >= end) return 0;
-  return *p;
+  return *gbl(p);
 }
 
 // w-bit value at bit `bitpos` of the stream starting at `base`; bytes at or
@@ -68,7 +68,7 @@ __device__ __forceinline__ uint32_t stream_bits32(const uint8_t* base, const uin
   if (p + nb <= end) {
     uintptr_t a = reinterpret_cast<uintptr_t>(p);
     const uint32_t sh = static_cast<uint32_t>(a & 3u);
-    const uint32_t* wp = reinterpret_cast<const uint32_t*>(a - sh);
+    const gptr<const uint32_t> wp = gbl(reinterpret_cast<const uint32_t*>(a - sh));
     const uint32_t need = (sh + nb + 3u) >> 2;  // words spanned, <= 3
     uint32_t w0 = wp[0], w1 = need > 1 ? wp[1] : 0u, w2 = need > 2 ? wp[2] : 0u;
     const uint64_t lo = static_cast<uint64_t>(w0) | (static_cast<uint64_t>(w1) << 32);
@@ -83,14 +83,14 @@ __device__ __forceinline__ uint32_t stream_bits32(const uint8_t* base, const uin
 // Sequential reader of w-bit fields (w <= 32) from an LSB-first stream,
 // loading each aligned word only when its bits are needed.
 struct BitCursor {
-  const uint32_t* w;
+  gptr<const uint32_t> w;
   uint64_t acc;
   uint32_t nbits;
   __device__ __forceinline__ BitCursor(const uint8_t* base, uint64_t bitpos)
   {
     uintptr_t a = reinterpret_cast<uintptr_t>(base + (bitpos >> 3));
     const uint32_t skip = static_cast<uint32_t>((a & 3u) * 8u + (bitpos & 7u));
-    w     = reinterpret_cast<const uint32_t*>(a & ~uintptr_t{3});
+    w     = gbl(reinterpret_cast<const uint32_t*>(a & ~uintptr_t{3}));
     acc   = static_cast<uint64_t>(*w++) >> skip;
     nbits = 32u - skip;
   }
@@ -123,15 +123,15 @@ __device__ __forceinline__ uint32_t sum_fields(const uint8_t* base, const uint8_
   return s;
 }
 
-// Sequential byte reader over [p, p + n) using aligned 4-byte loads.
+// Sequential byte reader over global bytes [p, p + n) using aligned 4-byte loads.
 struct ByteReader {
-  const uint32_t* w;
+  gptr<const uint32_t> w;
   uint32_t cur;
   uint32_t boff;
   __device__ __forceinline__ explicit ByteReader(const uint8_t* p)
   {
     boff = static_cast<uint32_t>(reinterpret_cast<uintptr_t>(p) & 3u);
-    w    = reinterpret_cast<const uint32_t*>(p - boff);
+    w    = gbl(reinterpret_cast<const uint32_t*>(p - boff));
     cur  = *w;
   }
   __device__ __forceinline__ uint32_t next()
@@ -217,13 +217,13 @@ __device__ __forceinline__ void fsst_walk(uint32_t n, Fetch fetch, Lit lit, Sym 
   }
 }
 
-// Decoded length of the n compressed bytes of one row at p, read a word at
-// a time (only the aligned words holding the row).
+// Decoded length of the n compressed bytes of one row at p (global memory),
+// read a word at a time (only the aligned words holding the row).
 template <class SM>
 __device__ __forceinline__ uint32_t fsst_row_len(const uint8_t* p, uint32_t n, const SM& sm)
 {
   const uint32_t boff = static_cast<uint32_t>(reinterpret_cast<uintptr_t>(p) & 3u);
-  const uint32_t* w   = reinterpret_cast<const uint32_t*>(p - boff);
+  const gptr<const uint32_t> w = gbl(reinterpret_cast<const uint32_t*>(p - boff));
   const uint32_t end  = boff + n;
   bool esc            = false;
   uint32_t d          = 0;
@@ -310,10 +310,10 @@ __device__ __attribute__((noinline)) void fsst_emit_rows(SM& sm, const uint8_t* 
     for (uintptr_t blk = pa + uintptr_t{tid} * 16u; blk < hi; blk += uintptr_t{FSST_THREADS} * 16u) {
       if (blk >= lo && blk + 16u <= hi) {
         const u32x4 v = *reinterpret_cast<const u32x4*>(&sm.out[blk - pa]);
-        __builtin_nontemporal_store(v, reinterpret_cast<u32x4*>(blk));
+        __builtin_nontemporal_store(v, gbl(reinterpret_cast<u32x4*>(blk)));
       } else {
         for (uintptr_t x = blk; x < blk + 16u; ++x)
-          if (x >= lo && x < hi) *reinterpret_cast<uint8_t*>(x) = sm.out[x - pa];
+          if (x >= lo && x < hi) *gbl(reinterpret_cast<uint8_t*>(x)) = sm.out[x - pa];
       }
     }
     __syncthreads();
@@ -630,10 +630,10 @@ __device__ __forceinline__ bool fsst_emit_staged(SM& sm, uint32_t rows, uint32_t
       for (uintptr_t blk = pa + uintptr_t{tid} * 16u; blk < g_hi; blk += uintptr_t{FSST_THREADS} * 16u) {
         if (blk >= g_lo && blk + 16u <= g_hi) {
           const u32x4 v = *reinterpret_cast<const u32x4*>(&sm.out[blk - pa]);
-          __builtin_nontemporal_store(v, reinterpret_cast<u32x4*>(blk));
+          __builtin_nontemporal_store(v, gbl(reinterpret_cast<u32x4*>(blk)));
         } else {
           for (uintptr_t x = blk; x < blk + 16u; ++x)
-            if (x >= g_lo && x < g_hi) *reinterpret_cast<uint8_t*>(x) = sm.out[x - pa];
+            if (x >= g_lo && x < g_hi) *gbl(reinterpret_cast<uint8_t*>(x)) = sm.out[x - pa];
         }
       }
     }
@@ -656,7 +656,7 @@ __device__ __forceinline__ uint64_t fsst_lookback(uint32_t c, uint32_t n_chunks,
 {
   if (threadIdx.x < WAVE) {
     const uint32_t lane = threadIdx.x;
-    uint64_t* status    = st->status;
+    const gptr<uint64_t> status = gbl(st->status);
     uint64_t excl64     = 0;
     if (c == 0) {
       if (lane == 0) __hip_atomic_store(&status[0], ST_INC | T, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -673,7 +673,7 @@ __device__ __forceinline__ uint64_t fsst_lookback(uint32_t c, uint32_t n_chunks,
         const uint64_t need = stop == WAVE - 1 ? ~0ull : ((2ull << stop) - 1ull);
         if (ballot(valid && flag == 0) & need) {  // a predecessor has not published yet
           if (++spins > spin_limit) {              // never expected: give up rather than hang
-            if (lane == 0) atomicOr(&st->error, 1u);
+            if (lane == 0) __hip_atomic_fetch_or(gbl(&st->error), 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             break;
           }
           __builtin_amdgcn_s_sleep(1);
@@ -688,7 +688,7 @@ __device__ __forceinline__ uint64_t fsst_lookback(uint32_t c, uint32_t n_chunks,
     }
     if (lane == 0) {
       sm.base = excl64;
-      if (c == n_chunks - 1) st->total = excl64 + T;
+      if (c == n_chunks - 1) *gbl(&st->total) = excl64 + T;
     }
   }
   __syncthreads();
@@ -701,9 +701,10 @@ __device__ __forceinline__ void fsst_offsets(const FsstChunk& ch, uint32_t c, ui
                                              uint32_t T, uint64_t base, int32_t* offsets, uint32_t total_rows,
                                              const SM& sm)
 {
+  const gptr<int32_t> off = gbl(offsets);
   for (uint32_t i = threadIdx.x; i < rows; i += FSST_THREADS)
-    offsets[ch.col_row + i] = static_cast<int32_t>(static_cast<uint32_t>(base + sm.dec[i]));
-  if (c == n_chunks - 1 && threadIdx.x == 0) offsets[total_rows] = static_cast<int32_t>(static_cast<uint32_t>(base + T));
+    off[ch.col_row + i] = static_cast<int32_t>(static_cast<uint32_t>(base + sm.dec[i]));
+  if (c == n_chunks - 1 && threadIdx.x == 0) off[total_rows] = static_cast<int32_t>(static_cast<uint32_t>(base + T));
 }
 
 // ---- segment pre-pass ----
@@ -759,7 +760,8 @@ __device__ __forceinline__ void fsst_prep_segment(const FsstSeg sg, uint32_t si,
   for (uint32_t k = tid; k * FSST_CHUNK_ROWS < sg.row_count; k += FSST_THREADS) {
     const uint32_t r0 = k * FSST_CHUNK_ROWS;
     const uint32_t n  = sg.row_count - r0 < FSST_CHUNK_ROWS ? sg.row_count - r0 : FSST_CHUNK_ROWS;
-    chunks[sg.first_chunk + k] = FsstChunk{gap ? FSST_GAP : si, gap ? 0u : r0, n, sg.col_row + r0};
+    const FsstChunk fc{gap ? FSST_GAP : si, gap ? 0u : r0, n, sg.col_row + r0};
+    *gbl(reinterpret_cast<u32x4*>(chunks + sg.first_chunk + k)) = __builtin_bit_cast(u32x4, fc);
   }
   if (gap) return;  // uniform: nothing else of a gap is read
   bool ok = false;
@@ -813,8 +815,8 @@ __device__ __forceinline__ void fsst_prep_segment(const FsstSeg sg, uint32_t si,
       }
     }
   }
-  tab->sym[tid] = sym;
-  tab->len[tid] = static_cast<uint8_t>(len);
+  gbl(tab->sym)[tid] = sym;
+  gbl(tab->len)[tid] = static_cast<uint8_t>(len);
 
   // chunk bounds: wave w sums chunks w, w + 4, ..., 16 fields per lane
   const uint32_t rows = sg.row_count;
@@ -860,7 +862,10 @@ __device__ __forceinline__ void fsst_prep_segment(const FsstSeg sg, uint32_t si,
       ci.flags      = flags;
       ci.bytes_size = sg.bytes_size;
       ci.pad        = 0;
-      info[sg.first_chunk + k] = ci;
+      const gptr<u32x4> dst = gbl(reinterpret_cast<u32x4*>(info + sg.first_chunk + k));
+      const u32x8 cv        = __builtin_bit_cast(u32x8, ci);
+      dst[0]                = cv.lo;
+      dst[1]                = cv.hi;
     }
     carry += T;
     __syncthreads();  // sum[] and huge[] are read before the next batch refills them

@@ -57,9 +57,9 @@ __global__ __launch_bounds__(BLOCK_THREADS) void k_batch(const BatchItem* __rest
     const BatchItem it = items[idx];
     switch (it.kind) {
       case BK_FSST: {
-        const FsstColumn col = *it.column;
-        const FsstChunk ch   = col.chunks[it.chunk];
-        fsst_chunk<FSST_OUT_BATCH>(ch, col.info[it.chunk], col.tabs + (ch.seg == FSST_GAP ? 0u : ch.seg), it.chunk,
+        const FsstColumn col = gload(it.column);
+        const FsstChunk ch   = gload(col.chunks + it.chunk);
+        fsst_chunk<FSST_OUT_BATCH>(ch, gload(col.info + it.chunk), col.tabs + (ch.seg == FSST_GAP ? 0u : ch.seg), it.chunk,
                                    col.n_chunks, col.state, col.offsets, col.chars, col.chars_cap, col.total_rows,
                                    BATCH_SPIN_LIMIT, sm.fsst);
         break;
@@ -99,7 +99,7 @@ __global__ __launch_bounds__(BLOCK_THREADS) void k_batch(const BatchItem* __rest
 __global__ void k_batch_reset(BatchQueue* q)
 {
   const uint32_t n = q->n_fsst_status;
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) q->fsst_status[i] = 0;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) gbl(q->fsst_status)[i] = 0;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     q->claimed = 0;
     q->error   = 0;
@@ -111,7 +111,7 @@ __global__ __launch_bounds__(FSST_THREADS) void k_batch_fsst_prep(const BatchQue
   __shared__ FsstPrepSmem sm;
   const uint32_t n = q->n_fsst_segs;
   for (uint32_t i = blockIdx.x; i < n; i += gridDim.x)
-    fsst_prep_segment(q->fsst_segs[i], i, q->fsst_chunks, q->fsst_info, q->fsst_tabs + i, sm);
+    fsst_prep_segment(gload(q->fsst_segs + i), i, q->fsst_chunks, q->fsst_info, q->fsst_tabs + i, sm);
 }
 
 hipError_t launch_batch_fsst_prep(BatchQueue* q, hipStream_t s)

@@ -84,14 +84,14 @@ __global__ __launch_bounds__(BLOCK_THREADS) void k_copy(const FixedSeg* __restri
   const uint64_t b0    = uint64_t{blockIdx.x} * COPY_UNIT;
   if (b0 >= total) return;
   const uint64_t b1 = total - b0 < COPY_UNIT ? total : b0 + COPY_UNIT;
-  const uint8_t* src = sd.bytes;
-  uint8_t* dst       = sd.out;
-  const bool al = ((reinterpret_cast<uintptr_t>(src) | reinterpret_cast<uintptr_t>(dst)) & 15u) == 0;
+  const gptr<const uint8_t> src = gbl(sd.bytes);
+  const gptr<uint8_t> dst       = gbl(sd.out);
+  const bool al = ((reinterpret_cast<uintptr_t>(sd.bytes) | reinterpret_cast<uintptr_t>(sd.out)) & 15u) == 0;
   if (al) {
     const uint64_t v0 = b0 / 16, v1 = b1 / 16;
     for (uint64_t i = v0 + threadIdx.x; i < v1; i += BLOCK_THREADS)
-      __builtin_nontemporal_store(__builtin_nontemporal_load(reinterpret_cast<const u32x4*>(src) + i),
-                                  reinterpret_cast<u32x4*>(dst) + i);
+      __builtin_nontemporal_store(__builtin_nontemporal_load(reinterpret_cast<gptr<const u32x4>>(src) + i),
+                                  reinterpret_cast<gptr<u32x4>>(dst) + i);
     for (uint64_t i = v1 * 16 + threadIdx.x; i < b1; i += BLOCK_THREADS) dst[i] = src[i];
   } else {
     for (uint64_t i = b0 + threadIdx.x; i < b1; i += BLOCK_THREADS) dst[i] = src[i];
@@ -107,8 +107,8 @@ __global__ __launch_bounds__(BLOCK_THREADS) void k_constant(const FixedSeg* __re
   if (b0 >= total) return;
   const uint64_t b1 = total - b0 < COPY_UNIT ? total : b0 + COPY_UNIT;
   uint8_t v[16];
-  for (uint32_t i = 0; i < type_size; ++i) v[i] = sd.bytes[i];
-  for (uint64_t i = b0 + threadIdx.x; i < b1; i += BLOCK_THREADS) sd.out[i] = v[i % type_size];
+  for (uint32_t i = 0; i < type_size; ++i) v[i] = gbl(sd.bytes)[i];
+  for (uint64_t i = b0 + threadIdx.x; i < b1; i += BLOCK_THREADS) gbl(sd.out)[i] = v[i % type_size];
 }
 
 __global__ void k_fill_u8(uint8_t* p, uint8_t v, uint64_t n)

@@ -115,6 +115,97 @@ int upload_async(void* d_dst, const void* h_src, size_t bytes, hipStream_t s)
   return SDR_OK;
 }
 
+namespace {
+// Descriptor slots for the per-call decode: a pinned host buffer and a
+// device buffer, reused once the event recorded after the last launch that
+// read them has completed.  At most DESC_SLOTS_PER_DEV are in flight per
+// device; a call beyond that waits for the oldest one (the GPU still has
+// the others queued), so once the slots exist a decode allocates nothing.
+// The depth lets the GPU ride out host stalls of a few milliseconds.
+struct DescSlot {
+  void* h         = nullptr;
+  void* d         = nullptr;
+  size_t cap      = 0;
+  hipEvent_t done = nullptr;
+  int dev         = -1;
+  bool held       = false;  // acquired and not yet released
+  uint64_t seq    = 0;      // release order
+};
+std::mutex g_desc_mu;
+std::vector<DescSlot*> g_desc;
+uint64_t g_desc_seq = 0;
+constexpr size_t DESC_SLOTS_PER_DEV = 32;  // about 8 ms of headline steps queued ahead of the host
+constexpr size_t DESC_MIN_BYTES     = 64u << 10;
+}  // namespace
+
+int desc_acquire(const void* h_src, size_t bytes, hipStream_t s, void** d_out, void** token)
+{
+  *d_out = nullptr;
+  *token = nullptr;
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lk(g_desc_mu);
+  DescSlot* pick   = nullptr;
+  DescSlot* oldest = nullptr;
+  size_t n_dev     = 0;
+  for (DescSlot* sl : g_desc) {
+    if (sl->dev != dev) continue;
+    ++n_dev;
+    if (sl->held) continue;
+    if (hipEventQuery(sl->done) == hipSuccess) {
+      if (!pick || (sl->cap >= bytes && pick->cap < bytes)) pick = sl;
+    } else if (!oldest || sl->seq < oldest->seq) {
+      oldest = sl;
+    }
+  }
+  if (!pick && (n_dev < DESC_SLOTS_PER_DEV || !oldest)) {
+    auto* n = new DescSlot;
+    n->dev  = dev;
+    if (hipEventCreateWithFlags(&n->done, hipEventDisableTiming) != hipSuccess) {
+      delete n;
+      return fail(SDR_E_HIP, "descriptor slot event creation failed");
+    }
+    g_desc.push_back(n);
+    pick = n;
+  }
+  if (!pick) {  // every slot in flight: wait for the oldest
+    HIP_TRY(hipEventSynchronize(oldest->done));
+    pick = oldest;
+  }
+  if (pick->cap < bytes) {  // its event has completed: nothing still reads the old buffers
+    if (pick->h) HIP_TRY(hipHostFree(pick->h));
+    if (pick->d) HIP_TRY(hipFree(pick->d));
+    pick->h = pick->d = nullptr;
+    pick->cap         = 0;
+    size_t cap        = DESC_MIN_BYTES;
+    while (cap < bytes) cap *= 2;
+    HIP_TRY(hipHostMalloc(&pick->h, cap, hipHostMallocDefault));
+    HIP_TRY(hipMalloc(&pick->d, cap));
+    pick->cap = cap;
+  }
+  std::memcpy(pick->h, h_src, bytes);
+  HIP_TRY(hipMemcpyAsync(pick->d, pick->h, bytes, hipMemcpyHostToDevice, s));
+  pick->held = true;
+  *d_out     = pick->d;
+  *token     = pick;
+  return SDR_OK;
+}
+
+int desc_release(void* token, hipStream_t s)
+{
+  if (!token) return SDR_OK;
+  auto* sl = static_cast<DescSlot*>(token);
+  std::lock_guard<std::mutex> lk(g_desc_mu);
+  const hipError_t e = hipEventRecord(sl->done, s);
+  sl->seq  = ++g_desc_seq;
+  sl->held = false;
+  if (e != hipSuccess) {  // the slot cannot be tracked: wait for the stream instead
+    (void)hipStreamSynchronize(s);
+    return fail(SDR_E_HIP, "descriptor slot event: %s", hipGetErrorString(e));
+  }
+  return SDR_OK;
+}
+
 bool type_ok_for(uint32_t codec, uint32_t ts)
 {
   switch (codec) {
@@ -303,18 +394,14 @@ int decode_runs(const std::vector<RunTarget>& targets, hipStream_t s, uint32_t m
   std::vector<FixedSeg> all;
   build_batches(targets, batches, all);
   if (all.empty()) return SDR_OK;
-  FixedSeg* d = nullptr;
-  int rc = device_alloc(reinterpret_cast<void**>(&d), all.size() * sizeof(FixedSeg), s);
+  void* d     = nullptr;
+  void* token = nullptr;
+  int rc = desc_acquire(all.data(), all.size() * sizeof(FixedSeg), s, &d, &token);
   if (rc) return rc;
-  rc = upload_async(d, all.data(), all.size() * sizeof(FixedSeg), s);
-  if (rc) {
-    device_free(d, s);
-    return rc;
-  }
-  hipError_t e = launch_batches(batches, all, d, mode, 0, s, nullptr);
-  device_free(d, s);
+  const hipError_t e = launch_batches(batches, all, static_cast<FixedSeg*>(d), mode, 0, s, nullptr);
+  rc                 = desc_release(token, s);
   if (e != hipSuccess) return fail(SDR_E_HIP, "kernel launch failed: %s", hipGetErrorString(e));
-  return SDR_OK;
+  return rc;
 }
 
 int decode_table(sdr_column* cols, uint32_t n_cols, hipStream_t s, uint32_t mode)

@@ -10,12 +10,15 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 namespace sdr {
 
 constexpr uint32_t WAVE = 64;
 
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+typedef uint32_t u32x8 __attribute__((ext_vector_type(8)));
 
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 
@@ -29,15 +32,48 @@ __device__ __forceinline__ void wave_lds_sync()
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// ---- global-memory pointers ------------------------------------------------
+//
+// Pointers read from descriptors (segment bytes, output columns, the FSST
+// chars base) are generic to the compiler, which then emits flat loads and
+// stores.  A flat access also counts on the LDS counter, so every later LDS
+// wait would also wait for outstanding stores.  Accesses through gbl() are
+// global instructions.
+
+template <typename T>
+using gptr = __attribute__((address_space(1))) T*;
+
+template <typename T>
+__device__ __forceinline__ gptr<T> gbl(T* p)
+{
+  return (gptr<T>)(p);
+}
+
+// A whole descriptor read through a global pointer (struct copies through an
+// address-space-qualified lvalue are not allowed).
+template <typename T>
+__device__ __forceinline__ T gload(const T* p)
+{
+  using W = std::conditional_t<alignof(T) >= 8, u32x2, uint32_t>;
+  static_assert(sizeof(T) % sizeof(W) == 0, "descriptor size");
+  struct alignas(alignof(T)) Words {
+    W w[sizeof(T) / sizeof(W)];
+  } v;
+  const gptr<const W> src = gbl(reinterpret_cast<const W*>(p));
+#pragma unroll
+  for (uint32_t i = 0; i < sizeof(T) / sizeof(W); ++i) v.w[i] = src[i];
+  return __builtin_bit_cast(T, v);
+}
+
 // ---- unaligned loads -------------------------------------------------------
 
-__device__ __forceinline__ uint32_t ld_u8(const uint8_t* p) { return *p; }
+__device__ __forceinline__ uint32_t ld_u8(const uint8_t* p) { return *gbl(p); }
 
 __device__ __forceinline__ uint32_t ld_u16_any(const uint8_t* p)
 {
   uintptr_t a = reinterpret_cast<uintptr_t>(p);
   uint32_t o  = static_cast<uint32_t>(a & 3u);
-  const uint32_t* w = reinterpret_cast<const uint32_t*>(a - o);
+  const gptr<const uint32_t> w = gbl(reinterpret_cast<const uint32_t*>(a - o));
   uint32_t lo = w[0];
   uint32_t v  = (o <= 2u) ? (lo >> (8u * o)) : __builtin_amdgcn_alignbit(w[1], lo, 24u);
   return v & 0xFFFFu;
@@ -47,7 +83,7 @@ __device__ __forceinline__ uint32_t ld_u32_any(const uint8_t* p)
 {
   uintptr_t a = reinterpret_cast<uintptr_t>(p);
   uint32_t o  = static_cast<uint32_t>(a & 3u);
-  const uint32_t* w = reinterpret_cast<const uint32_t*>(a - o);
+  const gptr<const uint32_t> w = gbl(reinterpret_cast<const uint32_t*>(a - o));
   uint32_t lo = w[0];
   if (o == 0) return lo;
   return __builtin_amdgcn_alignbit(w[1], lo, 8u * o);
@@ -270,7 +306,7 @@ __device__ __forceinline__ uint32_t block_stage(uint32_t* lds, const uint8_t* p,
 template <typename T>
 __device__ __forceinline__ void st_nt(T* p, T v)
 {
-  __builtin_nontemporal_store(v, p);
+  __builtin_nontemporal_store(v, gbl(p));
 }
 
 // Store VPL = 16 / sizeof(T) consecutive values; one 16-byte store when the
@@ -282,7 +318,7 @@ __device__ __forceinline__ void store_vec(T* p, const T* v, uint32_t n_valid, bo
   if (aligned16 && n_valid >= VPL) {
     u32x4 x;
     __builtin_memcpy(&x, v, 16);
-    __builtin_nontemporal_store(x, reinterpret_cast<u32x4*>(p));
+    __builtin_nontemporal_store(x, gbl(reinterpret_cast<u32x4*>(p)));
   } else {
 #pragma unroll
     for (uint32_t i = 0; i < VPL; ++i)

@@ -26,6 +26,11 @@ void device_free(void* p, hipStream_t s);
 // Host-to-device copy of a small host array (descriptors) through a pinned
 // staging slot: asynchronous for the host, ordered on s.
 int upload_async(void* d_dst, const void* h_src, size_t bytes, hipStream_t s);
+// Per-call descriptors in a reused slot (pinned + device buffer): copies
+// h_src to the slot's device buffer on s and returns it; desc_release marks
+// the slot free once the work enqueued on s so far has completed.
+int desc_acquire(const void* h_src, size_t bytes, hipStream_t s, void** d_out, void** token);
+int desc_release(void* token, hipStream_t s);
 
 bool type_ok_for(uint32_t codec, uint32_t ts);
 bool codec_fixed(uint32_t codec);
