@@ -37,8 +37,14 @@ constexpr uint32_t RLE_MAX_SUPERS  = (RLE_MAX_ENTRIES + RLE_SUPER - 1) / RLE_SUP
 constexpr uint32_t RLE_STAGE_WORDS = (RLE_SUPER * 2 + 48) / 4;
 static_assert(RLE_SUPER % RLE_TILE == 0, "passes hold whole tiles");
 
+// Per-wave bitmap of run ends over a window's rows: 64 rows of front pad,
+// RLE_BM_ROWS window rows and 64 rows a step may overrun.
+constexpr uint32_t RLE_BM_ROWS  = 4096;
+constexpr uint32_t RLE_BM_WORDS = (RLE_BM_ROWS + 128u) / 32u + 2u;
+
 struct RleSmem {
   uint32_t stage[RLE_STAGE_WORDS];         // counts of the resident pass
+  uint32_t bm[RLE_WAVES][RLE_BM_WORDS];    // run-end bitmap of each wave's window
   uint32_t chunk_base[RLE_THREADS];        // row base of each 32-entry chunk
   uint32_t wave_sum[RLE_WAVES];
   uint32_t pass_base[RLE_MAX_SUPERS + 1];  // row base of each pass
@@ -126,35 +132,34 @@ __device__ __forceinline__ uint32_t rle_pass(const uint8_t* counts, uint32_t t0,
   return total;
 }
 
-// Value width is a runtime (block-uniform) byte count: one code path serves
-// all four widths, branching only at the value load and the store.
-__device__ __forceinline__ uint64_t ld_width(const uint8_t* p, uint32_t ts)
+// The value width TS (1, 2, 4 or 8 bytes) is a template parameter: each
+// width is its own code path with no per-row dispatch.
+template <uint32_t TS>
+__device__ __forceinline__ uint64_t ld_width(const uint8_t* p)
 {
-  switch (ts) {
-    case 1: return ld_u8(p);
-    case 2: return ld_u16_any(p);
-    case 4: return ld_u32_any(p);
-    default: return ld_u64_any(p);
-  }
+  if constexpr (TS == 1) return ld_u8(p);
+  else if constexpr (TS == 2) return ld_u16_any(p);
+  else if constexpr (TS == 4) return ld_u32_any(p);
+  else return ld_u64_any(p);
 }
 
-__device__ __forceinline__ void st_width(uint8_t* out, uint32_t row, uint64_t v, uint32_t ts)
+template <uint32_t TS>
+__device__ __forceinline__ void st_width(uint8_t* out, uint32_t row, uint64_t v)
 {
-  switch (ts) {
-    case 1: st_nt(out + row, static_cast<uint8_t>(v)); break;
-    case 2: st_nt(reinterpret_cast<uint16_t*>(out) + row, static_cast<uint16_t>(v)); break;
-    case 4: st_nt(reinterpret_cast<uint32_t*>(out) + row, static_cast<uint32_t>(v)); break;
-    default: st_nt(reinterpret_cast<uint64_t*>(out) + row, v); break;
-  }
+  if constexpr (TS == 1) st_nt(out + row, static_cast<uint8_t>(v));
+  else if constexpr (TS == 2) st_nt(reinterpret_cast<uint16_t*>(out) + row, static_cast<uint16_t>(v));
+  else if constexpr (TS == 4) st_nt(reinterpret_cast<uint32_t*>(out) + row, static_cast<uint32_t>(v));
+  else st_nt(reinterpret_cast<uint64_t*>(out) + row, v);
 }
 
 // Expand rows [lo, hi) of one wave: entries [t0, j_end) are the pass
 // resident in LDS (byte skew head), chunk bases in sm.
-__device__ __forceinline__ void rle_expand(const uint8_t* values, uint32_t ts, uint32_t t0, uint32_t j_end,
-                                           uint32_t head, uint32_t lo, uint32_t hi, uint8_t* out,
-                                           const RleSmem& sm)
+template <uint32_t TS>
+__device__ __forceinline__ void rle_expand(const uint8_t* values, uint32_t t0, uint32_t j_end,
+                                           uint32_t head, uint32_t lo, uint32_t hi, uint8_t* out, RleSmem& sm)
 {
   const uint32_t lane = lane_id();
+  uint32_t* bm        = sm.bm[threadIdx.x / WAVE];
   // chunk holding the first run that ends after row lo: the last chunk with
   // entries whose base is <= lo (bases never decrease up to the match)
   uint32_t cnt = 0;
@@ -184,17 +189,40 @@ __device__ __forceinline__ void rle_expand(const uint8_t* values, uint32_t ts, u
     const uint32_t jl   = j + lane;
     const bool live     = jl < j_end;
     const uint32_t cnt1 = live ? lds_u16(sm.stage, head + 2u * (jl - t0)) : 0u;
-    const uint64_t val  = live ? ld_width(values + uint64_t{jl} * ts, ts) : 0u;
+    const uint64_t val  = live ? ld_width<TS>(values + uint64_t{jl} * TS) : 0u;
     const uint32_t incl = wave_incl_sum_u32(cnt1);
     const uint32_t end  = wstart + incl;  // exclusive end row of run jl
     const uint32_t wend = wstart + __builtin_amdgcn_readlane(incl, WAVE - 1);
     const bool dup      = ballot(live && cnt1 == 0u) != 0;
     const uint32_t a    = wstart > lo ? wstart : lo;
     const uint32_t b    = wend < hi ? wend : hi;
+    // A window of at most RLE_BM_ROWS rows marks its run ends in the wave's
+    // bitmap once (bit 64 + end - wstart); a step then reads its 64 bits
+    // instead of collecting the ends one readlane at a time.
+    const bool use_bm = !dup && wend - wstart <= RLE_BM_ROWS && a < b;
+    if (use_bm) {
+      const uint32_t nw = (wend - wstart + 128u) / 32u + 2u;
+      for (uint32_t i = lane; i < nw; i += WAVE) bm[i] = 0u;
+      wave_lds_sync();
+      if (live) {
+        const uint32_t bit = 64u + (end - wstart);
+        __hip_atomic_fetch_or(&bm[bit >> 5], 1u << (bit & 31u), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+      }
+      wave_lds_sync();
+    }
     for (uint32_t r0 = lo + ((a - lo) & ~(WAVE - 1u)); r0 < b; r0 += WAVE) {
       const uint32_t row = r0 + lane;
       uint32_t rank;
-      if (!dup) {
+      if (use_bm) {
+        // bits of ends r0 .. r0 + 63; r0 >= wstart - 63, so the index is >= 1
+        const uint32_t b0 = 64u + r0 - wstart;
+        const uint32_t q  = b0 >> 5, sh = b0 & 31u;
+        const uint32_t w0 = bm[q], w1 = bm[q + 1], w2 = bm[q + 2];
+        const uint64_t m  = (static_cast<uint64_t>(__builtin_amdgcn_alignbit(w2, w1, sh)) << 32) |
+                            __builtin_amdgcn_alignbit(w1, w0, sh);
+        rank = static_cast<uint32_t>(__builtin_popcountll(ballot(live && end <= r0))) +
+               static_cast<uint32_t>(__builtin_popcountll(m & upto & ~1ull));
+      } else if (!dup) {
         rank              = static_cast<uint32_t>(__builtin_popcountll(ballot(live && end <= r0)));
         uint64_t inside   = ballot(live && end > r0 && end - r0 < WAVE);
         unsigned long long m = 0;
@@ -215,8 +243,8 @@ __device__ __forceinline__ void rle_expand(const uint8_t* values, uint32_t ts, u
       const int src = static_cast<int>((rank < WAVE ? rank : WAVE - 1u) << 2);
       const uint32_t xl = __builtin_amdgcn_ds_bpermute(src, static_cast<int>(static_cast<uint32_t>(val)));
       const uint32_t xh =
-        ts == 8u ? __builtin_amdgcn_ds_bpermute(src, static_cast<int>(static_cast<uint32_t>(val >> 32))) : 0u;
-      if (row >= a && row < b) st_width(out, row, static_cast<uint64_t>(xl) | (static_cast<uint64_t>(xh) << 32), ts);
+        TS == 8u ? __builtin_amdgcn_ds_bpermute(src, static_cast<int>(static_cast<uint32_t>(val >> 32))) : 0u;
+      if (row >= a && row < b) st_width<TS>(out, row, static_cast<uint64_t>(xl) | (static_cast<uint64_t>(xh) << 32));
     }
     j += WAVE;
     wstart = wend;
@@ -226,8 +254,9 @@ __device__ __forceinline__ void rle_expand(const uint8_t* values, uint32_t ts, u
 // Decode rows [ra, rb) of an RLE segment of `rows` rows of `ts`-byte values
 // (1, 2, 4 or 8) into out[0, rows).  Called by all 256 threads of a block
 // (the control flow is block-uniform).
-__device__ void rle_slice(const uint8_t* seg, uint32_t bytes, uint32_t rows, uint32_t ra, uint32_t rb, uint8_t* out,
-                          uint32_t ts, RleSmem& sm)
+template <uint32_t TS>
+__device__ void rle_slice_t(const uint8_t* seg, uint32_t bytes, uint32_t rows, uint32_t ra, uint32_t rb, uint8_t* out,
+                            RleSmem& sm)
 {
   bool malformed        = true;
   uint64_t counts_off   = 0;
@@ -236,7 +265,7 @@ __device__ void rle_slice(const uint8_t* seg, uint32_t bytes, uint32_t rows, uin
     counts_off = su64(seg);
     if (counts_off >= 8u && counts_off < bytes) {
       const uint64_t vb = counts_off - 8u, cb = bytes - counts_off;
-      uint64_t n_real   = (vb % ts == 0) ? vb / ts : cb / 2u;
+      uint64_t n_real   = (vb % TS == 0) ? vb / TS : cb / 2u;
       if (n_real > RLE_MAX_ENTRIES) n_real = RLE_MAX_ENTRIES;
       if (n_real > cb / 2u) n_real = cb / 2u;
       n_max     = static_cast<uint32_t>(n_real);
@@ -264,7 +293,7 @@ __device__ void rle_slice(const uint8_t* seg, uint32_t bytes, uint32_t rows, uin
       if (bad != UINT32_MAX) break;
       base += total;
     }
-    if (!malformed && 8u + uint64_t{ec} * ts > bytes) malformed = true;
+    if (!malformed && 8u + uint64_t{ec} * TS > bytes) malformed = true;
   }
 
   const uint32_t w  = threadIdx.x / WAVE;
@@ -273,7 +302,7 @@ __device__ void rle_slice(const uint8_t* seg, uint32_t bytes, uint32_t rows, uin
   const uint32_t wa = ra + w * qw < rb ? ra + w * qw : rb;
   const uint32_t wb = wa + qw < rb ? wa + qw : rb;
   if (malformed) {
-    for (uint32_t r = wa + lane_id(); r < wb; r += WAVE) st_width(out, r, 0u, ts);
+    for (uint32_t r = wa + lane_id(); r < wb; r += WAVE) st_width<TS>(out, r, 0u);
     return;
   }
 
@@ -294,7 +323,19 @@ __device__ void rle_slice(const uint8_t* seg, uint32_t bytes, uint32_t rows, uin
     const uint32_t j_end = ec < t0 + RLE_SUPER ? ec : t0 + RLE_SUPER;
     const uint32_t lo    = wa > r_lo ? wa : r_lo;
     const uint32_t hi    = wb < r_hi ? wb : r_hi;
-    if (lo < hi) rle_expand(values, ts, t0, j_end, head, lo, hi, out, sm);
+    if (lo < hi) rle_expand<TS>(values, t0, j_end, head, lo, hi, out, sm);
+  }
+}
+
+// Runtime-width entry (the persistent batch kernel): one dispatch per slice.
+__device__ __forceinline__ void rle_slice(const uint8_t* seg, uint32_t bytes, uint32_t rows, uint32_t ra, uint32_t rb,
+                                          uint8_t* out, uint32_t ts, RleSmem& sm)
+{
+  switch (ts) {
+    case 1: rle_slice_t<1>(seg, bytes, rows, ra, rb, out, sm); break;
+    case 2: rle_slice_t<2>(seg, bytes, rows, ra, rb, out, sm); break;
+    case 4: rle_slice_t<4>(seg, bytes, rows, ra, rb, out, sm); break;
+    default: rle_slice_t<8>(seg, bytes, rows, ra, rb, out, sm); break;
   }
 }
 

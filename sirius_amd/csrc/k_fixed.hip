@@ -54,9 +54,9 @@ __global__ __launch_bounds__(BLOCK_THREADS) void k_alp(const FixedSeg* __restric
 // share one L2 and its counts are fetched from HBM about once.
 constexpr uint32_t N_XCD = 8;
 
+template <uint32_t TS>
 __global__ __launch_bounds__(RLE_THREADS) void k_rle(const FixedSeg* __restrict__ segs, uint32_t seg_base,
-                                                    uint32_t n_segs, uint32_t slices, uint32_t slice_rows,
-                                                    uint32_t type_size)
+                                                    uint32_t n_segs, uint32_t slices, uint32_t slice_rows)
 {
   __shared__ RleSmem sm;
   const uint32_t per_xcd = gridDim.x / N_XCD;
@@ -67,7 +67,7 @@ __global__ __launch_bounds__(RLE_THREADS) void k_rle(const FixedSeg* __restrict_
   const uint32_t ra = (logical % slices) * slice_rows;
   if (ra >= sd.row_count) return;
   const uint32_t rb = sd.row_count - ra < slice_rows ? sd.row_count : ra + slice_rows;
-  rle_slice(sd.bytes, sd.bytes_size, sd.row_count, ra, rb, sd.out, type_size, sm);
+  rle_slice_t<TS>(sd.bytes, sd.bytes_size, sd.row_count, ra, rb, sd.out, sm);
 }
 
 // ===========================================================================
@@ -180,8 +180,13 @@ hipError_t launch_rle(const FixedSeg* d_segs, uint32_t seg_base, uint32_t n, uin
   for (uint32_t b = 0; b < n; b += per) {
     const uint32_t m      = n - b < per ? n - b : per;
     const uint64_t blocks = (uint64_t{m} * slices + N_XCD - 1) / N_XCD * N_XCD;
-    hipLaunchKernelGGL(k_rle, dim3(static_cast<uint32_t>(blocks)), dim3(RLE_THREADS), 0, s, d_segs, seg_base + b, m,
-                       slices, sr, ts);
+    const dim3 grid(static_cast<uint32_t>(blocks)), block(RLE_THREADS);
+    switch (ts) {
+      case 1: hipLaunchKernelGGL(k_rle<1>, grid, block, 0, s, d_segs, seg_base + b, m, slices, sr); break;
+      case 2: hipLaunchKernelGGL(k_rle<2>, grid, block, 0, s, d_segs, seg_base + b, m, slices, sr); break;
+      case 4: hipLaunchKernelGGL(k_rle<4>, grid, block, 0, s, d_segs, seg_base + b, m, slices, sr); break;
+      default: hipLaunchKernelGGL(k_rle<8>, grid, block, 0, s, d_segs, seg_base + b, m, slices, sr); break;
+    }
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
   }
