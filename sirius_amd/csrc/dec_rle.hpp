@@ -38,9 +38,9 @@ constexpr uint32_t RLE_STAGE_WORDS = (RLE_SUPER * 2 + 48) / 4;
 static_assert(RLE_SUPER % RLE_TILE == 0, "passes hold whole tiles");
 
 // Per-wave bitmap of run ends over a window's rows: 64 rows of front pad,
-// RLE_BM_ROWS window rows and 64 rows a step may overrun.
+// RLE_BM_ROWS window rows and 128 rows a pair of steps may overrun.
 constexpr uint32_t RLE_BM_ROWS  = 4096;
-constexpr uint32_t RLE_BM_WORDS = (RLE_BM_ROWS + 128u) / 32u + 2u;
+constexpr uint32_t RLE_BM_WORDS = (RLE_BM_ROWS + 192u) / 32u + 2u;
 
 struct RleSmem {
   uint32_t stage[RLE_STAGE_WORDS];         // counts of the resident pass
@@ -200,8 +200,9 @@ __device__ __forceinline__ void rle_expand(const uint8_t* values, uint32_t t0, u
     // bitmap once (bit 64 + end - wstart); a step then reads its 64 bits
     // instead of collecting the ends one readlane at a time.
     const bool use_bm = !dup && wend - wstart <= RLE_BM_ROWS && a < b;
+    const uint32_t first = lo + ((a - lo) & ~(WAVE - 1u));
     if (use_bm) {
-      const uint32_t nw = (wend - wstart + 128u) / 32u + 2u;
+      const uint32_t nw = (wend - wstart + 192u) / 32u + 2u;
       for (uint32_t i = lane; i < nw; i += WAVE) bm[i] = 0u;
       wave_lds_sync();
       if (live) {
@@ -209,20 +210,38 @@ __device__ __forceinline__ void rle_expand(const uint8_t* values, uint32_t t0, u
         __hip_atomic_fetch_or(&bm[bit >> 5], 1u << (bit & 31u), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
       }
       wave_lds_sync();
-    }
-    for (uint32_t r0 = lo + ((a - lo) & ~(WAVE - 1u)); r0 < b; r0 += WAVE) {
-      const uint32_t row = r0 + lane;
-      uint32_t rank;
-      if (use_bm) {
-        // bits of ends r0 .. r0 + 63; r0 >= wstart - 63, so the index is >= 1
+      // two independent 64-row steps per iteration, so their LDS round trips
+      // overlap; r0 >= wstart - 63, so every bitmap index is >= 1
+      auto rank_at = [&](uint32_t r0) {
         const uint32_t b0 = 64u + r0 - wstart;
         const uint32_t q  = b0 >> 5, sh = b0 & 31u;
         const uint32_t w0 = bm[q], w1 = bm[q + 1], w2 = bm[q + 2];
         const uint64_t m  = (static_cast<uint64_t>(__builtin_amdgcn_alignbit(w2, w1, sh)) << 32) |
                             __builtin_amdgcn_alignbit(w1, w0, sh);
-        rank = static_cast<uint32_t>(__builtin_popcountll(ballot(live && end <= r0))) +
+        return static_cast<uint32_t>(__builtin_popcountll(ballot(live && end <= r0))) +
                static_cast<uint32_t>(__builtin_popcountll(m & upto & ~1ull));
-      } else if (!dup) {
+      };
+      for (uint32_t r0 = first; r0 < b; r0 += 2u * WAVE) {
+        const uint32_t rk0 = rank_at(r0), rk1 = rank_at(r0 + WAVE);
+        const int s0 = static_cast<int>((rk0 < WAVE ? rk0 : WAVE - 1u) << 2);
+        const int s1 = static_cast<int>((rk1 < WAVE ? rk1 : WAVE - 1u) << 2);
+        const uint32_t vl = static_cast<uint32_t>(val), vh = static_cast<uint32_t>(val >> 32);
+        const uint32_t l0 = __builtin_amdgcn_ds_bpermute(s0, static_cast<int>(vl));
+        const uint32_t l1 = __builtin_amdgcn_ds_bpermute(s1, static_cast<int>(vl));
+        const uint32_t h0 = TS == 8u ? __builtin_amdgcn_ds_bpermute(s0, static_cast<int>(vh)) : 0u;
+        const uint32_t h1 = TS == 8u ? __builtin_amdgcn_ds_bpermute(s1, static_cast<int>(vh)) : 0u;
+        const uint32_t row0 = r0 + lane, row1 = r0 + WAVE + lane;
+        if (row0 >= a && row0 < b) st_width<TS>(out, row0, static_cast<uint64_t>(l0) | (static_cast<uint64_t>(h0) << 32));
+        if (row1 >= a && row1 < b) st_width<TS>(out, row1, static_cast<uint64_t>(l1) | (static_cast<uint64_t>(h1) << 32));
+      }
+      j += WAVE;
+      wstart = wend;
+      continue;
+    }
+    for (uint32_t r0 = first; r0 < b; r0 += WAVE) {
+      const uint32_t row = r0 + lane;
+      uint32_t rank;
+      if (!dup) {
         rank              = static_cast<uint32_t>(__builtin_popcountll(ballot(live && end <= r0)));
         uint64_t inside   = ballot(live && end > r0 && end - r0 < WAVE);
         unsigned long long m = 0;
