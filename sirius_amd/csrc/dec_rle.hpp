@@ -185,11 +185,23 @@ __device__ __forceinline__ void rle_expand(const uint8_t* values, uint32_t t0, u
     wstart      = __builtin_amdgcn_readlane(end - c, l);
   }
   const unsigned long long upto = lane == WAVE - 1 ? ~0ull : ((2ull << lane) - 1ull);
+  // run counts and values of a window; the next window's are fetched before
+  // this one's rows are written, so the value load overlaps the stores
+  auto fetch = [&](uint32_t jw, uint32_t& c, uint64_t& v) {
+    const uint32_t jl = jw + lane;
+    const bool lv     = jl < j_end;
+    c                 = lv ? lds_u16(sm.stage, head + 2u * (jl - t0)) : 0u;
+    v                 = lv ? ld_width<TS>(values + uint64_t{jl} * TS) : 0u;
+  };
+  uint32_t cnt_next = 0;
+  uint64_t val_next = 0;
+  fetch(j, cnt_next, val_next);
   while (wstart < hi && j < j_end) {
     const uint32_t jl   = j + lane;
     const bool live     = jl < j_end;
-    const uint32_t cnt1 = live ? lds_u16(sm.stage, head + 2u * (jl - t0)) : 0u;
-    const uint64_t val  = live ? ld_width<TS>(values + uint64_t{jl} * TS) : 0u;
+    const uint32_t cnt1 = cnt_next;
+    const uint64_t val  = val_next;
+    if (j + WAVE < j_end) fetch(j + WAVE, cnt_next, val_next);
     const uint32_t incl = wave_incl_sum_u32(cnt1);
     const uint32_t end  = wstart + incl;  // exclusive end row of run jl
     const uint32_t wend = wstart + __builtin_amdgcn_readlane(incl, WAVE - 1);
