@@ -16,11 +16,12 @@
 //      entries cost one pass, i.e. one memory round trip.
 //   2. expansion: the slice is split over the 4 wavefronts; each finds its
 //      first run from the LDS chunk bases, then walks windows of 64 runs:
-//      a DPP scan gives each run's end row, and for every 64-row step the
-//      ends that fall inside the step become bits of a uniform 64-bit mask
-//      (built with scalar readlanes), so a row's run index is the ballot
-//      count of ends at or before the step plus a popcount of the mask.
-//      Windows holding a zero-length run take an exact per-lane count.
+//      a DPP scan gives each run's end row.  A window of at most 4096 rows
+//      marks its run ends in a per-wave LDS bitmap, and each 64-row step
+//      reads its 64 bits, so a row's run index is the ballot count of ends
+//      at or before the step plus a popcount of those bits.  Longer windows
+//      build the step's mask with scalar readlanes; windows holding a
+//      zero-length run take an exact per-lane count.
 #pragma once
 
 #include "sdr_device.hpp"
