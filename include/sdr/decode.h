@@ -356,8 +356,11 @@ uint64_t sdr_strings_upper_bound(const sdr_run* runs, uint32_t n_runs);
 
 /* Decode one VARCHAR column.  Replaces gpu_decode_strings_column
  * (src/include/cuda/scan/gpu_decode_strings.cuh:71-73,
- * src/cuda/scan/gpu_decode_strings.cu:100-402).  Synchronises `stream` once
- * to read back chars_used. */
+ * src/cuda/scan/gpu_decode_strings.cu:100-402).  Runs may mix FSST,
+ * DICTIONARY and UNCOMPRESSED segments (src/cuda/scan/strings/fsst.cu,
+ * dictionary.cu, uncompressed.cu); DICT_FSST returns
+ * SDR_E_UNSUPPORTED_CODEC.  Synchronises `stream` once to read back
+ * chars_used. */
 int sdr_decode_strings_column(sdr_string_column* col, sdr_stream_t stream);
 int sdr_decode_strings_column_mode(sdr_string_column* col, sdr_stream_t stream,
                                    uint32_t launch_mode);

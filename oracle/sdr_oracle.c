@@ -608,6 +608,97 @@ int oracle_fsst_decode(const uint8_t* seg, uint32_t bytes_size, uint32_t rows, u
 }
 
 /* ------------------------------------------------------------------------ */
+/* DICTIONARY and UNCOMPRESSED strings                                       */
+/* ------------------------------------------------------------------------ */
+
+/* One row of a plain (non-FSST) string segment: its length and the segment
+ * offset of its first byte.  Bytes of the row at or past bytes_size read as
+ * zero when copied (plain_copy).
+ *
+ * DICTIONARY follows src/cuda/scan/strings/dictionary.cu:17-33 (layout),
+ * :57-64 (header: dict_size, dict_end, index_buffer_offset,
+ * index_buffer_count, bitpacking_width; valid if the index buffer and
+ * dict_end fit and the width is <= 32), :84-108 (lengths: selection value
+ * sel from the bit-packed buffer at byte 20; 0 means NULL, sel >=
+ * index_buffer_count is empty, else idx[sel] - idx[sel-1]) and :117-140
+ * (bytes at dict_end - idx[sel]).
+ *
+ * UNCOMPRESSED follows src/cuda/scan/strings/uncompressed.cu:17-33 (layout),
+ * :56-81 (segment valid if bytes_size >= 8 + 4 rows; length |off[i]| -
+ * |off[i-1]| of the int32 offsets at byte 8) and :88-120 (bytes at dict_end -
+ * |off[i]|, dict_end a u32 at byte 4).
+ *
+ * Deliberate divergence, malformed input only: a row whose end passes
+ * dict_end or whose end falls below its start is empty.  The reference's
+ * gather would copy from outside the segment there (and, for DICTIONARY
+ * with sel >= index_buffer_count, copy bytes its length pass counted as
+ * zero). */
+typedef struct plain_seg {
+  uint32_t codec, ok, dict_end, ibo, ibc, width;
+} plain_seg;
+
+static plain_seg plain_prepare(uint32_t codec, const uint8_t* seg, uint32_t size, uint32_t rows)
+{
+  plain_seg h;
+  memset(&h, 0, sizeof(h));
+  h.codec = codec;
+  if (codec == SDR_CODEC_DICTIONARY) {
+    if (size < 20u) return h;
+    h.dict_end = ld32(seg + 4);
+    h.ibo      = ld32(seg + 8);
+    h.ibc      = ld32(seg + 12);
+    h.width    = ld32(seg + 16);
+    h.ok = (uint64_t)h.ibo + 4ull * h.ibc <= size && h.dict_end <= size && h.width <= 32u;
+  } else {
+    h.ok = (uint64_t)size >= 8ull + 4ull * rows;
+    h.dict_end = h.ok ? ld32(seg + 4) : 0u;
+  }
+  return h;
+}
+
+static uint32_t abs32(int32_t v) { return v < 0 ? (uint32_t)(-(int64_t)v) : (uint32_t)v; }
+
+static uint32_t plain_row(const plain_seg* h, const uint8_t* seg, uint32_t size, uint32_t i, uint32_t* start)
+{
+  uint32_t e0, e1;
+  *start = 0;
+  if (!h->ok) return 0;
+  if (h->codec == SDR_CODEC_DICTIONARY) {
+    uint32_t sel = (uint32_t)read_bits(seg + 20, size - 20u, (uint64_t)i * h->width, h->width);
+    if (sel == 0 || sel >= h->ibc) return 0;
+    e1 = ld32(seg + h->ibo + 4ull * sel);
+    e0 = ld32(seg + h->ibo + 4ull * (sel - 1u));
+  } else {
+    e1 = abs32((int32_t)ld32(seg + 8 + 4ull * i));
+    e0 = i ? abs32((int32_t)ld32(seg + 8 + 4ull * (i - 1u))) : 0u;
+  }
+  if (e1 > h->dict_end || e1 < e0) return 0;
+  *start = h->dict_end - e1;
+  return e1 - e0;
+}
+
+static void plain_lengths(uint32_t codec, const uint8_t* seg, uint32_t size, uint32_t rows, uint32_t* lengths)
+{
+  plain_seg h = plain_prepare(codec, seg, size, rows);
+  uint32_t st;
+  for (uint32_t i = 0; i < rows; ++i) lengths[i] = plain_row(&h, seg, size, i, &st);
+}
+
+static void plain_decode(uint32_t codec, const uint8_t* seg, uint32_t size, uint32_t rows, uint8_t* chars,
+                         uint64_t cap)
+{
+  plain_seg h = plain_prepare(codec, seg, size, rows);
+  uint64_t o = 0;
+  for (uint32_t i = 0; i < rows; ++i) {
+    uint32_t st, l = plain_row(&h, seg, size, i, &st);
+    for (uint32_t k = 0; k < l && o < cap; ++k, ++o) {
+      uint64_t at = (uint64_t)st + k;
+      chars[o] = at < size ? seg[at] : 0u; /* dict_end of UNCOMPRESSED is not bounded by the reference */
+    }
+  }
+}
+
+/* ------------------------------------------------------------------------ */
 /* UNCOMPRESSED / CONSTANT / validity                                        */
 /* follows src/cuda/scan/gpu_native_decode.cu:154-170 (CONSTANT), :221-286   */
 /* (UNCOMPRESSED), :396-476 (validity overlay), :519-538 (all-valid mask)    */
@@ -818,6 +909,7 @@ int oracle_decode_table_mt(sdr_column* cols, uint32_t n_cols, int n_threads)
 
 typedef struct str_job {
   const sdr_segment* g;
+  uint32_t codec;
   uint32_t* lengths; /* column-wide lengths */
   int32_t* offsets;  /* column-wide offsets (phase 2) */
   uint8_t* chars;
@@ -838,7 +930,15 @@ static void* str_worker(void* arg)
     if (i >= p->n) break;
     str_job* j = &p->jobs[i];
     const sdr_segment* g = j->g;
-    if (p->phase == 0) {
+    if (j->codec != SDR_CODEC_FSST) {
+      if (p->phase == 0) {
+        plain_lengths(j->codec, g->d_bytes, g->bytes_size, g->row_count, j->lengths + g->row_offset);
+      } else {
+        uint64_t base = (uint32_t)j->offsets[g->row_offset];
+        uint64_t end = (uint32_t)j->offsets[g->row_offset + g->row_count];
+        plain_decode(j->codec, g->d_bytes, g->bytes_size, g->row_count, j->chars + base, end - base);
+      }
+    } else if (p->phase == 0) {
       oracle_fsst_lengths(g->d_bytes, g->bytes_size, g->row_count, j->lengths + g->row_offset);
     } else {
       uint64_t base = (uint32_t)j->offsets[g->row_offset];
@@ -878,7 +978,9 @@ int oracle_decode_strings_column_mt(sdr_string_column* col, int n_threads)
   if (col->n_data && !col->data) return SDR_E_INVALID_ARG;
   if (n > 0x7FFFFFFFu) return SDR_E_BOUNDS;
   for (uint32_t r = 0; r < col->n_data; ++r)
-    if (col->data[r].codec != SDR_CODEC_FSST) return SDR_E_UNSUPPORTED_CODEC;
+    if (col->data[r].codec != SDR_CODEC_FSST && col->data[r].codec != SDR_CODEC_DICTIONARY &&
+        col->data[r].codec != SDR_CODEC_UNCOMPRESSED)
+      return SDR_E_UNSUPPORTED_CODEC;
   int rc = check_runs_bounds(col->data, col->n_data, n);
   if (rc) return rc;
   if (col->has_nulls && col->n_validity) {
@@ -900,7 +1002,7 @@ int oracle_decode_strings_column_mt(sdr_string_column* col, int n_threads)
     for (uint32_t s = 0; s < col->data[r].n_segments; ++s) {
       const sdr_segment* g = &col->data[r].segments[s];
       if (!g->row_count) continue;
-      str_job j = {g, lengths, col->d_offsets, col->d_chars};
+      str_job j = {g, col->data[r].codec, lengths, col->d_offsets, col->d_chars};
       p.jobs[p.n++] = j;
     }
   run_str_pool(&p, 0, n_threads);

@@ -651,7 +651,7 @@ __device__ __forceinline__ bool fsst_emit_staged(SM& sm, uint32_t rows, uint32_t
 // suffice; release/acquire orderings would add an L2 write-back /
 // invalidate per access.  Returns the exclusive prefix to every thread.
 template <class SM>
-__device__ __forceinline__ uint64_t fsst_lookback(uint32_t c, uint32_t n_chunks, uint32_t T, FsstState* st,
+__device__ __forceinline__ uint64_t fsst_lookback(uint32_t c, uint32_t n_chunks, uint64_t T, FsstState* st,
                                                   uint32_t spin_limit, SM& sm)
 {
   if (threadIdx.x < WAVE) {
@@ -742,6 +742,193 @@ __device__ __forceinline__ uint32_t fields_sum(const uint8_t* base, const uint8_
   return s;
 }
 
+// ---- DICTIONARY and UNCOMPRESSED strings ----
+//
+// DICTIONARY (strings/dictionary.cu:17-33): a 20-byte header {dict_size,
+// dict_end, index_buffer_offset, index_buffer_count, bitpacking_width}, the
+// bit-packed selection values from byte 20 (0 is NULL), u32 cumulative
+// entry ends at index_buffer_offset and entry bytes stored back to front
+// ending at dict_end: entry s is [dict_end - idx[s], dict_end - idx[s-1]).
+// The header is valid if the index buffer and dict_end lie inside the
+// segment and the width is <= 32 (:57-64).
+//
+// UNCOMPRESSED (strings/uncompressed.cu:17-33): {dict_size, dict_end} then
+// one int32 cumulative end per row from byte 8, read as |v| (the sign is
+// DuckDB's in-memory marker); row i is [dict_end - |off[i]|, dict_end -
+// |off[i-1]|).  The segment is valid if its offsets fit (:66); dict_end
+// must also lie inside the segment here (the reference does not check it).
+//
+// Malformed input only: a row whose end passes dict_end or falls below its
+// start decodes empty, where the reference would copy from outside the
+// segment; a DICTIONARY selection >= index_buffer_count is empty in both of
+// the reference's passes' intent (its length pass gives 0).
+
+// Per-chunk info of a plain segment, one thread per chunk.
+__device__ __forceinline__ void plain_prep_segment(const FsstSeg& sg, FsstChunkInfo* __restrict__ info)
+{
+  const uint8_t* seg = sg.bytes;
+  const bool dict    = sg.codec == SDR_CODEC_DICTIONARY;
+  uint32_t dict_end = 0, ibo = 0, ibc = 0, width = 0;
+  bool ok = false;
+  if (dict) {
+    if (sg.bytes_size >= 20u) {
+      dict_end = su32(seg + 4);
+      ibo      = su32(seg + 8);
+      ibc      = su32(seg + 12);
+      width    = su32(seg + 16);
+      ok = uint64_t{ibo} + 4ull * ibc <= sg.bytes_size && dict_end <= sg.bytes_size && width <= 32u;
+    }
+  } else if (uint64_t{sg.bytes_size} >= 8ull + 4ull * sg.row_count) {
+    dict_end = su32(seg + 4);
+    ok       = dict_end <= sg.bytes_size;
+  }
+  const uint32_t flags = (dict ? FI_DICT | (width & FI_WIDTH) : FI_UNC) | (ok ? FI_OK : 0u);
+  for (uint32_t k = threadIdx.x; k * FSST_CHUNK_ROWS < sg.row_count; k += FSST_THREADS) {
+    FsstChunkInfo ci;
+    ci.bytes      = seg;
+    ci.c_lo       = ibo;
+    ci.c_hi       = ibc;
+    ci.dict_end   = dict_end;
+    ci.flags      = flags;
+    ci.bytes_size = sg.bytes_size;
+    ci.pad        = 0;
+    const gptr<u32x4> dst = gbl(reinterpret_cast<u32x4*>(info + sg.first_chunk + k));
+    const u32x8 cv        = __builtin_bit_cast(u32x8, ci);
+    dst[0]                = cv.lo;
+    dst[1]                = cv.hi;
+  }
+}
+
+// Block sum of one u64 per thread (every thread gets it).
+template <class SM>
+__device__ __forceinline__ uint64_t block_sum_u64(uint64_t v, SM& sm)
+{
+  const uint64_t w = readlane_u64(wave_incl_sum_u64(v), WAVE - 1);
+  if (lane_id() == 0) {
+    sm.part[2u * (threadIdx.x / WAVE)]      = static_cast<uint32_t>(w);
+    sm.part[2u * (threadIdx.x / WAVE) + 1u] = static_cast<uint32_t>(w >> 32);
+  }
+  __syncthreads();
+  uint64_t t = 0;
+#pragma unroll
+  for (uint32_t k = 0; k < FSST_WAVES; ++k)
+    t += static_cast<uint64_t>(sm.part[2u * k]) | (static_cast<uint64_t>(sm.part[2u * k + 1u]) << 32);
+  __syncthreads();
+  return t;
+}
+
+// Chars of a plain chunk from dec[] (row starts) and comp[] (each row's
+// first byte in the segment), one row per thread, in 8 KiB LDS pieces
+// flushed as in fsst_emit_rows.
+template <class SM>
+__device__ __forceinline__ void plain_emit_rows(SM& sm, const uint8_t* seg, const uint8_t* end,
+                                                         uint32_t rows, uint32_t T, uint64_t base, uint8_t* chars,
+                                                         uint64_t chars_cap)
+{
+  const uint32_t tid = threadIdx.x;
+  const uintptr_t cb = reinterpret_cast<uintptr_t>(chars);
+  const uintptr_t g0 = cb + base;
+  const uintptr_t g1 = cb + ((base + T < chars_cap) ? base + T : chars_cap);
+  for (uintptr_t pa = g0 & ~uintptr_t{15}; pa < g1; pa += FSST_PIECE) {
+    const uintptr_t pb = pa + FSST_PIECE;
+#pragma unroll 1
+    for (uint32_t k = 0; k < ROWS_PER_THR; ++k) {
+      const uint32_t r = k * FSST_THREADS + tid;
+      if (r >= rows) continue;
+      const uint32_t d0 = sm.dec[r], dl = sm.dec[r + 1] - d0;
+      const uintptr_t o0 = g0 + d0;
+      if (dl == 0 || o0 + dl <= pa || o0 >= pb) continue;
+      const uint32_t j0 = o0 < pa ? static_cast<uint32_t>(pa - o0) : 0u;
+      const uint32_t j1 = o0 + dl > pb ? static_cast<uint32_t>(pb - o0) : dl;
+      const uint8_t* src = seg + sm.comp[r];
+      uint8_t* dst       = sm.out + (o0 - pa);
+      if (src + dl <= end) {
+        ByteReader br(src + j0);
+        for (uint32_t j = j0; j < j1; ++j) dst[j] = static_cast<uint8_t>(br.next());
+      } else {
+        for (uint32_t j = j0; j < j1; ++j) dst[j] = static_cast<uint8_t>(byte_at(src + j, end));
+      }
+    }
+    __syncthreads();
+    const uintptr_t lo = pa > g0 ? pa : g0;
+    const uintptr_t hi = pb < g1 ? pb : g1;
+    for (uintptr_t blk = pa + uintptr_t{tid} * 16u; blk < hi; blk += uintptr_t{FSST_THREADS} * 16u) {
+      if (blk >= lo && blk + 16u <= hi) {
+        const u32x4 v = *reinterpret_cast<const u32x4*>(&sm.out[blk - pa]);
+        __builtin_nontemporal_store(v, gbl(reinterpret_cast<u32x4*>(blk)));
+      } else {
+        for (uintptr_t x = blk; x < blk + 16u; ++x)
+          if (x >= lo && x < hi) *gbl(reinterpret_cast<uint8_t*>(x)) = sm.out[x - pa];
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// Decode chunk c of a valid DICTIONARY or UNCOMPRESSED segment: per-row
+// lengths (dictionary.cu:84-108, uncompressed.cu:56-81), the chunk's
+// look-back for its chars offset, offsets, then the bytes (dictionary.cu:
+// 117-140, uncompressed.cu:88-120).  Every thread must call it.
+// A leaf call with scalar arguments: descriptors passed by value, or a
+// call from here, would need a stack in scratch memory.
+template <class SM>
+__device__ __attribute__((noinline)) void plain_chunk(const uint8_t* seg, uint32_t bytes_size, uint32_t dict_end,
+                                                     uint32_t flags, uint32_t ibo, uint32_t ibc, uint32_t row_begin,
+                                                     uint32_t rows, uint32_t col_row, uint32_t c, uint32_t n_chunks,
+                                                     FsstState* __restrict__ st, int32_t* __restrict__ offsets,
+                                                     uint8_t* __restrict__ chars, uint64_t chars_cap,
+                                                     uint32_t total_rows, uint32_t spin_limit, SM& sm)
+{
+  const uint32_t tid  = threadIdx.x;
+  const uint8_t* end  = seg + bytes_size;
+  const bool dict     = flags & FI_DICT;
+  const uint32_t w    = flags & FI_WIDTH;
+  const FsstChunk ch{0u, row_begin, rows, col_row};
+  __syncthreads();  // the previous user of sm is done
+  uint32_t T   = 0;
+  uint64_t own = 0;
+#pragma unroll 1
+  for (uint32_t k = 0; k < ROWS_PER_THR; ++k) {
+    const uint32_t r = k * FSST_THREADS + tid;
+    uint32_t len = 0, src = 0;
+    if (r < rows) {
+      const uint32_t sr = ch.row_begin + r;
+      uint32_t e0 = 0, e1 = 0;
+      if (dict) {
+        const uint32_t sel = stream_bits32(seg + 20, end, uint64_t{sr} * w, w);
+        if (sel != 0u && sel < ibc) {
+          e1 = ld_u32_any(seg + ibo + 4ull * sel);
+          e0 = ld_u32_any(seg + ibo + 4ull * (sel - 1u));
+        }
+      } else {
+        const int32_t v1 = static_cast<int32_t>(ld_u32_any(seg + 8 + 4ull * sr));
+        const int32_t v0 = sr ? static_cast<int32_t>(ld_u32_any(seg + 8 + 4ull * (sr - 1u))) : 0;
+        e1 = v1 < 0 ? 0u - static_cast<uint32_t>(v1) : static_cast<uint32_t>(v1);
+        e0 = v0 < 0 ? 0u - static_cast<uint32_t>(v0) : static_cast<uint32_t>(v0);
+      }
+      if (e1 <= dict_end && e1 >= e0) {
+        len = e1 - e0;
+        src = dict_end - e1;
+      }
+      sm.comp[r] = src;
+    }
+    own += len;
+    uint32_t tot      = 0;
+    const uint32_t ex = block_excl_scan(len, sm.wave, &tot);
+    if (r < rows) sm.dec[r] = T + ex;
+    T += tot;
+  }
+  if (tid == 0) sm.dec[rows] = T;
+  // Rows are bounded by the segment, but a chunk of long rows can pass
+  // 2^31 chars: publish a saturated total so the column reports overflow.
+  const uint64_t T64 = block_sum_u64(own, sm);
+  const uint64_t Tp  = T64 > 0x7FFFFFFFull ? 0x80000000ull : T64;
+  const uint64_t b   = fsst_lookback(c, n_chunks, Tp, st, spin_limit, sm);
+  fsst_offsets(ch, c, n_chunks, rows, static_cast<uint32_t>(Tp), b, offsets, total_rows, sm);
+  if (T64 == 0 || Tp != T64 || b >= chars_cap) return;
+  plain_emit_rows(sm, seg, end, rows, T, b, chars, chars_cap);
+}
+
 // Range `si` with the calling 256-thread workgroup: its chunk descriptors
 // into chunks[first_chunk + k]; for a segment, header checks
 // (fsst.cu:79-87), the symbol table import, one code per thread
@@ -764,6 +951,10 @@ __device__ __forceinline__ void fsst_prep_segment(const FsstSeg sg, uint32_t si,
     *gbl(reinterpret_cast<u32x4*>(chunks + sg.first_chunk + k)) = __builtin_bit_cast(u32x4, fc);
   }
   if (gap) return;  // uniform: nothing else of a gap is read
+  if (sg.codec == SDR_CODEC_DICTIONARY || sg.codec == SDR_CODEC_UNCOMPRESSED) {
+    plain_prep_segment(sg, info);
+    return;  // uniform per segment
+  }
   bool ok = false;
   uint32_t dict_end = 0, width = 0, symoff = 0;
   if (sg.bytes_size >= 16u) {
@@ -891,6 +1082,11 @@ __device__ __forceinline__ void fsst_chunk(const FsstChunk ch, const FsstChunkIn
 {
   const uint32_t tid      = threadIdx.x;
   const bool ok           = ch.seg != FSST_GAP && (ci.flags & FI_OK);
+  if (ok && (ci.flags & (FI_DICT | FI_UNC))) {  // uniform per chunk
+    plain_chunk(ci.bytes, ci.bytes_size, ci.dict_end, ci.flags, ci.c_lo, ci.c_hi, ch.row_begin, ch.rows, ch.col_row,
+                c, n_chunks, st, offsets, chars, chars_cap, total_rows, spin_limit, sm);
+    return;
+  }
   const uint8_t* seg      = ok ? ci.bytes : nullptr;
   const uint8_t* end      = ok ? ci.bytes + ci.bytes_size : nullptr;
   const uint32_t dict_end = ok ? ci.dict_end : 0u;

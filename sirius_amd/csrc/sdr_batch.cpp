@@ -155,26 +155,27 @@ int batch_add_strings(sdr_batch* b, sdr_string_column* col)
   bc.chars      = col->d_chars;
   bc.cap        = col->d_chars ? col->chars_capacity : 0;
   bc.total_rows = n;
-  std::vector<const sdr_segment*> segs;
+  std::vector<std::pair<const sdr_segment*, uint32_t>> segs;  // segment, codec
   for (uint32_t r = 0; r < col->n_data; ++r)
     for (uint32_t k = 0; k < col->data[r].n_segments; ++k)
-      if (col->data[r].segments[k].row_count) segs.push_back(&col->data[r].segments[k]);
+      if (col->data[r].segments[k].row_count) segs.push_back({&col->data[r].segments[k], col->data[r].codec});
   std::stable_sort(segs.begin(), segs.end(),
-                   [](const sdr_segment* a, const sdr_segment* c) { return a->row_offset < c->row_offset; });
+                   [](const auto& a, const auto& c) { return a.first->row_offset < c.first->row_offset; });
   for (size_t i = 1; i < segs.size(); ++i)
-    if (segs[i]->row_offset < segs[i - 1]->row_offset + segs[i - 1]->row_count)
-      return fail(SDR_E_BOUNDS, "string segments overlap at row %u", segs[i]->row_offset);
-  auto add_range = [&](const uint8_t* bytes, uint32_t size, uint32_t col_row, uint32_t rows) {
-    bc.segs.push_back({bytes, size, rows, bc.n_chunks, col_row});
+    if (segs[i].first->row_offset < segs[i - 1].first->row_offset + segs[i - 1].first->row_count)
+      return fail(SDR_E_BOUNDS, "string segments overlap at row %u", segs[i].first->row_offset);
+  auto add_range = [&](const uint8_t* bytes, uint32_t size, uint32_t col_row, uint32_t rows, uint32_t codec) {
+    bc.segs.push_back({bytes, size, rows, bc.n_chunks, col_row, codec, 0});
     bc.n_chunks += fsst_chunk_count(rows);
   };
   uint32_t cursor = 0;
-  for (const sdr_segment* g : segs) {
-    if (g->row_offset > cursor) add_range(nullptr, 0, cursor, g->row_offset - cursor);
-    add_range(g->d_bytes, g->bytes_size, g->row_offset, g->row_count);
+  for (const auto& sc : segs) {
+    const sdr_segment* g = sc.first;
+    if (g->row_offset > cursor) add_range(nullptr, 0, cursor, g->row_offset - cursor, 0);
+    add_range(g->d_bytes, g->bytes_size, g->row_offset, g->row_count, sc.second);
     cursor = g->row_offset + g->row_count;
   }
-  if (cursor < n) add_range(nullptr, 0, cursor, n - cursor);
+  if (cursor < n) add_range(nullptr, 0, cursor, n - cursor, 0);
   bc.item_first = static_cast<uint32_t>(b->items.size());
   const uint32_t ci = static_cast<uint32_t>(b->cols.size());
   for (uint32_t k = 0; k < bc.n_chunks; ++k) {

@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "../../include/sdr/decode.h"
+
 namespace sdr {
 
 constexpr uint32_t FSST_CHUNK_ROWS = 1024;        // rows per workgroup
@@ -13,14 +15,18 @@ constexpr uint32_t FSST_SPIN_LIMIT = 1u << 24;     // look-back wait bound (neve
 // A row range of a column: a segment, or (bytes == nullptr) rows no
 // segment covers, which decode to empty strings.  Its rows are chunks
 // first_chunk, first_chunk + 1, ... of the column's chunk list, which the
-// segment pre-pass writes; the range starts at column row col_row.
+// segment pre-pass writes; the range starts at column row col_row.  codec
+// is the segment's sdr_codec (FSST, DICTIONARY or UNCOMPRESSED; 0 for a gap).
 struct FsstSeg {
   const uint8_t* bytes;
   uint32_t bytes_size;
   uint32_t row_count;
   uint32_t first_chunk;
   uint32_t col_row;
+  uint32_t codec;
+  uint32_t pad;
 };
+static_assert(sizeof(FsstSeg) == 32, "FsstSeg layout");
 
 // Chunks covering `rows` rows.
 inline uint32_t fsst_chunk_count(uint32_t rows) { return (rows + FSST_CHUNK_ROWS - 1u) / FSST_CHUNK_ROWS; }
@@ -41,11 +47,15 @@ constexpr uint32_t FI_WIDTH = 0xFFu;      // bits 0-7: length-stream bit width
 constexpr uint32_t FI_OK    = 1u << 8;    // header valid, symbol table imported
 constexpr uint32_t FI_LFAST = 1u << 9;    // the chunk's length fields lie inside the segment
 constexpr uint32_t FI_HUGE  = 1u << 10;   // a row of the chunk has >= 1 MiB of compressed bytes
+constexpr uint32_t FI_DICT  = 1u << 11;   // a DICTIONARY segment (bits 0-7: selection width)
+constexpr uint32_t FI_UNC   = 1u << 12;   // an UNCOMPRESSED string segment
 
 struct FsstChunkInfo {
   const uint8_t* bytes;  // the segment
   uint32_t c_lo;         // compressed bytes of the segment's rows before the chunk
+                         // (DICTIONARY: index_buffer_offset)
   uint32_t c_hi;         // ... up to and including the chunk's last row
+                         // (DICTIONARY: index_buffer_count)
   uint32_t dict_end;
   uint32_t flags;        // width and FI_* bits
   uint32_t bytes_size;

@@ -1,4 +1,5 @@
-// sdr_strings.cpp — C ABI of the VARCHAR decode path (FSST).
+// sdr_strings.cpp — C ABI of the VARCHAR decode path (FSST, DICTIONARY and
+// UNCOMPRESSED segments, in any mix).
 //
 // Mirrors gpu_decode_strings_column (src/cuda/scan/gpu_decode_strings.cu:
 // 100-402): per-run validation, one column-wide int32 offset space, chars
@@ -20,10 +21,12 @@ int validate_string_column(const sdr_string_column* col)
   if (col->total_rows > 0x7FFFFFFFu) return fail(SDR_E_BOUNDS, "total_rows (%u) > INT32_MAX", col->total_rows);
   int rc = check_runs(col->data, col->n_data, col->total_rows, "data");
   if (rc) return rc;
-  for (uint32_t r = 0; r < col->n_data; ++r)
-    if (col->data[r].codec != SDR_CODEC_FSST)
-      return fail(SDR_E_UNSUPPORTED_CODEC, "viability invariant violated: data codec %u (%s) not implemented",
-                  col->data[r].codec, sdr_codec_name(col->data[r].codec));
+  for (uint32_t r = 0; r < col->n_data; ++r) {
+    const uint32_t k = col->data[r].codec;
+    if (k != SDR_CODEC_FSST && k != SDR_CODEC_DICTIONARY && k != SDR_CODEC_UNCOMPRESSED)
+      return fail(SDR_E_UNSUPPORTED_CODEC, "viability invariant violated: data codec %u (%s) not implemented", k,
+                  sdr_codec_name(k));
+  }
   if (col->has_nulls && col->n_validity) {
     rc = check_validity_runs(col->validity, col->n_validity, col->total_rows);
     if (rc) return rc;
@@ -50,31 +53,32 @@ int decode_strings(sdr_string_column* col, hipStream_t s, uint32_t mode)
   }
 
   // segments in row order; rows they do not cover become empty strings
-  std::vector<const sdr_segment*> segs;
+  std::vector<std::pair<const sdr_segment*, uint32_t>> segs;  // segment, codec
   for (uint32_t r = 0; r < col->n_data; ++r)
     for (uint32_t k = 0; k < col->data[r].n_segments; ++k)
-      if (col->data[r].segments[k].row_count) segs.push_back(&col->data[r].segments[k]);
+      if (col->data[r].segments[k].row_count) segs.push_back({&col->data[r].segments[k], col->data[r].codec});
   std::stable_sort(segs.begin(), segs.end(),
-                   [](const sdr_segment* a, const sdr_segment* b) { return a->row_offset < b->row_offset; });
+                   [](const auto& a, const auto& b) { return a.first->row_offset < b.first->row_offset; });
   for (size_t i = 1; i < segs.size(); ++i)
-    if (segs[i]->row_offset < segs[i - 1]->row_offset + segs[i - 1]->row_count)
-      return fail(SDR_E_BOUNDS, "string segments overlap at row %u", segs[i]->row_offset);
+    if (segs[i].first->row_offset < segs[i - 1].first->row_offset + segs[i - 1].first->row_count)
+      return fail(SDR_E_BOUNDS, "string segments overlap at row %u", segs[i].first->row_offset);
 
   // row ranges: the segments and the gaps between them; the pre-pass
   // writes their chunk lists on the device
   std::vector<FsstSeg> hsegs;
   uint32_t n_chunks = 0;
-  auto add_range = [&](const uint8_t* bytes, uint32_t size, uint32_t col_row, uint32_t rows) {
-    hsegs.push_back({bytes, size, rows, n_chunks, col_row});
+  auto add_range = [&](const uint8_t* bytes, uint32_t size, uint32_t col_row, uint32_t rows, uint32_t codec) {
+    hsegs.push_back({bytes, size, rows, n_chunks, col_row, codec, 0});
     n_chunks += fsst_chunk_count(rows);
   };
   uint32_t cursor = 0;
-  for (const sdr_segment* g : segs) {
-    if (g->row_offset > cursor) add_range(nullptr, 0, cursor, g->row_offset - cursor);
-    add_range(g->d_bytes, g->bytes_size, g->row_offset, g->row_count);
+  for (const auto& sc : segs) {
+    const sdr_segment* g = sc.first;
+    if (g->row_offset > cursor) add_range(nullptr, 0, cursor, g->row_offset - cursor, 0);
+    add_range(g->d_bytes, g->bytes_size, g->row_offset, g->row_count, sc.second);
     cursor = g->row_offset + g->row_count;
   }
-  if (cursor < n) add_range(nullptr, 0, cursor, n - cursor);
+  if (cursor < n) add_range(nullptr, 0, cursor, n - cursor, 0);
   const uint32_t n_ranges = static_cast<uint32_t>(hsegs.size());
 
   auto al = [](size_t v) { return (v + 255u) & ~size_t{255}; };

@@ -46,7 +46,8 @@ def oracle_strings(case):
 
 def test_fixture_inventory():
     codecs = {tuple(c["src"]["codecs"]) for c in CASES}
-    assert ("BitPacking",) in codecs and ("RLE",) in codecs and ("FSST",) in codecs
+    for codec in ("BitPacking", "RLE", "FSST", "Dictionary", "Uncompressed"):
+        assert (codec,) in codecs
     assert any(c["vtable"] is not None for c in CASES)
 
 
@@ -55,7 +56,7 @@ def test_tpch_fixed_matches_parquet(case):
     np.testing.assert_array_equal(oracle_fixed(case), case["expected"])
 
 
-@pytest.mark.parametrize("case", STRINGS, ids=lambda c: c["name"])
+@pytest.mark.parametrize("case", [c for c in STRINGS if c["chars"] is not None], ids=lambda c: c["name"])
 def test_tpch_strings_match_parquet(case):
     offs, chars = oracle_strings(case)
     np.testing.assert_array_equal(offs, case["offsets"])
@@ -75,6 +76,17 @@ def test_tpcds_segments_meet_their_statistics(case):
             assert live.min() == lo and live.max() == hi
             checked += 1
     assert checked == len(case["table"])
+
+
+@pytest.mark.parametrize("case", [c for c in STRINGS if c["chars"] is None], ids=lambda c: c["name"])
+def test_tpcds_strings_meet_their_statistics(case):
+    offs, chars = oracle_strings(case)
+    lens = np.diff(offs)
+    valid = validity_bits(case).astype(bool)
+    assert (lens[~valid] == 0).all()  # NULL rows are stored as empty strings
+    for (o, bs, ro, rc, codec, _), (has, _, max_len) in zip(case["table"], case["stats"]):
+        seg_lens = lens[int(ro):int(ro) + int(rc)]
+        assert has and seg_lens.max() == max_len
 
 
 def test_rle_fixture_has_many_runs():

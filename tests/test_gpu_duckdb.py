@@ -10,7 +10,7 @@ import pytest
 import sirius_amd._abi as A
 from sirius_amd import lib as L
 from tests.helpers import duckdb_cases, duckdb_segments, fixed_column, string_column, validity_bits
-from tests.test_duckdb_segments import oracle_fixed
+from tests.test_duckdb_segments import oracle_fixed, oracle_strings
 
 pytestmark = pytest.mark.gpu
 
@@ -65,17 +65,31 @@ def test_duckdb_fixed_columns(case, mode):
 
 
 @pytest.mark.parametrize("case", STRINGS, ids=lambda c: c["name"])
-@pytest.mark.parametrize("mode", MODES, ids=["batched", "plain"])
+@pytest.mark.parametrize("mode", MODES + [A.LAUNCH_PERSISTENT], ids=["batched", "plain", "persistent"])
 def test_duckdb_string_columns(case, mode):
     n = case["total"]
+    if case["chars"] is not None:
+        want_offs, want_chars = case["offsets"], case["chars"]
+    else:
+        want_offs, want_chars = oracle_strings(case)
     d, base = upload(case["blob"])
-    cap = len(case["chars"]) + 64
+    cap = len(want_chars) + 64
     offs = torch.full((n + 1,), -7, dtype=torch.int32, device=_dev())
     chars = torch.full((cap,), 0xCC, dtype=torch.uint8, device=_dev())
-    col, keep = string_column(duckdb_segments(case["table"], base), n, offs.data_ptr(), chars.data_ptr(), cap)
+    validity, mask, keep_v = None, None, None
+    if case["vtable"] is not None:
+        keep_v, vbase = upload(case["vblob"])
+        validity = duckdb_segments(case["vtable"], vbase)
+        mask = torch.zeros((n + 7) // 8, dtype=torch.uint8, device=_dev())
+    col, keep = string_column(duckdb_segments(case["table"], base), n, offs.data_ptr(), chars.data_ptr(), cap,
+                              validity, mask.data_ptr() if mask is not None else None)
     rc = L.load().sdr_decode_strings_column_mode(C.byref(col), torch.cuda.current_stream().cuda_stream, mode)
     torch.cuda.synchronize()
     L.check(rc)
-    np.testing.assert_array_equal(offs.cpu().numpy(), case["offsets"])
-    assert col.chars_used == len(case["chars"])
-    np.testing.assert_array_equal(chars.cpu().numpy()[:col.chars_used], case["chars"])
+    np.testing.assert_array_equal(offs.cpu().numpy(), want_offs)
+    assert col.chars_used == len(want_chars)
+    np.testing.assert_array_equal(chars.cpu().numpy()[:col.chars_used], want_chars)
+    if mask is not None:
+        valid = validity_bits(case)
+        assert col.null_count == int((valid == 0).sum())
+        np.testing.assert_array_equal(np.unpackbits(mask.cpu().numpy(), bitorder="little")[:n], valid)

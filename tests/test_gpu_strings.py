@@ -135,7 +135,7 @@ def test_capacity_overflow_reports_total():
 def test_unsupported_string_codec():
     chars, offs, enc = _zipf(100, 4)
     d, base = upload(enc.blob)
-    rc, col, _, _ = decode_strings_gpu([(A.DICTIONARY, enc.segments(base))], enc.n_rows, 10_000)
+    rc, col, _, _ = decode_strings_gpu([(A.DICT_FSST, enc.segments(base))], enc.n_rows, 10_000)
     assert rc == A.SDR_E_UNSUPPORTED_CODEC
 
 
