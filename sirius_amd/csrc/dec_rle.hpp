@@ -38,14 +38,15 @@ constexpr uint32_t RLE_MAX_SUPERS  = (RLE_MAX_ENTRIES + RLE_SUPER - 1) / RLE_SUP
 constexpr uint32_t RLE_STAGE_WORDS = (RLE_SUPER * 2 + 48) / 4;
 static_assert(RLE_SUPER % RLE_TILE == 0, "passes hold whole tiles");
 
-// Per-wave bitmap of run ends over a window's rows: 64 rows of front pad,
-// RLE_BM_ROWS window rows and 128 rows a pair of steps may overrun.
-constexpr uint32_t RLE_BM_ROWS  = 4096;
-constexpr uint32_t RLE_BM_WORDS = (RLE_BM_ROWS + 192u) / 32u + 2u;
+// Per-wave bitmap of run ends over a window's 64-row steps, one 64-bit word
+// per step: RLE_BM_ROWS window rows, the up to 63 rows the first step starts
+// before the window, and the step a pair may overrun.
+constexpr uint32_t RLE_BM_ROWS   = 4096;
+constexpr uint32_t RLE_BM_QWORDS = (RLE_BM_ROWS + 63u) / 64u + 3u;
 
 struct RleSmem {
   uint32_t stage[RLE_STAGE_WORDS];         // counts of the resident pass
-  uint32_t bm[RLE_WAVES][RLE_BM_WORDS];    // run-end bitmap of each wave's window
+  uint64_t bm[RLE_WAVES][RLE_BM_QWORDS];   // run-end bitmap of each wave's window
   uint32_t chunk_base[RLE_THREADS];        // row base of each 32-entry chunk
   uint32_t wave_sum[RLE_WAVES];
   uint32_t pass_base[RLE_MAX_SUPERS + 1];  // row base of each pass
@@ -160,7 +161,7 @@ __device__ __forceinline__ void rle_expand(const uint8_t* values, uint32_t t0, u
                                            uint32_t head, uint32_t lo, uint32_t hi, uint8_t* out, RleSmem& sm)
 {
   const uint32_t lane = lane_id();
-  uint32_t* bm        = sm.bm[threadIdx.x / WAVE];
+  uint64_t* bm        = sm.bm[threadIdx.x / WAVE];
   // chunk holding the first run that ends after row lo: the last chunk with
   // entries whose base is <= lo (bases never decrease up to the match)
   uint32_t cnt = 0;
@@ -210,27 +211,26 @@ __device__ __forceinline__ void rle_expand(const uint8_t* values, uint32_t t0, u
     const uint32_t a    = wstart > lo ? wstart : lo;
     const uint32_t b    = wend < hi ? wend : hi;
     // A window of at most RLE_BM_ROWS rows marks its run ends in the wave's
-    // bitmap once (bit 64 + end - wstart); a step then reads its 64 bits
-    // instead of collecting the ends one readlane at a time.
+    // bitmap once; a step then reads its 64 bits instead of collecting the
+    // ends one readlane at a time.
     const bool use_bm = !dup && wend - wstart <= RLE_BM_ROWS && a < b;
     const uint32_t first = lo + ((a - lo) & ~(WAVE - 1u));
     if (use_bm) {
-      const uint32_t nw = (wend - wstart + 192u) / 32u + 2u;
-      for (uint32_t i = lane; i < nw; i += WAVE) bm[i] = 0u;
+      // word k holds the ends in rows [first + 64 k, first + 64 k + 64):
+      // bit l of a step's word is an end at row r0 + l, so a step reads one
+      // aligned word.  Ends before `first` are in every step's ballot.
+      const uint32_t nq = (b - first + 63u) / 64u + 2u;
+      for (uint32_t i = lane; i < nq; i += WAVE) bm[i] = 0ull;
       wave_lds_sync();
-      if (live) {
-        const uint32_t bit = 64u + (end - wstart);
-        __hip_atomic_fetch_or(&bm[bit >> 5], 1u << (bit & 31u), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+      if (live && end >= first && end - first < 64u * nq) {
+        const uint32_t bit = end - first;
+        __hip_atomic_fetch_or(reinterpret_cast<uint32_t*>(bm) + (bit >> 5), 1u << (bit & 31u), __ATOMIC_RELAXED,
+                              __HIP_MEMORY_SCOPE_WAVEFRONT);
       }
       wave_lds_sync();
-      // two independent 64-row steps per iteration, so their LDS round trips
-      // overlap; r0 >= wstart - 63, so every bitmap index is >= 1
+      // two independent 64-row steps per iteration, so their LDS round trips overlap
       auto rank_at = [&](uint32_t r0) {
-        const uint32_t b0 = 64u + r0 - wstart;
-        const uint32_t q  = b0 >> 5, sh = b0 & 31u;
-        const uint32_t w0 = bm[q], w1 = bm[q + 1], w2 = bm[q + 2];
-        const uint64_t m  = (static_cast<uint64_t>(__builtin_amdgcn_alignbit(w2, w1, sh)) << 32) |
-                            __builtin_amdgcn_alignbit(w1, w0, sh);
+        const uint64_t m = bm[(r0 - first) >> 6];
         return static_cast<uint32_t>(__builtin_popcountll(ballot(live && end <= r0))) +
                static_cast<uint32_t>(__builtin_popcountll(m & upto & ~1ull));
       };
