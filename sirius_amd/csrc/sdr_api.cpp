@@ -127,6 +127,7 @@ struct DescSlot {
   void* d         = nullptr;
   size_t cap      = 0;
   hipEvent_t done = nullptr;
+  hipEvent_t up   = nullptr;  // recorded after the slot's upload on the copy stream
   int dev         = -1;
   bool held       = false;  // acquired and not yet released
   uint64_t seq    = 0;      // release order
@@ -136,6 +137,20 @@ std::vector<DescSlot*> g_desc;
 uint64_t g_desc_seq = 0;
 constexpr size_t DESC_SLOTS_PER_DEV = 32;  // about 8 ms of headline steps queued ahead of the host
 constexpr size_t DESC_MIN_BYTES     = 64u << 10;
+
+// One copy stream per device (default priority, not synchronising with the
+// null stream) carries descriptor uploads, so a call's upload runs while
+// the caller's stream still works on earlier calls; the caller's stream
+// then waits only for that copy's event.  Guarded by g_desc_mu.
+hipStream_t g_copy_stream[64] = {};
+
+int copy_stream(int dev, hipStream_t* out)
+{
+  if (dev < 0 || dev >= 64) return fail(SDR_E_HIP, "device %d out of range", dev);
+  if (!g_copy_stream[dev]) HIP_TRY(hipStreamCreateWithFlags(&g_copy_stream[dev], hipStreamNonBlocking));
+  *out = g_copy_stream[dev];
+  return SDR_OK;
+}
 }  // namespace
 
 int desc_acquire(const void* h_src, size_t bytes, hipStream_t s, void** d_out, void** token)
@@ -161,7 +176,9 @@ int desc_acquire(const void* h_src, size_t bytes, hipStream_t s, void** d_out, v
   if (!pick && (n_dev < DESC_SLOTS_PER_DEV || !oldest)) {
     auto* n = new DescSlot;
     n->dev  = dev;
-    if (hipEventCreateWithFlags(&n->done, hipEventDisableTiming) != hipSuccess) {
+    if (hipEventCreateWithFlags(&n->done, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&n->up, hipEventDisableTiming) != hipSuccess) {
+      if (n->done) (void)hipEventDestroy(n->done);
       delete n;
       return fail(SDR_E_HIP, "descriptor slot event creation failed");
     }
@@ -184,7 +201,20 @@ int desc_acquire(const void* h_src, size_t bytes, hipStream_t s, void** d_out, v
     pick->cap = cap;
   }
   std::memcpy(pick->h, h_src, bytes);
-  HIP_TRY(hipMemcpyAsync(pick->d, pick->h, bytes, hipMemcpyHostToDevice, s));
+  // Under stream capture the copy stays on the caller's stream, as a node
+  // of the caller's graph.
+  hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+  HIP_TRY(hipStreamIsCapturing(s, &cap));
+  if (cap == hipStreamCaptureStatusNone) {
+    hipStream_t cs = nullptr;
+    int rc = copy_stream(dev, &cs);
+    if (rc) return rc;
+    HIP_TRY(hipMemcpyAsync(pick->d, pick->h, bytes, hipMemcpyHostToDevice, cs));
+    HIP_TRY(hipEventRecord(pick->up, cs));
+    HIP_TRY(hipStreamWaitEvent(s, pick->up, 0));
+  } else {
+    HIP_TRY(hipMemcpyAsync(pick->d, pick->h, bytes, hipMemcpyHostToDevice, s));
+  }
   pick->held = true;
   *d_out     = pick->d;
   *token     = pick;
