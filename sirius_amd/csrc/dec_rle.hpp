@@ -28,6 +28,8 @@
 
 namespace sdr {
 
+constexpr int ICMP_ULE = 37;  // LLVM's unsigned <= predicate, for __builtin_amdgcn_uicmp
+
 constexpr uint32_t RLE_TILE        = 2048;
 constexpr uint32_t RLE_MAX_ENTRIES = 90112;
 constexpr uint32_t RLE_THREADS     = 256;
@@ -59,14 +61,29 @@ __device__ __forceinline__ uint32_t lds_u16(const uint32_t* lds, uint32_t byte)
   return __builtin_amdgcn_alignbit(lds[w + 1], lds[w], (byte & 3u) * 8u) & 0xFFFFu;
 }
 
+// Inclusive prefix sum within each 16-lane row of the wave.
+__device__ __forceinline__ uint32_t row_incl_sum_u32(uint32_t v)
+{
+  v += dpp_u32<0x111>(v);  // row_shr:1
+  v += dpp_u32<0x112>(v);  // row_shr:2
+  v += dpp_u32<0x114>(v);  // row_shr:4
+  v += dpp_u32<0x118>(v);  // row_shr:8
+  return v;
+}
+
 // Stage pass `t0` (n entries) and compute every chunk's row base from
 // `base`.  With `detect`, also record the first exact match and the first
 // zero/overshoot entry of the pass in sm.match / sm.bad (UINT32_MAX: none).
 // Returns the pass's count total.  Called by all threads of the block.
+//
+// Wave w reads words [1024 w, 1024 w + 1024) of the staged counts (two
+// counts a word), 64 consecutive words per step, so the LDS reads are
+// conflict-free; a 16-lane row of a step is one 32-entry chunk, summed by a
+// DPP row scan.
 __device__ __forceinline__ uint32_t rle_pass(const uint8_t* counts, uint32_t t0, uint32_t n, uint32_t base,
                                              uint32_t rows, bool detect, uint32_t& head, RleSmem& sm)
 {
-  __syncthreads();  // earlier readers of sm.stage are done
+  __syncthreads();  // earlier readers of sm.stage and sm.chunk_base are done
   if (detect && threadIdx.x == 0) {
     sm.match = UINT32_MAX;
     sm.bad   = UINT32_MAX;
@@ -74,28 +91,27 @@ __device__ __forceinline__ uint32_t rle_pass(const uint8_t* counts, uint32_t t0,
   head = block_stage<RLE_THREADS>(sm.stage, counts + 2u * t0, 2u * n);
   __syncthreads();
 
-  // Thread t owns entries [32 t, 32 t + 32) of the pass: 16 words of two
-  // counts each, re-read from LDS (re-aligned by the uniform byte skew) so
-  // the counts never all sit in registers at once.
-  const uint32_t t     = threadIdx.x;
-  const uint32_t q     = RLE_PER_THREAD / 2u * t + (head >> 2);
-  const uint32_t sh    = (head & 3u) * 8u;
-  const uint32_t e0    = RLE_PER_THREAD * t;
-  const uint32_t nlive = e0 >= n ? 0u : (n - e0 < RLE_PER_THREAD ? n - e0 : RLE_PER_THREAD);
-  // word k of the chunk, counts past the pass reading as zero
-  auto word = [&](uint32_t k) {
-    const uint32_t w = __builtin_amdgcn_alignbit(sm.stage[q + k + 1u], sm.stage[q + k], sh);
-    return w & ((2u * k + 1u < nlive ? 0xFFFF0000u : 0u) | (2u * k < nlive ? 0xFFFFu : 0u));
+  const uint32_t t    = threadIdx.x;
+  const uint32_t lane = lane_id();
+  const uint32_t wv   = t / WAVE;
+  const uint32_t row  = lane >> 4;
+  const uint32_t hq   = head >> 2;
+  const uint32_t sh   = (head & 3u) * 8u;
+  // word i of the pass (counts 2i and 2i + 1), counts past the pass reading as zero
+  auto word = [&](uint32_t i) {
+    const uint32_t w = __builtin_amdgcn_alignbit(sm.stage[hq + i + 1u], sm.stage[hq + i], sh);
+    return w & ((2u * i + 1u < n ? 0xFFFF0000u : 0u) | (2u * i < n ? 0xFFFFu : 0u));
   };
-  uint32_t s = 0;
 #pragma unroll 4
   for (uint32_t k = 0; k < RLE_PER_THREAD / 2; ++k) {
-    const uint32_t w = word(k);
-    s += (w & 0xFFFFu) + (w >> 16);
+    const uint32_t w  = word(1024u * wv + 64u * k + lane);
+    const uint32_t rs = row_incl_sum_u32((w & 0xFFFFu) + (w >> 16));
+    if ((lane & 15u) == 15u) sm.chunk_base[64u * wv + 4u * k + row] = rs;  // the chunk's sum
   }
+  __syncthreads();
+  const uint32_t s    = sm.chunk_base[t];  // chunk t's sum
   const uint32_t incl = wave_incl_sum_u32(s);
-  const uint32_t wv   = t / WAVE;
-  if (lane_id() == WAVE - 1) sm.wave_sum[wv] = incl;
+  if (lane == WAVE - 1) sm.wave_sum[wv] = incl;
   __syncthreads();
   uint32_t woff = 0, total = 0;
 #pragma unroll
@@ -104,31 +120,27 @@ __device__ __forceinline__ uint32_t rle_pass(const uint8_t* counts, uint32_t t0,
     woff += k < wv ? ws : 0u;
     total += ws;
   }
-  const uint32_t cbase = base + woff + incl - s;
-  sm.chunk_base[t]     = cbase;
+  sm.chunk_base[t] = base + woff + incl - s;  // only thread t reads or writes entry t here
   if (detect) {
-    // bit i: entry e0 + i matches / is a zero or an overshoot
-    uint32_t run = cbase, hits = 0, bads = 0;
+    __syncthreads();  // chunk bases visible
+    uint32_t m = UINT32_MAX, bad = UINT32_MAX;  // this lane's first entries (they rise with k)
 #pragma unroll 4
     for (uint32_t k = 0; k < RLE_PER_THREAD / 2; ++k) {
-      const uint32_t w  = word(k);
+      const uint32_t i  = 1024u * wv + 64u * k + lane;
+      const uint32_t w  = word(i);
       const uint32_t c0 = w & 0xFFFFu, c1 = w >> 16;
-      run += c0;
-      hits |= (run == rows ? 1u : 0u) << (2u * k);
-      bads |= (((c0 == 0u) | (run > rows)) ? 1u : 0u) << (2u * k);
-      run += c1;
-      hits |= (run == rows ? 1u : 0u) << (2u * k + 1u);
-      bads |= (((c1 == 0u) | (run > rows)) ? 1u : 0u) << (2u * k + 1u);
+      const uint32_t rs = row_incl_sum_u32(c0 + c1);
+      const uint32_t r0 = sm.chunk_base[64u * wv + 4u * k + row] + rs - (c0 + c1) + c0;
+      const uint32_t r1 = r0 + c1;
+      const bool l0 = 2u * i < n, l1 = 2u * i + 1u < n;
+      const uint32_t e  = t0 + 2u * i;
+      const uint32_t hm = (l0 && r0 == rows) ? e : ((l1 && r1 == rows) ? e + 1u : UINT32_MAX);
+      const uint32_t hb = (l0 && (c0 == 0u || r0 > rows)) ? e : ((l1 && (c1 == 0u || r1 > rows)) ? e + 1u : UINT32_MAX);
+      m   = m < hm ? m : hm;
+      bad = bad < hb ? bad : hb;
     }
-    const uint32_t live = nlive >= RLE_PER_THREAD ? ~0u : (1u << nlive) - 1u;
-    hits &= live;
-    bads &= live;
-    const uint32_t m   = hits ? t0 + e0 + static_cast<uint32_t>(__builtin_ctz(hits)) : UINT32_MAX;
-    const uint32_t bad = bads ? t0 + e0 + static_cast<uint32_t>(__builtin_ctz(bads)) : UINT32_MAX;
-    // entries are in thread order, so the first lane holding one is the minimum
-    const uint64_t bm = ballot(m != UINT32_MAX), bb = ballot(bad != UINT32_MAX);
-    if (bm && lane_id() == static_cast<uint32_t>(__builtin_ctzll(bm))) atomicMin(&sm.match, m);
-    if (bb && lane_id() == static_cast<uint32_t>(__builtin_ctzll(bb))) atomicMin(&sm.bad, bad);
+    if (ballot(m != UINT32_MAX) != 0ull && m != UINT32_MAX) atomicMin(&sm.match, m);
+    if (ballot(bad != UINT32_MAX) != 0ull && bad != UINT32_MAX) atomicMin(&sm.bad, bad);
   }
   __syncthreads();
   return total;
@@ -206,15 +218,17 @@ __device__ __forceinline__ void rle_expand(const uint8_t* values, uint32_t t0, u
     if (j + WAVE < j_end) fetch(j + WAVE, cnt_next, val_next);
     const uint32_t incl = wave_incl_sum_u32(cnt1);
     const uint32_t end  = wstart + incl;  // exclusive end row of run jl
-    const uint32_t wend = wstart + __builtin_amdgcn_readlane(incl, WAVE - 1);
+    // wave-uniform values are pinned to scalar registers (readfirstlane), so
+    // the step loop's bounds and branches are scalar work
+    const uint32_t wend = __builtin_amdgcn_readfirstlane(wstart + __builtin_amdgcn_readlane(incl, WAVE - 1));
     const bool dup      = ballot(live && cnt1 == 0u) != 0;
-    const uint32_t a    = wstart > lo ? wstart : lo;
-    const uint32_t b    = wend < hi ? wend : hi;
+    const uint32_t a    = __builtin_amdgcn_readfirstlane(wstart > lo ? wstart : lo);
+    const uint32_t b    = __builtin_amdgcn_readfirstlane(wend < hi ? wend : hi);
     // A window of at most RLE_BM_ROWS rows marks its run ends in the wave's
     // bitmap once; a step then reads its 64 bits instead of collecting the
     // ends one readlane at a time.
     const bool use_bm = !dup && wend - wstart <= RLE_BM_ROWS && a < b;
-    const uint32_t first = lo + ((a - lo) & ~(WAVE - 1u));
+    const uint32_t first = __builtin_amdgcn_readfirstlane(lo + ((a - lo) & ~(WAVE - 1u)));
     if (use_bm) {
       // word k holds the ends in rows [first + 64 k, first + 64 k + 64):
       // bit l of a step's word is an end at row r0 + l, so a step reads one
@@ -228,24 +242,36 @@ __device__ __forceinline__ void rle_expand(const uint8_t* values, uint32_t t0, u
                               __HIP_MEMORY_SCOPE_WAVEFRONT);
       }
       wave_lds_sync();
-      // two independent 64-row steps per iteration, so their LDS round trips overlap
+      // two independent 64-row steps per iteration, so their LDS round trips
+      // overlap; dead lanes' ends read as UINT32_MAX, so one unsigned compare
+      // is the ballot of ends at or before r0
+      const uint32_t end_cmp = live ? end : UINT32_MAX;
       auto rank_at = [&](uint32_t r0) {
         const uint64_t m = bm[(r0 - first) >> 6];
-        return static_cast<uint32_t>(__builtin_popcountll(ballot(live && end <= r0))) +
+        return static_cast<uint32_t>(__builtin_popcountll(__builtin_amdgcn_uicmp(end_cmp, r0, ICMP_ULE))) +
                static_cast<uint32_t>(__builtin_popcountll(m & upto & ~1ull));
       };
       for (uint32_t r0 = first; r0 < b; r0 += 2u * WAVE) {
-        const uint32_t rk0 = rank_at(r0), rk1 = rank_at(r0 + WAVE);
-        const int s0 = static_cast<int>((rk0 < WAVE ? rk0 : WAVE - 1u) << 2);
-        const int s1 = static_cast<int>((rk1 < WAVE ? rk1 : WAVE - 1u) << 2);
+        // A stored row (< b <= wend) has rank <= 63; larger ranks belong to
+        // rows past the window, which are not stored.  ds_bpermute reads
+        // lane (index / 4) mod 64, so those need no clamp.
+        const int s0 = static_cast<int>(rank_at(r0) << 2);
+        const int s1 = static_cast<int>(rank_at(r0 + WAVE) << 2);
         const uint32_t vl = static_cast<uint32_t>(val), vh = static_cast<uint32_t>(val >> 32);
         const uint32_t l0 = __builtin_amdgcn_ds_bpermute(s0, static_cast<int>(vl));
         const uint32_t l1 = __builtin_amdgcn_ds_bpermute(s1, static_cast<int>(vl));
         const uint32_t h0 = TS == 8u ? __builtin_amdgcn_ds_bpermute(s0, static_cast<int>(vh)) : 0u;
         const uint32_t h1 = TS == 8u ? __builtin_amdgcn_ds_bpermute(s1, static_cast<int>(vh)) : 0u;
         const uint32_t row0 = r0 + lane, row1 = r0 + WAVE + lane;
-        if (row0 >= a && row0 < b) st_width<TS>(out, row0, static_cast<uint64_t>(l0) | (static_cast<uint64_t>(h0) << 32));
-        if (row1 >= a && row1 < b) st_width<TS>(out, row1, static_cast<uint64_t>(l1) | (static_cast<uint64_t>(h1) << 32));
+        const uint64_t v0 = static_cast<uint64_t>(l0) | (static_cast<uint64_t>(h0) << 32);
+        const uint64_t v1 = static_cast<uint64_t>(l1) | (static_cast<uint64_t>(h1) << 32);
+        if (r0 >= a && r0 + 2u * WAVE <= b) {  // uniform: both steps inside [a, b)
+          st_width<TS>(out, row0, v0);
+          st_width<TS>(out, row1, v1);
+        } else {
+          if (row0 >= a && row0 < b) st_width<TS>(out, row0, v0);
+          if (row1 >= a && row1 < b) st_width<TS>(out, row1, v1);
+        }
       }
       j += WAVE;
       wstart = wend;
