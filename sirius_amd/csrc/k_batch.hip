@@ -42,7 +42,8 @@ __device__ __forceinline__ void fixed_units(const BatchItem& it, uint32_t w, Bat
   }
 }
 
-__global__ __launch_bounds__(BLOCK_THREADS) void k_batch(const BatchItem* __restrict__ items, BatchQueue* __restrict__ q)
+__global__ __launch_bounds__(BLOCK_THREADS) __attribute__((amdgpu_waves_per_eu(4))) void k_batch(const BatchItem* __restrict__ items,
+                                                                                          BatchQueue* __restrict__ q)
 {
   __shared__ BatchSmem sm;
   __shared__ uint32_t s_idx;

@@ -139,9 +139,10 @@ constexpr size_t DESC_SLOTS_PER_DEV = 32;  // about 8 ms of headline steps queue
 constexpr size_t DESC_MIN_BYTES     = 64u << 10;
 
 // One copy stream per device (default priority, not synchronising with the
-// null stream) carries descriptor uploads, so a call's upload runs while
-// the caller's stream still works on earlier calls; the caller's stream
-// then waits only for that copy's event.  Guarded by g_desc_mu.
+// null stream) carries descriptor uploads while the caller's stream is busy,
+// so a call's upload runs while that stream still works on earlier calls;
+// the caller's stream then waits only for that copy's event.  Guarded by
+// g_desc_mu.
 hipStream_t g_copy_stream[64] = {};
 
 int copy_stream(int dev, hipStream_t* out)
@@ -201,11 +202,13 @@ int desc_acquire(const void* h_src, size_t bytes, hipStream_t s, void** d_out, v
     pick->cap = cap;
   }
   std::memcpy(pick->h, h_src, bytes);
-  // Under stream capture the copy stays on the caller's stream, as a node
-  // of the caller's graph.
+  // The copy goes on the copy stream only while the caller's stream still
+  // has work queued: on an idle stream the cross-stream wait would only add
+  // latency.  Under stream capture it stays on the caller's stream, as a
+  // node of the caller's graph.
   hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
   HIP_TRY(hipStreamIsCapturing(s, &cap));
-  if (cap == hipStreamCaptureStatusNone) {
+  if (cap == hipStreamCaptureStatusNone && hipStreamQuery(s) == hipErrorNotReady) {
     hipStream_t cs = nullptr;
     int rc = copy_stream(dev, &cs);
     if (rc) return rc;
