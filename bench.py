@@ -298,7 +298,8 @@ def main() -> int:
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported: the decode path runs on a "
-              f"single MI355X here (how it shards across GPUs is described in DESIGN.md)", file=sys.stderr)
+              f"single MI355X. Across GPUs it shards by placement (one process per GPU decoding its own "
+              f"segments; decode never communicates)", file=sys.stderr)
         return 2
 
     import torch
