@@ -365,6 +365,30 @@ int sdr_decode_strings_column(sdr_string_column* col, sdr_stream_t stream);
 int sdr_decode_strings_column_mode(sdr_string_column* col, sdr_stream_t stream,
                                    uint32_t launch_mode);
 
+/* ---- staging segment bytes from a file ----------------------------------- */
+
+/* One byte range of a file (a segment).  device_offset is an output: the
+ * range's position in the staged device image. */
+typedef struct sdr_extent {
+  uint64_t file_offset;
+  uint64_t bytes;
+  uint64_t device_offset;
+} sdr_extent;
+
+/* Lay out `n` extents: sorted by file offset, ranges whose gap is at most
+ * `max_gap` bytes share a piece, pieces are packed 16-byte aligned.  Fills
+ * every device_offset and returns the image size in bytes.  Host only. */
+uint64_t sdr_stage_plan(sdr_extent* extents, uint32_t n, uint64_t max_gap);
+
+/* Read the extents of the file at `path` into pinned host memory (one read
+ * per piece) and copy the image to `d_dst` (dst_capacity bytes) with one
+ * host-to-device copy on `stream`; device_offset as sdr_stage_plan.  The
+ * reads complete before return; the copy is asynchronous.  Replaces the
+ * reference's submit_and_await (src/op/scan/duckdb_native_decoder.cpp:
+ * 675-855), which copies each segment separately. */
+int sdr_stage_file(const char* path, sdr_extent* extents, uint32_t n, uint64_t max_gap, void* d_dst,
+                   uint64_t dst_capacity, sdr_stream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -53,6 +53,15 @@ TUNE_NAMES = {TUNE_RLE_SLICE_ROWS: "rle_slice_rows", TUNE_BP_GROUPS_PER_WAVE: "b
               TUNE_BP_CHUNK_ROWS: "bp_chunk_rows", TUNE_ALP_VECTORS_PER_WAVE: "alp_vectors_per_wave"}
 
 
+class Extent(C.Structure):
+    """sdr_extent: a byte range of a file and its position in the staged device image."""
+    _fields_ = [
+        ("file_offset", C.c_uint64),
+        ("bytes", C.c_uint64),
+        ("device_offset", C.c_uint64),
+    ]
+
+
 class Segment(C.Structure):
     _fields_ = [
         ("d_bytes", C.c_void_p),

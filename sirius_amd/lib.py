@@ -26,6 +26,7 @@ DECLARED = [
     "sdr_server_create", "sdr_server_stage", "sdr_server_unstage", "sdr_server_submit", "sdr_server_progress",
     "sdr_server_poll", "sdr_server_release", "sdr_server_pending", "sdr_server_held_bytes", "sdr_server_now_ms",
     "sdr_server_destroy",
+    "sdr_stage_plan", "sdr_stage_file",
 ]
 
 _lib = None
@@ -43,6 +44,13 @@ def load() -> C.CDLL:
         return _lib
     if not LIB_PATH.exists():
         raise RuntimeError(f"HIP decode library not built: {LIB_PATH} (run python -m sirius_amd.build)")
+    # One HIP runtime per process: torch ships its own, and when torch loads
+    # after this library the process ends up with two, the second of which
+    # finds no device.  Loading torch first makes the library bind to it.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = C.CDLL(str(LIB_PATH))
     vp, u32, i32 = C.c_void_p, C.c_uint32, C.c_int
     P = C.POINTER
@@ -108,6 +116,10 @@ def load() -> C.CDLL:
     lib.sdr_tune_get.restype = C.c_uint64
     lib.sdr_tune_load.argtypes = [C.c_char_p]
     lib.sdr_tune_load.restype = i32
+    lib.sdr_stage_plan.argtypes = [P(_abi.Extent), u32, C.c_uint64]
+    lib.sdr_stage_plan.restype = C.c_uint64
+    lib.sdr_stage_file.argtypes = [C.c_char_p, P(_abi.Extent), u32, C.c_uint64, vp, C.c_uint64, vp]
+    lib.sdr_stage_file.restype = i32
     lib.sdr_batch_create.argtypes = [P(vp)]
     lib.sdr_batch_set_option.argtypes = [vp, u32, C.c_int64]
     lib.sdr_batch_add_table.argtypes = [vp, P(_abi.Column), u32]
