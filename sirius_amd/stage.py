@@ -15,7 +15,6 @@ from __future__ import annotations
 import ctypes as C
 import struct
 
-import numpy as np
 
 from . import _abi as A
 from . import lib as L
@@ -122,7 +121,12 @@ def scan_column(f: DuckDBFile, table: str, column: str, device, stream=None, max
         sc.d_offsets, sc.d_chars, sc.chars_capacity = offsets.data_ptr(), chars.data_ptr(), int(cap)
         if validity:
             sc.has_nulls, sc.validity, sc.n_validity, sc.d_null_mask = 1, vr, len(validity), mask.data_ptr()
-        L.check(lib.sdr_decode_strings_column(C.byref(sc), sh))
+        rc = lib.sdr_decode_strings_column(C.byref(sc), sh)
+        if rc == A.SDR_E_OVERFLOW and sc.chars_used <= 0x7FFFFFFF:  # the estimate was short: exact size
+            chars = torch.empty(max(1, int(sc.chars_used)), dtype=torch.uint8, device=device)
+            sc.d_chars, sc.chars_capacity = chars.data_ptr(), int(sc.chars_used)
+            rc = lib.sdr_decode_strings_column(C.byref(sc), sh)
+        L.check(rc)
         out.update(offsets=offsets, chars=chars[:sc.chars_used], null_count=sc.null_count)
     else:
         values = torch.empty(max(1, n * ts), dtype=torch.uint8, device=device)
