@@ -142,3 +142,63 @@ def test_reader_rejects_other_files(tmp_path):
     q.write_bytes(bytes(head))
     with pytest.raises(FormatError):
         DuckDBFile(str(q))
+
+
+# ---------------------------------------------------------------------------
+# a database file written around the fixture segments (tests/duckdb_writer.py)
+# ---------------------------------------------------------------------------
+def fixture_tables(cases):
+    """One table per fixture column, its segments and validity as the fixture holds them."""
+    tables = []
+    for c in cases:
+        src = c["src"]
+        ty = {"BIGINT": 14, "INTEGER": 13, "DATE": 15, "VARCHAR": 25}.get(src["type"].split("(")[0], 21)
+        width, scale = 0, 0
+        if ty == 21:
+            width, scale = (int(x) for x in src["type"][8:-1].split(","))
+        segs = []
+        for (o, bs, ro, rc, codec, msl), (has, lo, hi) in zip(c["table"], c["stats"]):
+            s = {"row_start": int(ro), "rows": int(rc), "codec": int(codec),
+                 "bytes": c["blob"][int(o):int(o) + int(bs)].tobytes()}
+            if ty == 25:
+                s["max_string_length"] = int(hi)
+            elif has:
+                s["min"], s["max"] = int(lo), int(hi)
+            segs.append(s)
+        val = []
+        if c["vtable"] is not None:
+            for o, bs, ro, rc, codec, _ in c["vtable"]:
+                val.append({"row_start": int(ro), "rows": int(rc), "codec": 1, "has_null": True,
+                            "bytes": c["vblob"][int(o):int(o) + int(bs)].tobytes()})
+        tables.append({"name": c["name"], "rows": c["total"],
+                       "columns": [{"name": src["column"], "type_id": ty, "width": width, "scale": scale,
+                                    "segments": segs, "validity": val}]})
+    # a CONSTANT column: its value lives in the statistics only
+    tables.append({"name": "constants", "rows": 5000,
+                   "columns": [{"name": "k", "type_id": 14, "segments": [
+                       {"row_start": 0, "rows": 3000, "codec": 2, "min": -77, "max": -77},
+                       {"row_start": 3000, "rows": 2000, "codec": 2, "min": 123456789012, "max": 123456789012}]}]})
+    return tables
+
+
+def test_reader_on_a_written_file(tmp_path):
+    from tests.duckdb_writer import write_database
+    path = write_database(tmp_path / "fixtures.duckdb", fixture_tables(CASES))
+    with DuckDBFile(str(path)) as f:
+        assert f.library_version == "v1.4.3"
+        assert set(f.tables) == {c["name"] for c in CASES} | {"constants"}
+        for c in CASES:
+            t = f.tables[c["name"]]
+            assert t.rows == c["total"] and len(t.columns) == 1
+            col = t.columns[0]
+            assert col.name == c["src"]["column"]
+            assert col.type.is_string == c["string"]
+            if not c["string"]:
+                assert col.type.type_size == c["ts"]
+            assert len(col.segments) == len(c["table"])
+            for s, (o, bs, ro, rc, codec, _) in zip(col.segments, c["table"]):
+                assert (s.row_start, s.tuple_count, s.codec) == (ro, rc, codec)
+                assert f.segment_bytes(s)[:int(bs)] == c["blob"][int(o):int(o) + int(bs)].tobytes()
+            assert len([v for v in col.validity if v.codec == 1]) == (0 if c["vtable"] is None else len(c["vtable"]))
+        k = f.tables["constants"].columns[0]
+        assert [(s.codec, s.stats["min"]) for s in k.segments] == [(2, -77), (2, 123456789012)]

@@ -1,0 +1,55 @@
+"""scan_column end to end: a database file written around the DuckDB-written
+fixture segments (tests/duckdb_writer.py) is read by the file reader, staged
+with sdr_stage_file and decoded on the GPU, bit-exact against the rows of the
+reference's TPC-H Parquet files or the oracle; CONSTANT segments decode from
+their statistics."""
+import numpy as np
+import pytest
+
+from sirius_amd.duckdb_file import DuckDBFile
+from sirius_amd.stage import scan_column
+from tests.duckdb_writer import write_database
+from tests.helpers import duckdb_cases, validity_bits
+from tests.test_duckdb_segments import fixture_tables, oracle_fixed, oracle_strings
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+CASES = duckdb_cases()
+
+
+@pytest.fixture(scope="module")
+def dbfile(tmp_path_factory):
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    path = write_database(tmp_path_factory.mktemp("db") / "fixtures.duckdb", fixture_tables(CASES))
+    with DuckDBFile(str(path)) as f:
+        yield f
+
+
+@pytest.mark.parametrize("case", CASES, ids=lambda c: c["name"])
+def test_scan_column(dbfile, case):
+    print(f"[gpu] scan {case['name']}", flush=True)
+    out = scan_column(dbfile, case["name"], case["src"]["column"], torch.device("cuda:0"))
+    torch.cuda.synchronize()
+    if case["string"]:
+        want_o, want_c = (case["offsets"], case["chars"]) if case["chars"] is not None else oracle_strings(case)
+        np.testing.assert_array_equal(out["offsets"].cpu().numpy(), want_o)
+        np.testing.assert_array_equal(out["chars"].cpu().numpy(), want_c)
+    else:
+        want = case["expected"] if case["expected"] is not None else oracle_fixed(case)
+        np.testing.assert_array_equal(out["values"].cpu().numpy(), want)
+    if case["vtable"] is not None:
+        valid = validity_bits(case)
+        assert out["null_count"] == int((valid == 0).sum()) > 0
+        got = np.unpackbits(out["mask"].cpu().numpy(), bitorder="little")[:case["total"]]
+        np.testing.assert_array_equal(got, valid)
+    else:
+        assert out["mask"] is None and out["null_count"] == 0
+
+
+def test_scan_constant_column(dbfile):
+    out = scan_column(dbfile, "constants", "k", torch.device("cuda:0"))
+    torch.cuda.synchronize()
+    v = out["values"].cpu().numpy().view(np.int64)
+    np.testing.assert_array_equal(v, np.r_[np.full(3000, -77), np.full(2000, 123456789012)])
