@@ -365,6 +365,23 @@ int sdr_decode_strings_column(sdr_string_column* col, sdr_stream_t stream);
 int sdr_decode_strings_column_mode(sdr_string_column* col, sdr_stream_t stream,
                                    uint32_t launch_mode);
 
+/* ---- row ids --------------------------------------------------------------- */
+
+/* Rows [first_row, first_row + count) of a table, written as int64 to
+ * d_out[out_offset, out_offset + count). */
+typedef struct sdr_row_range {
+  int64_t first_row;
+  uint64_t out_offset;
+  uint32_t count;
+  uint32_t reserved;
+} sdr_row_range;
+
+/* Fill the row-id column of a scan (one range per row group).  Replaces
+ * build_rowid_column (src/op/scan/duckdb_native_decoder.cpp:924-949), a
+ * sequence per row group then a concatenation; here one launch writes every
+ * range in place. */
+int sdr_rowids(const sdr_row_range* ranges, uint32_t n, int64_t* d_out, sdr_stream_t stream);
+
 /* ---- staging segment bytes from a file ----------------------------------- */
 
 /* One byte range of a file (a segment).  device_offset is an output: the

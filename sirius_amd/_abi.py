@@ -53,6 +53,16 @@ TUNE_NAMES = {TUNE_RLE_SLICE_ROWS: "rle_slice_rows", TUNE_BP_GROUPS_PER_WAVE: "b
               TUNE_BP_CHUNK_ROWS: "bp_chunk_rows", TUNE_ALP_VECTORS_PER_WAVE: "alp_vectors_per_wave"}
 
 
+class RowRange(C.Structure):
+    """sdr_row_range: table rows [first_row, first_row + count) written at out_offset."""
+    _fields_ = [
+        ("first_row", C.c_int64),
+        ("out_offset", C.c_uint64),
+        ("count", C.c_uint32),
+        ("reserved", C.c_uint32),
+    ]
+
+
 class Extent(C.Structure):
     """sdr_extent: a byte range of a file and its position in the staged device image."""
     _fields_ = [

@@ -232,6 +232,28 @@ hipError_t launch_constant(const FixedSeg* d_segs, uint32_t seg_base, uint32_t n
   });
 }
 
+// One y-block row per range (the reference's cudf::sequence per row group,
+// duckdb_native_decoder.cpp:924-949, without the concatenation).
+__global__ __launch_bounds__(BLOCK_THREADS) void k_rowids(const RowRange* __restrict__ ranges, int64_t* __restrict__ out)
+{
+  const RowRange r = gload(ranges + blockIdx.y);
+  const gptr<int64_t> o = gbl(out + r.out_offset);
+  for (uint32_t i = blockIdx.x * BLOCK_THREADS + threadIdx.x; i < r.count; i += gridDim.x * BLOCK_THREADS)
+    o[i] = r.first_row + static_cast<int64_t>(i);
+}
+
+hipError_t launch_rowids(const RowRange* d_ranges, uint32_t n, uint32_t max_count, int64_t* out, hipStream_t s)
+{
+  if (n == 0 || max_count == 0) return hipSuccess;
+  uint32_t gx = (max_count + BLOCK_THREADS - 1) / BLOCK_THREADS;
+  if (gx > 1024) gx = 1024;
+  for (uint32_t b = 0; b < n; b += 65535) {  // grid y limit
+    const uint32_t m = n - b < 65535 ? n - b : 65535;
+    hipLaunchKernelGGL(k_rowids, dim3(gx, m), dim3(BLOCK_THREADS), 0, s, d_ranges + b, out);
+  }
+  return hipGetLastError();
+}
+
 hipError_t launch_fill_u8(uint8_t* p, uint8_t v, uint64_t n, hipStream_t s)
 {
   if (n == 0) return hipSuccess;

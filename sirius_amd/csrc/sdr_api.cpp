@@ -653,6 +653,26 @@ int sdr_plan_destroy(sdr_plan* plan)
   return SDR_OK;
 }
 
+int sdr_rowids(const sdr_row_range* ranges, uint32_t n, int64_t* d_out, sdr_stream_t stream)
+{
+  clear_error();
+  if (n == 0) return SDR_OK;
+  if (!ranges) return fail(SDR_E_INVALID_ARG, "ranges is NULL");
+  static_assert(sizeof(sdr_row_range) == sizeof(RowRange), "sdr_row_range layout");
+  uint32_t max_count = 0;
+  for (uint32_t i = 0; i < n; ++i) max_count = ranges[i].count > max_count ? ranges[i].count : max_count;
+  if (max_count && !d_out) return fail(SDR_E_INVALID_ARG, "d_out is NULL");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  void* d       = nullptr;
+  void* token   = nullptr;
+  int rc        = desc_acquire(ranges, sizeof(sdr_row_range) * n, s, &d, &token);
+  if (rc) return rc;
+  const hipError_t e = launch_rowids(static_cast<const RowRange*>(d), n, max_count, d_out, s);
+  rc                 = desc_release(token, s);
+  if (e != hipSuccess) return fail(SDR_E_HIP, "row id launch: %s", hipGetErrorString(e));
+  return rc;
+}
+
 int sdr_decode_bitpacking_data(const sdr_run* run, void* d_out, uint32_t ts, sdr_stream_t stream)
 {
   return decode_one_run(SDR_CODEC_BITPACKING, run, d_out, ts, static_cast<hipStream_t>(stream));

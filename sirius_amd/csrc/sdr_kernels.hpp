@@ -32,6 +32,15 @@ hipError_t launch_uncompressed(const FixedSeg* d_segs, uint32_t seg_base, uint32
 hipError_t launch_constant(const FixedSeg* d_segs, uint32_t seg_base, uint32_t n, uint32_t max_rows,
                            uint32_t type_size, hipStream_t s);
 
+// Row ids: out[r.out_offset + i] = r.first_row + i for each range r (i < r.count).
+struct RowRange {
+  int64_t first_row;
+  uint64_t out_offset;
+  uint32_t count;
+  uint32_t reserved;
+};
+hipError_t launch_rowids(const RowRange* d_ranges, uint32_t n, uint32_t max_count, int64_t* out, hipStream_t s);
+
 // Validity: fill mask bytes, then count zero bits among the first `rows`.
 hipError_t launch_fill_u8(uint8_t* p, uint8_t v, uint64_t n, hipStream_t s);
 hipError_t launch_count_nulls(const uint8_t* mask, uint32_t rows, unsigned long long* d_count,

@@ -26,7 +26,7 @@ DECLARED = [
     "sdr_server_create", "sdr_server_stage", "sdr_server_unstage", "sdr_server_submit", "sdr_server_progress",
     "sdr_server_poll", "sdr_server_release", "sdr_server_pending", "sdr_server_held_bytes", "sdr_server_now_ms",
     "sdr_server_destroy",
-    "sdr_stage_plan", "sdr_stage_file",
+    "sdr_stage_plan", "sdr_stage_file", "sdr_rowids",
 ]
 
 _lib = None
@@ -120,6 +120,8 @@ def load() -> C.CDLL:
     lib.sdr_stage_plan.restype = C.c_uint64
     lib.sdr_stage_file.argtypes = [C.c_char_p, P(_abi.Extent), u32, C.c_uint64, vp, C.c_uint64, vp]
     lib.sdr_stage_file.restype = i32
+    lib.sdr_rowids.argtypes = [P(_abi.RowRange), u32, vp, vp]
+    lib.sdr_rowids.restype = i32
     lib.sdr_batch_create.argtypes = [P(vp)]
     lib.sdr_batch_set_option.argtypes = [vp, u32, C.c_int64]
     lib.sdr_batch_add_table.argtypes = [vp, P(_abi.Column), u32]

@@ -139,3 +139,28 @@ def scan_column(f: DuckDBFile, table: str, column: str, device, stream=None, max
         out.update(values=values[:n * ts], null_count=arr[0].null_count)
     del keep, vkeep
     return out
+
+
+def scan_table(f: DuckDBFile, table: str, columns=None, device="cuda:0", stream=None, rowid: bool = False,
+               max_gap: int = DEFAULT_MAX_GAP):
+    """Decode columns of a table (all by default): name -> scan_column's output.  With
+    `rowid`, "rowid" holds each row's int64 row id, one range per row group (sdr_rowids).
+    The outputs are flat device buffers (torch tensors, exportable without a copy through
+    torch.utils.dlpack)."""
+    import torch
+
+    t = f.tables[table]
+    names = list(columns) if columns is not None else [c.name for c in t.columns]
+    out = {n: scan_column(f, table, n, device, stream, max_gap) for n in names}
+    if rowid:
+        groups = [(start, count) for start, count in t.row_groups if count]
+        rr = (A.RowRange * max(1, len(groups)))()
+        pos = 0
+        for i, (start, count) in enumerate(groups):
+            rr[i] = A.RowRange(start, pos, count, 0)
+            pos += count
+        ids = torch.empty(max(1, pos), dtype=torch.int64, device=device)
+        sh = (stream or torch.cuda.current_stream(device)).cuda_stream
+        L.check(L.load().sdr_rowids(rr, len(groups), C.c_void_p(ids.data_ptr()), sh))
+        out["rowid"] = ids[:pos]
+    return out

@@ -53,3 +53,28 @@ def test_scan_constant_column(dbfile):
     torch.cuda.synchronize()
     v = out["values"].cpu().numpy().view(np.int64)
     np.testing.assert_array_equal(v, np.r_[np.full(3000, -77), np.full(2000, 123456789012)])
+
+
+def test_rowids_per_range():
+    import ctypes as C
+
+    import sirius_amd._abi as A
+    from sirius_amd import lib as L
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    ranges = [(0, 0, 122880), (245760, 122880, 1000), (10**12, 123880, 0), (-5, 123880, 300000)]
+    rr = (A.RowRange * len(ranges))(*[A.RowRange(a, b, c, 0) for a, b, c in ranges])
+    total = 123880 + 300000
+    out = torch.full((total + 8,), -1, dtype=torch.int64, device="cuda:0")
+    L.check(L.load().sdr_rowids(rr, len(ranges), C.c_void_p(out.data_ptr()), torch.cuda.current_stream().cuda_stream))
+    torch.cuda.synchronize()
+    want = np.r_[np.arange(122880), 245760 + np.arange(1000), -5 + np.arange(300000), np.full(8, -1)]
+    np.testing.assert_array_equal(out.cpu().numpy(), want)
+
+
+def test_scan_table_with_rowids(dbfile):
+    from sirius_amd.stage import scan_table
+    out = scan_table(dbfile, "constants", device=torch.device("cuda:0"), rowid=True)
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(out["rowid"].cpu().numpy(), np.arange(5000))
+    assert set(out) == {"k", "rowid"}
