@@ -142,6 +142,9 @@ class _Reader:
 
     def __init__(self, f: "DuckDBFile", pointer: int, offset: int):
         self.f = f
+        # no object spans more than all metadata pieces could hold: a cyclic
+        # chain or a corrupt length ends in FormatError, not a loop
+        self.budget = f.block_count * METADATA_PIECES * f.piece_size
         self._jump(pointer, offset)
 
     def _jump(self, pointer: int, offset: int):
@@ -153,6 +156,9 @@ class _Reader:
         self.pos = max(offset, 8)
 
     def read(self, n: int) -> bytes:
+        self.budget -= n
+        if self.budget < 0:
+            raise FormatError("metadata object longer than the file's metadata")
         out = bytearray()
         while n:
             if self.pos >= self.f.piece_size:
@@ -313,7 +319,10 @@ class DuckDBFile:
             raise FormatError(f"{path}: {nblocks} blocks do not fit in {len(self.buf)} bytes")
         self.tables: dict[str, Table] = {}
         if meta >= 0:
-            self._walk(meta)
+            try:
+                self._walk(meta)
+            except RecursionError:  # objects nested without end
+                raise FormatError(f"{path}: metadata nests too deeply") from None
         self._bound_sizes()
 
     # byte offset of a block's payload (the reference's duckdb_block_payload_offset)

@@ -202,3 +202,20 @@ def test_reader_on_a_written_file(tmp_path):
             assert len([v for v in col.validity if v.codec == 1]) == (0 if c["vtable"] is None else len(c["vtable"]))
         k = f.tables["constants"].columns[0]
         assert [(s.codec, s.stats["min"]) for s in k.segments] == [(2, -77), (2, 123456789012)]
+
+
+def test_reader_rejects_a_cyclic_metadata_chain(tmp_path):
+    from tests.duckdb_writer import write_database
+    path = write_database(tmp_path / "c.duckdb", fixture_tables([c for c in CASES if c["name"] == "tpch_s_name"]))
+    raw = bytearray(path.read_bytes())
+    # point the catalog piece's next pointer at itself and make its list count huge
+    import struct
+    head = struct.unpack_from("<q", raw, 4096 + 16)[0]
+    piece = head >> 56
+    base = 3 * 4096 + 8 + piece * ((262136 // 64) & ~7)
+    struct.pack_into("<Q", raw, base, head)
+    raw[base + 8 + 2:base + 8 + 7] = b"\xff\xff\xff\xff\x0f"  # catalog list count: a huge varint
+    bad = tmp_path / "bad.duckdb"
+    bad.write_bytes(bytes(raw))
+    with pytest.raises(FormatError):
+        DuckDBFile(str(bad))
