@@ -195,6 +195,10 @@ constexpr uint32_t ALP_VEC = 1024;
 // the right stream (<= 8 KiB + head + guard).
 constexpr uint32_t ALPRD_LEFT_WORDS = 112;
 constexpr uint32_t ALP_STAGE_WORDS  = ALPRD_LEFT_WORDS + STAGE_WORDS;
+// ALP stages a vector's packed codes 4 KiB at a time (one stage unless the
+// width exceeds 31 bits), so its kernel keeps 8 waves per SIMD.
+constexpr uint32_t ALP_CHUNK_BYTES = 4096;
+constexpr uint32_t ALP_CHUNK_WORDS = (ALP_CHUNK_BYTES + 48) / 4;
 
 __constant__ int64_t c_alp_fact[19] = {1LL,
                                        10LL,
@@ -302,25 +306,35 @@ __device__ void alp_vector(const uint8_t* seg, uint32_t bytes, uint32_t v, uint3
 
   // ---- compressed ----
   const uint8_t* pk = seg + data_off + 13u;
-  wave_lds_sync();
-  const uint32_t head = stage_to_lds(lds, pk, (fill * bw + 7u) / 8u);
-  wave_lds_sync();
   const T fact = static_cast<T>(c_alp_fact[fac]);
   T frac;
   if constexpr (TS == 8) frac = c_alp_frac_f64[exp];
   else frac = c_alp_frac_f32[exp];
-  for (uint32_t s0 = 0; s0 < fill; s0 += WAVE * VPL) {
-    const uint32_t r = s0 + lane_id() * VPL;
-    T x[VPL];
+  // rows per stage: the whole vector when its codes fit ALP_CHUNK_BYTES,
+  // else a multiple of the store step (wide codes, bw > 31)
+  constexpr uint32_t STEP = WAVE * VPL;
+  uint32_t ch = fill;
+  if ((fill * bw + 7u) / 8u + 16u > ALP_CHUNK_BYTES) ch = ((ALP_CHUNK_BYTES - 16u) * 8u / bw) / STEP * STEP;
+  for (uint32_t r0 = 0; r0 < fill; r0 += ch) {
+    const uint32_t r1   = fill - r0 < ch ? fill : r0 + ch;
+    const uint32_t bit0 = r0 * bw;
+    const uint32_t b0   = bit0 >> 3;
+    wave_lds_sync();
+    const uint32_t head = stage_to_lds(lds, pk + b0, (r1 * bw + 7u) / 8u - b0) + (bit0 & 7u);
+    wave_lds_sync();
+    for (uint32_t s0 = r0; s0 < r1; s0 += STEP) {
+      const uint32_t r = s0 + lane_id() * VPL;
+      T x[VPL];
 #pragma unroll
-    for (uint32_t i = 0; i < VPL; ++i) {
-      const uint32_t row = r + i;
-      const uint64_t u   = row < fill ? lds_bits64(lds, head + row * bw, bw) : 0;
-      const int64_t enc  = static_cast<int64_t>(u + for_v);
-      // ((T)enc * (T)FACT[fac]) * FRAC[exp], left to right, no contraction
-      x[i] = static_cast<T>(enc) * fact * frac;
+      for (uint32_t i = 0; i < VPL; ++i) {
+        const uint32_t row = r + i;
+        const uint64_t u   = row < r1 ? lds_bits64(lds, head + (row - r0) * bw, bw) : 0;
+        const int64_t enc  = static_cast<int64_t>(u + for_v);
+        // ((T)enc * (T)FACT[fac]) * FRAC[exp], left to right, no contraction
+        x[i] = static_cast<T>(enc) * fact * frac;
+      }
+      if (r < r1) store_vec<T>(out + r, x, r1 - r, al);
     }
-    if (r < fill) store_vec<T>(out + r, x, fill - r, al);
   }
   if (ec) {
     const uint8_t* exc = pk + packed;

@@ -269,3 +269,21 @@ def test_config2_columns_full_size():
         enc = f(v)
         got = decode_gpu(enc)
         assert np.array_equal(got, v.view(np.uint8))
+
+
+@pytest.mark.parametrize("bits", [31, 32, 45, 48, 52])
+@pytest.mark.parametrize("dtype", [np.float64, np.float32], ids=["f64", "f32"])
+def test_alp_wide_codes_staged_in_chunks(bits, dtype):
+    """ALP vectors whose codes do not fit one 4 KiB stage (widths above 31 bits) decode in
+    row chunks; a float column tops out at 32-bit codes."""
+    if dtype == np.float32 and bits > 32:
+        pytest.skip("float codes are at most 32 bits")
+    rng = np.random.default_rng(bits)
+    n = 100_000
+    k = rng.integers(0, 2 ** min(bits, 53), n, dtype=np.int64)
+    vals = (k.astype(np.float64) / 100.0).astype(dtype)
+    enc = E.encode_alp(vals)
+    np.testing.assert_array_equal(O.decode_encoded(enc).view(dtype), vals)  # oracle round trip
+    for mode in (A.LAUNCH_BATCHED, A.LAUNCH_PERSISTENT):
+        got = decode_gpu(enc, mode)
+        np.testing.assert_array_equal(got, O.decode_encoded(enc))
