@@ -450,31 +450,89 @@ __device__ void alprd_vector(const uint8_t* seg, uint32_t bytes, uint32_t v, uin
 }
 
 // ===========================================================================
-// UNCOMPRESSED / CONSTANT units: bytes [b0, b1) of the segment's output,
-// written by one wavefront.
+// UNCOMPRESSED / CONSTANT spans: bytes [b0, b1) of a segment's output, written
+// by NT cooperating threads (t = 0 .. NT-1; NT >= 16).  Bytes up to the next
+// 16-byte boundary of the destination, and the last < 16 bytes, are single
+// byte stores; the rest are 16-byte stores, four per thread in flight.
 // ===========================================================================
-__device__ __forceinline__ void copy_unit(const uint8_t* src_p, uint8_t* dst_p, uint64_t b0, uint64_t b1)
+template <uint32_t NT>
+__device__ __forceinline__ void copy_span(const uint8_t* src_p, uint8_t* dst_p, uint64_t b0, uint64_t b1, uint32_t t)
 {
   const gptr<const uint8_t> src = gbl(src_p);
   const gptr<uint8_t> dst       = gbl(dst_p);
-  const bool al = ((reinterpret_cast<uintptr_t>(src) | reinterpret_cast<uintptr_t>(dst)) & 15u) == 0;
-  if (al) {
-    const uint64_t v0 = (b0 + 15u) / 16u, v1 = b1 / 16u;
-    for (uint64_t i = b0 + lane_id(); i < v0 * 16u && i < b1; i += WAVE) dst[i] = src[i];
-    for (uint64_t i = v0 + lane_id(); i < v1; i += WAVE)
-      __builtin_nontemporal_store(__builtin_nontemporal_load(reinterpret_cast<gptr<const u32x4>>(src) + i),
-                                  reinterpret_cast<gptr<u32x4>>(dst) + i);
-    for (uint64_t i = (v1 > v0 ? v1 * 16u : b1) + lane_id(); i < b1; i += WAVE) dst[i] = src[i];
-  } else {
-    for (uint64_t i = b0 + lane_id(); i < b1; i += WAVE) dst[i] = src[i];
+  const uintptr_t sa = reinterpret_cast<uintptr_t>(src_p) + b0, da = reinterpret_cast<uintptr_t>(dst_p) + b0;
+  if ((sa ^ da) & 15u) {  // source and destination at different phases of 16: bytes
+    for (uint64_t i = b0 + t; i < b1; i += NT) dst[i] = src[i];
+    return;
   }
+  const uint64_t head = (16u - (da & 15u)) & 15u;
+  const uint64_t h    = head < b1 - b0 ? head : b1 - b0;
+  if (t < h) dst[b0 + t] = src[b0 + t];
+  const uint64_t v0 = b0 + h;
+  const uint64_t nv = (b1 - v0) / 16u;
+  const gptr<const u32x4> vs = reinterpret_cast<gptr<const u32x4>>(src + v0);
+  const gptr<u32x4> vd       = reinterpret_cast<gptr<u32x4>>(dst + v0);
+  uint64_t k = t;
+  for (; k + 3u * NT < nv; k += 4u * NT) {
+    const u32x4 x0 = __builtin_nontemporal_load(vs + k);
+    const u32x4 x1 = __builtin_nontemporal_load(vs + k + NT);
+    const u32x4 x2 = __builtin_nontemporal_load(vs + k + 2u * NT);
+    const u32x4 x3 = __builtin_nontemporal_load(vs + k + 3u * NT);
+    __builtin_nontemporal_store(x0, vd + k);
+    __builtin_nontemporal_store(x1, vd + k + NT);
+    __builtin_nontemporal_store(x2, vd + k + 2u * NT);
+    __builtin_nontemporal_store(x3, vd + k + 3u * NT);
+  }
+  for (; k < nv; k += NT) __builtin_nontemporal_store(__builtin_nontemporal_load(vs + k), vd + k);
+  const uint64_t tb = v0 + nv * 16u;
+  if (tb + t < b1) dst[tb + t] = src[tb + t];
 }
 
-__device__ __forceinline__ void const_unit(const uint8_t* val, uint32_t ts, uint8_t* dst, uint64_t b0, uint64_t b1)
+// The output repeats a ts-byte value (ts = 1, 2, 4, 8 or 16, so the period
+// divides 16): byte i of the output is byte (i & (ts - 1)) of the value.
+template <uint32_t NT>
+__device__ __forceinline__ void const_span(const uint8_t* val_p, uint32_t ts, uint8_t* dst_p, uint64_t b0, uint64_t b1,
+                                           uint32_t t)
 {
-  uint8_t v[16];
-  for (uint32_t i = 0; i < ts; ++i) v[i] = gbl(val)[i];
-  for (uint64_t i = b0 + lane_id(); i < b1; i += WAVE) gbl(dst)[i] = v[i % ts];
+  const gptr<const uint8_t> val = gbl(val_p);
+  uint64_t lo = 0, hi = 0;  // the value's bytes 0..7 and 8..15
+  for (uint32_t i = 0; i < ts && i < 8u; ++i) lo |= uint64_t{val[i]} << (8u * i);
+  for (uint32_t i = 8; i < ts; ++i) hi |= uint64_t{val[i]} << (8u * (i - 8u));
+  if (ts < 16u) {  // replicate the period over 16 bytes
+    for (uint32_t w = ts; w < 8u; w *= 2u) lo |= lo << (8u * w);
+    hi = lo;
+  }
+  const uint32_t m = ts - 1u;
+  auto byte_at = [&](uint64_t i) -> uint8_t {
+    const uint32_t j = static_cast<uint32_t>(i) & m;  // ts <= 16: j < 16
+    return static_cast<uint8_t>((j < 8u ? lo >> (8u * j) : hi >> (8u * (j - 8u))) & 0xFFu);
+  };
+  const gptr<uint8_t> dst = gbl(dst_p);
+  const uintptr_t da      = reinterpret_cast<uintptr_t>(dst_p) + b0;
+  const uint64_t head     = (16u - (da & 15u)) & 15u;
+  const uint64_t h        = head < b1 - b0 ? head : b1 - b0;
+  if (t < h) dst[b0 + t] = byte_at(b0 + t);
+  const uint64_t v0 = b0 + h;
+  const uint64_t nv = (b1 - v0) / 16u;
+  // the 16 bytes from v0 on: the pattern rotated right by (v0 mod 16) bytes
+  const uint32_t ph = static_cast<uint32_t>(v0) & 15u;
+  uint64_t a = lo, c = hi;
+  if (ph >= 8u) {
+    a = hi;
+    c = lo;
+  }
+  const uint32_t r = (ph & 7u) * 8u;
+  const uint64_t plo = r ? (a >> r) | (c << (64u - r)) : a;
+  const uint64_t phi = r ? (c >> r) | (a << (64u - r)) : c;
+  u32x4 x;
+  x.x = static_cast<uint32_t>(plo);
+  x.y = static_cast<uint32_t>(plo >> 32);
+  x.z = static_cast<uint32_t>(phi);
+  x.w = static_cast<uint32_t>(phi >> 32);
+  const gptr<u32x4> vd = reinterpret_cast<gptr<u32x4>>(dst + v0);
+  for (uint64_t k = t; k < nv; k += NT) __builtin_nontemporal_store(x, vd + k);
+  const uint64_t tb = v0 + nv * 16u;
+  if (tb + t < b1) dst[tb + t] = byte_at(tb + t);
 }
 
 }  // namespace sdr

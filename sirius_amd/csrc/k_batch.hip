@@ -79,8 +79,8 @@ __global__ __launch_bounds__(BLOCK_THREADS) __attribute__((amdgpu_waves_per_eu(4
         for (uint32_t u = it.unit0 + w; u < it.unit0 + it.n_units; u += WAVES_PER_BLOCK) {
           const uint64_t b0 = uint64_t{u} * COPY_UNIT_BYTES;
           const uint64_t b1 = total - b0 < COPY_UNIT_BYTES ? total : b0 + COPY_UNIT_BYTES;
-          if (it.kind == BK_COPY) copy_unit(it.seg, it.out, b0, b1);
-          else const_unit(it.seg, it.ts, it.out, b0, b1);
+          if (it.kind == BK_COPY) copy_span<WAVE>(it.seg, it.out, b0, b1, lane_id());
+          else const_span<WAVE>(it.seg, it.ts, it.out, b0, b1, lane_id());
         }
         break;
       }

@@ -230,6 +230,30 @@ def test_uncompressed_and_constant():
         np.testing.assert_array_equal(got[n * ts:], np.tile(const, n))
 
 
+@pytest.mark.parametrize("mode", [A.LAUNCH_BATCHED, A.LAUNCH_PERSISTENT], ids=["batched", "persistent"])
+@pytest.mark.parametrize("skew,out_skew", [(0, 0), (0, 5), (3, 3), (8, 1), (13, 0)])
+def test_uncompressed_and_constant_any_alignment(mode, skew, out_skew):
+    """Copies and fills at every phase of 16 between source and destination
+    (vector stores after a byte head, or bytes throughout), spans of several
+    16 KiB units with odd tails; bytes outside the output keep their fill."""
+    rng = np.random.default_rng(11 + skew + 7 * out_skew)
+    for ts in (1, 2, 4, 8, 16):
+        n = 2101 if ts >= 8 else 9001
+        raw = rng.integers(0, 256, n * ts, dtype=np.uint8)
+        const = rng.integers(0, 256, ts, dtype=np.uint8)
+        d1, b1 = upload(raw, skew)
+        d2, b2 = upload(const, skew + 1)
+        out = torch.full((2 * n * ts + out_skew + 64,), 0xCC, dtype=torch.uint8, device=_dev())
+        col, keep = fixed_column([(A.UNCOMPRESSED, [A.Segment(b1, n * ts, 0, n, 0)]),
+                                  (A.CONSTANT, [A.Segment(b2, ts, n, n, 0)])], ts, 2 * n, out.data_ptr() + out_skew)
+        run_table([col], mode)
+        got = out.cpu().numpy()
+        np.testing.assert_array_equal(got[out_skew:out_skew + n * ts], raw, err_msg=f"copy ts={ts}")
+        np.testing.assert_array_equal(got[out_skew + n * ts:out_skew + 2 * n * ts], np.tile(const, n),
+                                      err_msg=f"constant ts={ts}")
+        assert (got[:out_skew] == 0xCC).all() and (got[out_skew + 2 * n * ts:] == 0xCC).all()
+
+
 def test_errors_are_returned_not_thrown():
     n = 100
     enc = E.encode_bitpacking(E.gen_bp_int32(81, n))
