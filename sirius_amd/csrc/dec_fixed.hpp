@@ -452,40 +452,51 @@ __device__ void alprd_vector(const uint8_t* seg, uint32_t bytes, uint32_t v, uin
 // ===========================================================================
 // UNCOMPRESSED / CONSTANT spans: bytes [b0, b1) of a segment's output, written
 // by NT cooperating threads (t = 0 .. NT-1; NT >= 16).  Bytes up to the next
-// 16-byte boundary of the destination, and the last < 16 bytes, are single
-// byte stores; the rest are 16-byte stores, four per thread in flight.
+// word boundary of the destination, and the last partial word, are single
+// byte stores; the rest are word stores, four per thread in flight.
 // ===========================================================================
-template <uint32_t NT>
-__device__ __forceinline__ void copy_span(const uint8_t* src_p, uint8_t* dst_p, uint64_t b0, uint64_t b1, uint32_t t)
+template <uint32_t NT, typename W>
+__device__ __forceinline__ void copy_words(gptr<const uint8_t> src, gptr<uint8_t> dst, uintptr_t da, uint64_t b0,
+                                           uint64_t b1, uint32_t t)
 {
-  const gptr<const uint8_t> src = gbl(src_p);
-  const gptr<uint8_t> dst       = gbl(dst_p);
-  const uintptr_t sa = reinterpret_cast<uintptr_t>(src_p) + b0, da = reinterpret_cast<uintptr_t>(dst_p) + b0;
-  if ((sa ^ da) & 15u) {  // source and destination at different phases of 16: bytes
-    for (uint64_t i = b0 + t; i < b1; i += NT) dst[i] = src[i];
-    return;
-  }
-  const uint64_t head = (16u - (da & 15u)) & 15u;
-  const uint64_t h    = head < b1 - b0 ? head : b1 - b0;
+  constexpr uint32_t G = sizeof(W);
+  const uint64_t head  = (G - (da & (G - 1u))) & (G - 1u);
+  const uint64_t h     = head < b1 - b0 ? head : b1 - b0;
   if (t < h) dst[b0 + t] = src[b0 + t];
   const uint64_t v0 = b0 + h;
-  const uint64_t nv = (b1 - v0) / 16u;
-  const gptr<const u32x4> vs = reinterpret_cast<gptr<const u32x4>>(src + v0);
-  const gptr<u32x4> vd       = reinterpret_cast<gptr<u32x4>>(dst + v0);
+  const uint64_t nv = (b1 - v0) / G;
+  const gptr<const W> vs = reinterpret_cast<gptr<const W>>(src + v0);
+  const gptr<W> vd       = reinterpret_cast<gptr<W>>(dst + v0);
   uint64_t k = t;
   for (; k + 3u * NT < nv; k += 4u * NT) {
-    const u32x4 x0 = __builtin_nontemporal_load(vs + k);
-    const u32x4 x1 = __builtin_nontemporal_load(vs + k + NT);
-    const u32x4 x2 = __builtin_nontemporal_load(vs + k + 2u * NT);
-    const u32x4 x3 = __builtin_nontemporal_load(vs + k + 3u * NT);
+    const W x0 = __builtin_nontemporal_load(vs + k);
+    const W x1 = __builtin_nontemporal_load(vs + k + NT);
+    const W x2 = __builtin_nontemporal_load(vs + k + 2u * NT);
+    const W x3 = __builtin_nontemporal_load(vs + k + 3u * NT);
     __builtin_nontemporal_store(x0, vd + k);
     __builtin_nontemporal_store(x1, vd + k + NT);
     __builtin_nontemporal_store(x2, vd + k + 2u * NT);
     __builtin_nontemporal_store(x3, vd + k + 3u * NT);
   }
   for (; k < nv; k += NT) __builtin_nontemporal_store(__builtin_nontemporal_load(vs + k), vd + k);
-  const uint64_t tb = v0 + nv * 16u;
+  const uint64_t tb = v0 + nv * G;
   if (tb + t < b1) dst[tb + t] = src[tb + t];
+}
+
+// The word is the widest (16, 8 or 4 bytes) at which source and destination
+// are at the same phase; below 4, bytes.
+template <uint32_t NT>
+__device__ __forceinline__ void copy_span(const uint8_t* src_p, uint8_t* dst_p, uint64_t b0, uint64_t b1, uint32_t t)
+{
+  const gptr<const uint8_t> src = gbl(src_p);
+  const gptr<uint8_t> dst       = gbl(dst_p);
+  const uintptr_t sa = reinterpret_cast<uintptr_t>(src_p) + b0, da = reinterpret_cast<uintptr_t>(dst_p) + b0;
+  const uintptr_t x  = sa ^ da;
+  if ((x & 15u) == 0) copy_words<NT, u32x4>(src, dst, da, b0, b1, t);
+  else if ((x & 7u) == 0) copy_words<NT, uint64_t>(src, dst, da, b0, b1, t);
+  else if ((x & 3u) == 0) copy_words<NT, uint32_t>(src, dst, da, b0, b1, t);
+  else
+    for (uint64_t i = b0 + t; i < b1; i += NT) dst[i] = src[i];
 }
 
 // The output repeats a ts-byte value (ts = 1, 2, 4, 8 or 16, so the period

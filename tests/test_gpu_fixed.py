@@ -231,11 +231,12 @@ def test_uncompressed_and_constant():
 
 
 @pytest.mark.parametrize("mode", [A.LAUNCH_BATCHED, A.LAUNCH_PERSISTENT], ids=["batched", "persistent"])
-@pytest.mark.parametrize("skew,out_skew", [(0, 0), (0, 5), (3, 3), (8, 1), (13, 0)])
+@pytest.mark.parametrize("skew,out_skew", [(0, 0), (0, 5), (3, 3), (8, 1), (13, 0), (8, 0), (4, 0), (5, 9), (2, 0)])
 def test_uncompressed_and_constant_any_alignment(mode, skew, out_skew):
-    """Copies and fills at every phase of 16 between source and destination
-    (vector stores after a byte head, or bytes throughout), spans of several
-    16 KiB units with odd tails; bytes outside the output keep their fill."""
+    """Copies and fills at phases of 16 between source and destination that
+    take each word width (16, 8, 4 bytes after a byte head, or bytes
+    throughout), spans of several units with odd tails; bytes outside the
+    output keep their fill."""
     rng = np.random.default_rng(11 + skew + 7 * out_skew)
     for ts in (1, 2, 4, 8, 16):
         n = 2101 if ts >= 8 else 9001
