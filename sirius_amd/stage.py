@@ -62,8 +62,13 @@ def scan_column(f: DuckDBFile, table: str, column: str, device, stream=None, max
     """Decode one column of a table: stage its segments, then sdr_decode_table (fixed width)
     or sdr_decode_strings_column (VARCHAR).  Returns a dict with the device tensors:
     fixed width: values (uint8, rows * type_size), mask (or None), null_count;
-    VARCHAR: offsets (int32, rows + 1), chars (uint8), mask (or None), null_count."""
+    VARCHAR: offsets (int32, rows + 1), chars (uint8), mask (or None), null_count.
+    With `stream`, every allocation, copy and launch runs on that stream."""
     import torch
+
+    if stream is not None and stream != torch.cuda.current_stream(device):
+        with torch.cuda.stream(stream):
+            return scan_column(f, table, column, device, None, max_gap)
 
     t = f.tables[table]
     col = next(c for c in t.columns if c.name == column)
@@ -148,6 +153,10 @@ def scan_table(f: DuckDBFile, table: str, columns=None, device="cuda:0", stream=
     The outputs are flat device buffers (torch tensors, exportable without a copy through
     torch.utils.dlpack)."""
     import torch
+
+    if stream is not None and stream != torch.cuda.current_stream(device):
+        with torch.cuda.stream(stream):
+            return scan_table(f, table, columns, device, None, rowid, max_gap)
 
     t = f.tables[table]
     names = list(columns) if columns is not None else [c.name for c in t.columns]

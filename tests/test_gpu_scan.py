@@ -78,3 +78,27 @@ def test_scan_table_with_rowids(dbfile):
     torch.cuda.synchronize()
     np.testing.assert_array_equal(out["rowid"].cpu().numpy(), np.arange(5000))
     assert set(out) == {"k", "rowid"}
+
+
+def test_scan_on_a_side_stream(dbfile):
+    """A scan given its own stream allocates, copies and decodes there: the
+    results match the same scans on the current stream, with a CONSTANT
+    column (host-built tail), a VARCHAR column and a column with NULLs."""
+    from sirius_amd.stage import scan_table
+    dev = torch.device("cuda:0")
+    names = ["constants"]
+    names += [next(c["name"] for c in CASES if c["string"])]
+    names += [next(c["name"] for c in CASES if c["vtable"] is not None)]
+    side = torch.cuda.Stream(device=dev)
+    for name in names:
+        print(f"[gpu] side-stream scan {name}", flush=True)
+        want = scan_table(dbfile, name, device=dev, rowid=True)
+        torch.cuda.synchronize()
+        got = scan_table(dbfile, name, device=dev, stream=side, rowid=True)
+        side.synchronize()
+        for col, w in want.items():
+            for key in ("values", "offsets", "chars", "mask"):
+                if isinstance(w, dict) and w.get(key) is not None:
+                    np.testing.assert_array_equal(got[col][key].cpu().numpy(), w[key].cpu().numpy(), err_msg=f"{name}.{col}.{key}")
+            if not isinstance(w, dict):
+                np.testing.assert_array_equal(got[col].cpu().numpy(), w.cpu().numpy())
