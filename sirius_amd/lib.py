@@ -27,6 +27,7 @@ DECLARED = [
     "sdr_server_poll", "sdr_server_release", "sdr_server_pending", "sdr_server_held_bytes", "sdr_server_now_ms",
     "sdr_server_destroy",
     "sdr_stage_plan", "sdr_stage_file", "sdr_rowids",
+    "sdr_arrow_export",
 ]
 
 _lib = None
@@ -116,6 +117,8 @@ def load() -> C.CDLL:
     lib.sdr_tune_get.restype = C.c_uint64
     lib.sdr_tune_load.argtypes = [C.c_char_p]
     lib.sdr_tune_load.restype = i32
+    lib.sdr_arrow_export.argtypes = [vp, vp, vp]
+    lib.sdr_arrow_export.restype = i32
     lib.sdr_stage_plan.argtypes = [P(_abi.Extent), u32, C.c_uint64]
     lib.sdr_stage_plan.restype = C.c_uint64
     lib.sdr_stage_file.argtypes = [C.c_char_p, P(_abi.Extent), u32, C.c_uint64, vp, C.c_uint64, vp]

@@ -19,11 +19,11 @@ def declared(header: Path):
 
 
 def test_decode_header_symbols_exported():
-    names = declared(ROOT / "include" / "sdr" / "decode.h")
-    assert "sdr_decode_table" in names and "sdr_decode_strings_column" in names
+    names = sorted(set().union(*(declared(h) for h in (ROOT / "include" / "sdr").glob("*.h"))))
+    assert "sdr_decode_table" in names and "sdr_decode_strings_column" in names and "sdr_arrow_export" in names
     lib = L.load()
     for n in names:
-        assert hasattr(lib, n), f"{n} declared in include/sdr/decode.h but not exported"
+        assert hasattr(lib, n), f"{n} declared in include/sdr/ but not exported"
     assert sorted(L.DECLARED) == names
 
 

@@ -102,3 +102,52 @@ def test_scan_on_a_side_stream(dbfile):
                     np.testing.assert_array_equal(got[col][key].cpu().numpy(), w[key].cpu().numpy(), err_msg=f"{name}.{col}.{key}")
             if not isinstance(w, dict):
                 np.testing.assert_array_equal(got[col].cpu().numpy(), w.cpu().numpy())
+
+
+def test_arrow_export_of_scanned_columns(dbfile):
+    """Scanned columns leave as Arrow device arrays on ROCm memory without a
+    copy (buffer addresses are the decode's tensors); host copies of those
+    buffers, imported by pyarrow, hold the fixture's values and nulls."""
+    pa = pytest.importorskip("pyarrow")
+    import sirius_amd.arrow as X
+    dev = torch.device("cuda:0")
+    picks = [next(c for c in CASES if c["string"] and c["vtable"] is not None),
+             next(c for c in CASES if not c["string"] and c["vtable"] is not None),
+             next(c for c in CASES if c["string"] and c["vtable"] is None)]
+    for case in picks:
+        name, column = case["name"], case["src"]["column"]
+        print(f"[gpu] arrow export {name}", flush=True)
+        out = scan_column(dbfile, name, column, dev)
+        ctype = next(c for c in dbfile.tables[name].columns if c.name == column).type
+        exp = X.export_scan(out, ctype, name=column)
+        a = exp.array
+        assert a.device_type == X.ARROW_DEVICE_ROCM and a.device_id == 0 and a.sync_event
+        main = out["offsets"] if ctype.is_string else out["values"]
+        assert a.array.buffers[1] == main.data_ptr()
+        assert (a.array.buffers[0] or 0) == (out["mask"].data_ptr() if out["mask"] is not None else 0)
+        assert a.array.null_count == out["null_count"] and exp.schema.name == column.encode()
+        torch.cuda.synchronize()
+        # the same layout from host copies, through pyarrow
+        host = {k: out[k].cpu().numpy() for k in ("values", "offsets", "chars", "mask") if out.get(k) is not None}
+        t, vb, prec, scale = X.arrow_type(ctype)
+        n = a.array.length
+        hexp = X.export(t, n, (host["offsets"] if ctype.is_string else host["values"]).ctypes.data,
+                        validity=host["mask"].ctypes.data if "mask" in host else 0,
+                        chars=host["chars"].ctypes.data if ctype.is_string else 0, null_count=out["null_count"],
+                        value_bytes=vb, precision=prec, scale=scale, device_type=X.ARROW_DEVICE_CPU, keep=host)
+        arr = pa.Array._import_from_c_device(hexp.array_ptr, hexp.schema_ptr)
+        valid = validity_bits(case) if case["vtable"] is not None else np.ones(n, dtype=np.uint8)
+        assert arr.null_count == int((valid == 0).sum())
+        if ctype.is_string:
+            want_o, want_c = (case["offsets"], case["chars"]) if case["chars"] is not None else oracle_strings(case)
+            want = [bytes(want_c[want_o[i]:want_o[i + 1]]).decode() if valid[i] else None for i in range(n)]
+            assert arr.to_pylist() == want
+        else:
+            raw = case["expected"] if case["expected"] is not None else oracle_fixed(case)
+            storage = pa.Array.from_buffers(pa.uint8(), n * ctype.type_size, [None, arr.buffers()[1]])
+            np.testing.assert_array_equal(np.asarray(storage), np.asarray(raw).view(np.uint8)[:n * ctype.type_size])
+            assert [v is not None for v in arr.to_pylist()] == [bool(x) for x in valid]
+            del storage
+        del arr
+        exp.release()
+    assert X.live_exports() == 0
