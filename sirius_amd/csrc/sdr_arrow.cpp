@@ -12,6 +12,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <new>
 
 namespace sdr {
 namespace {
@@ -109,7 +110,8 @@ extern "C" int sdr_arrow_export(const sdr_arrow_column* col, ArrowSchema* schema
   if (c.device_type != ARROW_DEVICE_ROCM && c.device_type != ARROW_DEVICE_CPU)
     return fail(SDR_E_INVALID_ARG, "device type %d (ROCm or CPU)", c.device_type);
 
-  auto* ss = new SchemaState();
+  auto* ss = new (std::nothrow) SchemaState();
+  if (!ss) return fail(SDR_E_CAPACITY, "out of host memory");
   if (!format_of(c, ss->format)) {  // unreachable after the range check
     delete ss;
     return fail(SDR_E_INVALID_ARG, "unknown Arrow type %u", c.type);
@@ -119,7 +121,12 @@ extern "C" int sdr_arrow_export(const sdr_arrow_column* col, ArrowSchema* schema
     delete ss;
     return fail(SDR_E_CAPACITY, "out of host memory");
   }
-  auto* as    = new ArrayState();
+  auto* as = new (std::nothrow) ArrayState();
+  if (!as) {
+    std::free(ss->name);
+    delete ss;
+    return fail(SDR_E_CAPACITY, "out of host memory");
+  }
   as->buffers[0] = c.validity;
   as->buffers[1] = c.values;
   as->buffers[2] = c.chars;
