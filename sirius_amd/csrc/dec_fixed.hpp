@@ -192,11 +192,12 @@ __device__ void bp_group(const uint8_t* seg, uint32_t bytes, uint32_t k, uint32_
 // ===========================================================================
 constexpr uint32_t ALP_VEC = 1024;
 // ALPRD stages the left stream (<= 1024 * 3 bits + head + guard) ahead of
-// the right stream (<= 8 KiB + head + guard).
+// the right stream.
 constexpr uint32_t ALPRD_LEFT_WORDS = 112;
 constexpr uint32_t ALP_STAGE_WORDS  = ALPRD_LEFT_WORDS + STAGE_WORDS;
-// ALP stages a vector's packed codes 4 KiB at a time (one stage unless the
-// width exceeds 31 bits), so its kernel keeps 8 waves per SIMD.
+// ALP stages a vector's packed codes, and ALPRD its right parts, 4 KiB at a
+// time (one stage unless the width exceeds 31 bits), so their kernels keep 8
+// waves per SIMD.
 constexpr uint32_t ALP_CHUNK_BYTES = 4096;
 constexpr uint32_t ALP_CHUNK_WORDS = (ALP_CHUNK_BYTES + 48) / 4;
 
@@ -406,46 +407,58 @@ __device__ void alprd_vector(const uint8_t* seg, uint32_t bytes, uint32_t v, uin
 
   const uint8_t* lp = seg + data_off + 2u;
   const uint8_t* rp = lp + lpb;
-  // left stream (<= 384 B) staged at word 0, right stream (<= 8 KiB) after it
-  uint32_t* lds_l   = lds;
-  uint32_t* lds_r   = lds + ALPRD_LEFT_WORDS;
-  wave_lds_sync();
-  const uint32_t rhead = stage_to_lds(lds_r, rp, (fill * rbw + 7u) / 8u);
-  const uint32_t lhead = stage_to_lds(lds_l, lp, (fill * lbw + 7u) / 8u);
-  wave_lds_sync();
+  // left stream (<= 384 B) staged once at word 0; the right stream after it,
+  // 4 KiB at a time (one stage unless the vector's right parts pass that),
+  // so the kernel keeps 8 waves per SIMD
+  uint32_t* lds_l = lds;
+  uint32_t* lds_r = lds + ALPRD_LEFT_WORDS;
+  constexpr uint32_t STEP = WAVE * VPL;
+  uint32_t ch = fill;
+  if ((fill * rbw + 7u) / 8u + 16u > ALP_CHUNK_BYTES) ch = ((ALP_CHUNK_BYTES - 16u) * 8u / rbw) / STEP * STEP;
   const bool full_right = rbw >= TBITS;
-  for (uint32_t s0 = 0; s0 < fill; s0 += WAVE * VPL) {
-    const uint32_t r = s0 + lane_id() * VPL;
-    T x[VPL];
+  const uint8_t* exc    = rp + rpb;
+  const uint8_t* pos    = exc + ec * 2u;
+  wave_lds_sync();
+  const uint32_t lhead = stage_to_lds(lds_l, lp, (fill * lbw + 7u) / 8u);
+  for (uint32_t r0 = 0; r0 < fill; r0 += ch) {
+    const uint32_t r1   = fill - r0 < ch ? fill : r0 + ch;
+    const uint32_t bit0 = r0 * rbw;
+    const uint32_t b0   = bit0 >> 3;
+    if (r0) wave_lds_sync();  // the previous chunk's readers are done
+    const uint32_t rhead = stage_to_lds(lds_r, rp + b0, (r1 * rbw + 7u) / 8u - b0) + (bit0 & 7u);
+    wave_lds_sync();
+    for (uint32_t s0 = r0; s0 < r1; s0 += STEP) {
+      const uint32_t r = s0 + lane_id() * VPL;
+      T x[VPL];
 #pragma unroll
-    for (uint32_t i = 0; i < VPL; ++i) {
-      const uint32_t row = r + i;
-      R bits             = 0;
-      if (row < fill) {
-        const R right = static_cast<R>(lds_bits64(lds_r, rhead + row * rbw, rbw));
-        if (full_right) {
-          bits = right;
-        } else {
-          const uint32_t li = lds_bits32(lds_l, lhead + row * lbw, lbw);
-          R dv              = dict[0];
+      for (uint32_t i = 0; i < VPL; ++i) {
+        const uint32_t row = r + i;
+        R bits             = 0;
+        if (row < r1) {
+          const R right = static_cast<R>(lds_bits64(lds_r, rhead + (row - r0) * rbw, rbw));
+          if (full_right) {
+            bits = right;
+          } else {
+            const uint32_t li = lds_bits32(lds_l, lhead + row * lbw, lbw);
+            R dv              = dict[0];
 #pragma unroll
-          for (uint32_t d = 1; d < 8; ++d) dv = li == d ? dict[d] : dv;
-          bits = static_cast<R>(dv << rbw) | right;
+            for (uint32_t d = 1; d < 8; ++d) dv = li == d ? dict[d] : dv;
+            bits = static_cast<R>(dv << rbw) | right;
+          }
         }
+        x[i] = __builtin_bit_cast(T, bits);
       }
-      x[i] = __builtin_bit_cast(T, bits);
+      if (r < r1) store_vec<T>(out + r, x, r1 - r, al);
     }
-    if (r < fill) store_vec<T>(out + r, x, fill - r, al);
-  }
-  if (ec) {
-    const uint8_t* exc = rp + rpb;
-    const uint8_t* pos = exc + ec * 2u;
-    patch_exceptions(ec, fill, pos, [&](uint32_t e, uint32_t p) {
-      const R right = static_cast<R>(lds_bits64(lds_r, rhead + p * rbw, rbw));
-      const R bits  = full_right ? right
-                                 : static_cast<R>(static_cast<R>(ld_u16_any(exc + 2u * e)) << rbw) | right;
-      gbl(out)[p]   = __builtin_bit_cast(T, bits);
-    });
+    // exceptions of this chunk, while its right parts are staged
+    if (ec)
+      patch_exceptions(ec, fill, pos, [&](uint32_t e, uint32_t p) {
+        if (p < r0 || p >= r1) return;
+        const R right = static_cast<R>(lds_bits64(lds_r, rhead + (p - r0) * rbw, rbw));
+        const R bits  = full_right ? right
+                                   : static_cast<R>(static_cast<R>(ld_u16_any(exc + 2u * e)) << rbw) | right;
+        gbl(out)[p]   = __builtin_bit_cast(T, bits);
+      });
   }
 }
 

@@ -34,7 +34,7 @@ template <typename T, bool RD>
 __global__ __launch_bounds__(BLOCK_THREADS) void k_alp(const FixedSeg* __restrict__ segs, uint32_t seg_base,
                                                        uint32_t vpw)
 {
-  __shared__ uint32_t lds[WAVES_PER_BLOCK][RD ? ALP_STAGE_WORDS : ALP_CHUNK_WORDS];
+  __shared__ uint32_t lds[WAVES_PER_BLOCK][RD ? ALPRD_LEFT_WORDS + ALP_CHUNK_WORDS : ALP_CHUNK_WORDS];
   const FixedSeg sd  = segs[seg_base + blockIdx.y];
   const uint32_t w   = wave_in_block();
   const uint32_t nv  = (sd.row_count + ALP_VEC - 1) / ALP_VEC;
