@@ -18,20 +18,33 @@ namespace sdr {
 
 constexpr uint32_t BATCH_SPIN_LIMIT = FSST_SPIN_LIMIT;
 
-union __attribute__((aligned(16))) BatchSmem {
-  uint32_t fixed[WAVES_PER_BLOCK][ALP_STAGE_WORDS];
+// Per-wave staging of the fixed-width codecs: 4 KiB of bitpacking or ALP
+// codes, or the ALP-RD left stream plus 4 KiB of its right stream.
+constexpr uint32_t BATCH_BP_STAGE    = 4096;
+constexpr uint32_t BATCH_FIXED_WORDS = ALPRD_LEFT_WORDS + ALP_CHUNK_WORDS;
+static_assert(BATCH_FIXED_WORDS >= (BATCH_BP_STAGE + 48) / 4, "bitpacking stage fits");
+
+// Queues with VARCHAR entries need the FSST chunk state; fixed-width-only
+// queues run a kernel without it, at half the LDS.
+union __attribute__((aligned(16))) BatchSmemFixed {
+  uint32_t fixed[WAVES_PER_BLOCK][BATCH_FIXED_WORDS];
+  RleSmem rle;
+};
+union __attribute__((aligned(16))) BatchSmemAll {
+  uint32_t fixed[WAVES_PER_BLOCK][BATCH_FIXED_WORDS];
   RleSmem rle;
   FsstSmemT<FSST_OUT_BATCH> fsst;
 };
 
-template <typename T>
-__device__ __forceinline__ void fixed_units(const BatchItem& it, uint32_t w, BatchSmem& sm)
+template <typename T, typename SM>
+__device__ __forceinline__ void fixed_units(const BatchItem& it, uint32_t w, SM& sm)
 {
   const uint32_t end = it.unit0 + it.n_units;
   for (uint32_t u = it.unit0 + w; u < end; u += WAVES_PER_BLOCK) {
     if (it.kind == BK_BITPACKING) {
       const uint32_t grc = it.row_count - u * BP_GROUP < BP_GROUP ? it.row_count - u * BP_GROUP : BP_GROUP;
-      bp_group<T>(it.seg, it.bytes_size, u, grc, reinterpret_cast<T*>(it.out) + size_t{u} * BP_GROUP, sm.fixed[w]);
+      bp_group<T, BATCH_BP_STAGE>(it.seg, it.bytes_size, u, grc, reinterpret_cast<T*>(it.out) + size_t{u} * BP_GROUP,
+                                  sm.fixed[w]);
     } else if constexpr (sizeof(T) >= 4) {
       const uint32_t fill = it.row_count - u * ALP_VEC < ALP_VEC ? it.row_count - u * ALP_VEC : ALP_VEC;
       using F = std::conditional_t<sizeof(T) == 8, double, float>;
@@ -42,11 +55,10 @@ __device__ __forceinline__ void fixed_units(const BatchItem& it, uint32_t w, Bat
   }
 }
 
-__global__ __launch_bounds__(BLOCK_THREADS) __attribute__((amdgpu_waves_per_eu(4))) void k_batch(const BatchItem* __restrict__ items,
-                                                                                          BatchQueue* __restrict__ q)
+template <bool STRINGS, typename SM>
+__device__ __forceinline__ void batch_loop(const BatchItem* __restrict__ items, BatchQueue* __restrict__ q, SM& sm,
+                                           uint32_t& s_idx)
 {
-  __shared__ BatchSmem sm;
-  __shared__ uint32_t s_idx;
   const uint32_t n = q->n_items;
   const uint32_t w = wave_in_block();
   for (;;) {
@@ -57,14 +69,15 @@ __global__ __launch_bounds__(BLOCK_THREADS) __attribute__((amdgpu_waves_per_eu(4
     if (idx >= n) break;
     const BatchItem it = items[idx];
     switch (it.kind) {
-      case BK_FSST: {
-        const FsstColumn col = gload(it.column);
-        const FsstChunk ch   = gload(col.chunks + it.chunk);
-        fsst_chunk<FSST_OUT_BATCH>(ch, gload(col.info + it.chunk), col.tabs + (ch.seg == FSST_GAP ? 0u : ch.seg), it.chunk,
-                                   col.n_chunks, col.state, col.offsets, col.chars, col.chars_cap, col.total_rows,
-                                   BATCH_SPIN_LIMIT, sm.fsst);
+      case BK_FSST:
+        if constexpr (STRINGS) {
+          const FsstColumn col = gload(it.column);
+          const FsstChunk ch   = gload(col.chunks + it.chunk);
+          fsst_chunk<FSST_OUT_BATCH>(ch, gload(col.info + it.chunk), col.tabs + (ch.seg == FSST_GAP ? 0u : ch.seg),
+                                     it.chunk, col.n_chunks, col.state, col.offsets, col.chars, col.chars_cap,
+                                     col.total_rows, BATCH_SPIN_LIMIT, sm.fsst);
+        }  // else unreachable: queues with VARCHAR entries run k_batch
         break;
-      }
       case BK_RLE:
         for (uint32_t u = it.unit0; u < it.unit0 + it.n_units; ++u) {
           const uint32_t sr = uint32_t{it.reserved} * 64u;
@@ -97,6 +110,22 @@ __global__ __launch_bounds__(BLOCK_THREADS) __attribute__((amdgpu_waves_per_eu(4
   }
 }
 
+__global__ __launch_bounds__(BLOCK_THREADS) __attribute__((amdgpu_waves_per_eu(4))) void k_batch(const BatchItem* __restrict__ items,
+                                                                                          BatchQueue* __restrict__ q)
+{
+  __shared__ BatchSmemAll sm;
+  __shared__ uint32_t s_idx;
+  batch_loop<true>(items, q, sm, s_idx);
+}
+
+__global__ __launch_bounds__(BLOCK_THREADS) void k_batch_fixed(const BatchItem* __restrict__ items,
+                                                               BatchQueue* __restrict__ q)
+{
+  __shared__ BatchSmemFixed sm;
+  __shared__ uint32_t s_idx;
+  batch_loop<false>(items, q, sm, s_idx);
+}
+
 __global__ void k_batch_reset(BatchQueue* q)
 {
   const uint32_t n = q->n_fsst_status;
@@ -127,18 +156,18 @@ hipError_t launch_batch_reset(BatchQueue* q, hipStream_t s)
   return hipGetLastError();
 }
 
-hipError_t launch_batch(const BatchItem* items, BatchQueue* q, uint32_t grid, hipStream_t s)
+hipError_t launch_batch(const BatchItem* items, BatchQueue* q, uint32_t grid, bool strings, hipStream_t s)
 {
-  hipLaunchKernelGGL(k_batch, dim3(grid), dim3(BLOCK_THREADS), 0, s, items, q);
+  if (strings) hipLaunchKernelGGL(k_batch, dim3(grid), dim3(BLOCK_THREADS), 0, s, items, q);
+  else hipLaunchKernelGGL(k_batch_fixed, dim3(grid), dim3(BLOCK_THREADS), 0, s, items, q);
   return hipGetLastError();
 }
 
-int batch_occupancy_blocks_per_cu()
+int batch_occupancy_blocks_per_cu(bool strings)
 {
   int blocks = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, reinterpret_cast<const void*>(k_batch), BLOCK_THREADS, 0) !=
-      hipSuccess)
-    return 1;
+  const void* k = strings ? reinterpret_cast<const void*>(k_batch) : reinterpret_cast<const void*>(k_batch_fixed);
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k, BLOCK_THREADS, 0) != hipSuccess) return 1;
   return blocks > 0 ? blocks : 1;
 }
 

@@ -52,6 +52,7 @@ struct sdr_batch {
   hipGraph_t graph     = nullptr;
   uint8_t* graph_arena = nullptr;
   uint32_t graph_grid  = 0;
+  bool graph_strings   = false;
   hipStream_t last_stream = nullptr;
   bool launched           = false;  // a launch whose results are not collected yet
   ~sdr_batch();
@@ -305,18 +306,20 @@ int batch_launch(sdr_batch* b, hipStream_t s)
   if (rc) return rc;
   auto* q     = reinterpret_cast<BatchQueue*>(b->d_arena);
   auto* items = reinterpret_cast<const BatchItem*>(b->d_arena + b->off_items);
+  const bool strings = std::any_of(b->items.begin(), b->items.end(),
+                                   [](const BatchItem& it) { return it.kind == BK_FSST; });
   uint32_t grid = b->grid;
   if (grid == 0) {
     int dev = 0, cus = 0;
     HIP_TRY(hipGetDevice(&dev));
     HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-    grid = static_cast<uint32_t>(cus * batch_occupancy_blocks_per_cu());
+    grid = static_cast<uint32_t>(cus * batch_occupancy_blocks_per_cu(strings));
   }
   grid = std::max<uint32_t>(1, std::min<uint32_t>(grid, std::max<uint32_t>(1, static_cast<uint32_t>(b->items.size()))));
   if (b->use_graph) {
     // the graph's arguments are the arena addresses; entries change through
     // the upload above, so one instantiation replays every batch
-    if (!b->gexec || b->graph_arena != b->d_arena || b->graph_grid != grid) {
+    if (!b->gexec || b->graph_arena != b->d_arena || b->graph_grid != grid || b->graph_strings != strings) {
       if (b->gexec) HIP_TRY(hipGraphExecDestroy(b->gexec));
       if (b->graph) HIP_TRY(hipGraphDestroy(b->graph));
       b->gexec = nullptr;
@@ -325,8 +328,8 @@ int batch_launch(sdr_batch* b, hipStream_t s)
       HIP_TRY(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
       hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
       if (e == hipSuccess) e = launch_batch_reset(q, cs);
-      if (e == hipSuccess) e = launch_batch_fsst_prep(q, cs);
-      if (e == hipSuccess) e = launch_batch(items, q, grid, cs);
+      if (e == hipSuccess && strings) e = launch_batch_fsst_prep(q, cs);
+      if (e == hipSuccess) e = launch_batch(items, q, grid, strings, cs);
       hipGraph_t g = nullptr;
       hipError_t e2 = hipStreamEndCapture(cs, &g);
       (void)hipStreamDestroy(cs);
@@ -337,13 +340,14 @@ int batch_launch(sdr_batch* b, hipStream_t s)
       b->graph = g;
       HIP_TRY(hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0));
       b->graph_arena = b->d_arena;
-      b->graph_grid  = grid;
+      b->graph_grid    = grid;
+      b->graph_strings = strings;
     }
     HIP_TRY(hipGraphLaunch(b->gexec, s));
   } else {
     HIP_TRY(launch_batch_reset(q, s));
-    HIP_TRY(launch_batch_fsst_prep(q, s));
-    HIP_TRY(launch_batch(items, q, grid, s));
+    if (strings) HIP_TRY(launch_batch_fsst_prep(q, s));
+    HIP_TRY(launch_batch(items, q, grid, strings, s));
   }
   // validity masks (UNCOMPRESSED overlays) after the data decode
   auto* d_nulls = reinterpret_cast<unsigned long long*>(b->d_arena + b->off_nulls);

@@ -74,8 +74,10 @@ hipError_t launch_batch_reset(BatchQueue* q, hipStream_t s);
 // FSST segment pre-pass over the queue's segments (a fixed grid that loops,
 // so one recorded launch serves every batch).
 hipError_t launch_batch_fsst_prep(BatchQueue* q, hipStream_t s);
-hipError_t launch_batch(const BatchItem* items, BatchQueue* q, uint32_t grid, hipStream_t s);
+// `strings`: the queue holds VARCHAR (BK_FSST) entries; without them a
+// kernel that carries no FSST state runs, at higher occupancy.
+hipError_t launch_batch(const BatchItem* items, BatchQueue* q, uint32_t grid, bool strings, hipStream_t s);
 // Blocks per CU the persistent kernel can keep resident.
-int batch_occupancy_blocks_per_cu();
+int batch_occupancy_blocks_per_cu(bool strings);
 
 }  // namespace sdr
