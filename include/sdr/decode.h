@@ -215,9 +215,9 @@ typedef enum sdr_batch_option {
   SDR_BATCH_USE_GRAPH          = 1,  /* 1 (default): replay a captured hipGraph     */
   SDR_BATCH_GRID               = 2,  /* blocks; 0 (default): resident capacity     */
   SDR_BATCH_UNITS_BITPACKING   = 16, /* work units per queue entry, per codec:     */
-  SDR_BATCH_UNITS_RLE          = 17, /*   2048-row groups, 16384-row RLE chunks,   */
-  SDR_BATCH_UNITS_ALP          = 18, /*   1024-row vectors, 16 KiB byte ranges     */
-  SDR_BATCH_UNITS_ALPRD        = 19,
+  SDR_BATCH_UNITS_RLE          = 17, /*   2048-row groups (default 16), RLE slices */
+  SDR_BATCH_UNITS_ALP          = 18, /*   of rle_slice_rows (2), 1024-row vectors  */
+  SDR_BATCH_UNITS_ALPRD        = 19, /*   (16 each), 16 KiB byte ranges (4 each)   */
   SDR_BATCH_UNITS_UNCOMPRESSED = 20,
   SDR_BATCH_UNITS_CONSTANT     = 21
 } sdr_batch_option;
