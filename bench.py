@@ -289,8 +289,11 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--rows", type=int, default=None)
     ap.add_argument("--workload", choices=WORKLOADS, default="int_date_mix_64M")
-    ap.add_argument("--only", choices=("all", "batched"), default="all",
-                    help="batched: the drop-in steps and the per-kernel loop only (for profiler runs)")
+    ap.add_argument("--only", choices=("all", "batched", "served"), default="all",
+                    help="batched: the drop-in steps and the per-kernel loop only; served: the served "
+                         "mixed-codec stream only (both for profiler runs)")
+    ap.add_argument("--no-served-baselines", action="store_true",
+                    help="serve the stream through the persistent path only")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the other workloads")
     ap.add_argument("--served-requests", type=int, default=10000,
@@ -310,6 +313,10 @@ def main() -> int:
     lib = L.load()
     tuning = apply_tuning(lib)
     dev = torch.device("cuda:0")
+    if args.only == "served":
+        sr = served(lib, args.served_requests, baselines=not args.no_served_baselines)
+        print(json.dumps({"workload": "served", **sr}), flush=True)
+        return 0
     work = build_workload(args.workload, args.rows)
 
     if args.only == "batched":
@@ -474,7 +481,7 @@ def staged(lib, work, dev, steps):
             "note": "file in the page cache; host-clock timing around sdr_stage_file + sdr_decode_table"}
 
 
-def served(lib, n_requests):
+def served(lib, n_requests, baselines=True):
     """The served mixed-codec stream: requests of log-uniform size
     (2 Ki..16 Mi rows), 40% BITPACKING INT32 / 20% RLE INT64 / 25% ALP DOUBLE
     / 15% FSST VARCHAR over segments resident in the server's pool, Poisson
@@ -518,8 +525,8 @@ def served(lib, n_requests):
         for f in main["failed"]:
             log(f"[bench] served MISMATCH {f}")
         base = {}
-        for name, mode, nb in (("one_launch_per_request", A.LAUNCH_BATCHED, n),
-                               ("one_launch_per_segment", A.LAUNCH_PER_SEGMENT, n)):
+        for name, mode, nb in ((("one_launch_per_request", A.LAUNCH_BATCHED, n),
+                                ("one_launch_per_segment", A.LAUNCH_PER_SEGMENT, n)) if baselines else ()):
             bs = copy.deepcopy(run_specs[:nb])
             S.poisson_arrivals(bs, rate)
             base[name] = S.run(lib, pool, bs, mode, verify_every=8)
@@ -536,8 +543,9 @@ def served(lib, n_requests):
         "persistent": rounded(main),
         "frac_hbm": round(main["achieved_gbs"] / HBM_PEAK_GBS, 4),
         "baselines": {k: rounded(v) for k, v in base.items()},
-        "vs_one_launch_per_request": round(main["decoded_gbs"] / base["one_launch_per_request"]["decoded_gbs"], 3),
-        "vs_one_launch_per_segment": round(main["decoded_gbs"] / base["one_launch_per_segment"]["decoded_gbs"], 3),
+        **({"vs_one_launch_per_request": round(main["decoded_gbs"] / base["one_launch_per_request"]["decoded_gbs"], 3),
+            "vs_one_launch_per_segment": round(main["decoded_gbs"] / base["one_launch_per_segment"]["decoded_gbs"], 3)}
+           if baselines else {}),
     }
 
 

@@ -334,8 +334,13 @@ typedef enum sdr_tune_key {
                                        in turn, 1..16 (1)                    */
   SDR_TUNE_BP_CHUNK_ROWS       = 3, /* most rows a wavefront stages per LDS
                                        round: 512, 1024 or 2048 (2048)       */
-  SDR_TUNE_ALP_VECTORS_PER_WAVE = 4 /* 1024-row ALP/ALPRD vectors one
+  SDR_TUNE_ALP_VECTORS_PER_WAVE = 4, /* 1024-row ALP/ALPRD vectors one
                                        wavefront decodes in turn, 1..16 (1)  */
+  SDR_TUNE_FSST_SPIN_LIMIT     = 5  /* sleeps a VARCHAR chunk waits for its
+                                       predecessors' chars offsets before its
+                                       decode reports SDR_E_HIP, 0..2^31
+                                       (2^24, never reached); 0 reports at
+                                       once without waiting (for tests)      */
 } sdr_tune_key;
 
 /* Set a knob; SDR_E_INVALID_ARG for an unknown key or a value out of range. */
@@ -343,8 +348,8 @@ int sdr_tune_set(uint32_t key, uint64_t value);
 /* Current value of a knob, 0 for an unknown key. */
 uint64_t sdr_tune_get(uint32_t key);
 /* Apply a table of "name value" lines (names: rle_slice_rows,
- * bp_groups_per_wave, bp_chunk_rows, alp_vectors_per_wave; '#' starts a
- * comment).  Stops at the first bad line with SDR_E_INVALID_ARG. */
+ * bp_groups_per_wave, bp_chunk_rows, alp_vectors_per_wave, fsst_spin_limit;
+ * '#' starts a comment).  Stops at the first bad line with SDR_E_INVALID_ARG. */
 int sdr_tune_load(const char* path);
 
 /* ---- string decode ----------------------------------------------------- */

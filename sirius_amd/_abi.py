@@ -49,8 +49,10 @@ TUNE_RLE_SLICE_ROWS = 1
 TUNE_BP_GROUPS_PER_WAVE = 2
 TUNE_BP_CHUNK_ROWS = 3
 TUNE_ALP_VECTORS_PER_WAVE = 4
+TUNE_FSST_SPIN_LIMIT = 5
 TUNE_NAMES = {TUNE_RLE_SLICE_ROWS: "rle_slice_rows", TUNE_BP_GROUPS_PER_WAVE: "bp_groups_per_wave",
-              TUNE_BP_CHUNK_ROWS: "bp_chunk_rows", TUNE_ALP_VECTORS_PER_WAVE: "alp_vectors_per_wave"}
+              TUNE_BP_CHUNK_ROWS: "bp_chunk_rows", TUNE_ALP_VECTORS_PER_WAVE: "alp_vectors_per_wave",
+              TUNE_FSST_SPIN_LIMIT: "fsst_spin_limit"}
 
 
 class RowRange(C.Structure):

@@ -645,7 +645,10 @@ __device__ __forceinline__ bool fsst_emit_staged(SM& sm, uint32_t rows, uint32_t
 
 // Decoupled look-back for the column-wide chars offset of chunk c, by wave
 // 0: publish this chunk's aggregate T, then sum predecessors 64 at a time
-// back to the nearest published inclusive prefix.  A status word carries
+// back to the nearest published inclusive prefix.  A chunk waits at most
+// `spin_limit` sleeps for an unpublished predecessor; past that (or at once
+// when spin_limit is 0, which only tests set) it sets the column's error
+// flag and ends, so the host reports the failure and no wave hangs.  A status word carries
 // its flag and value together and is the only datum exchanged between
 // chunks, so device-scope relaxed atomics (coherent across the XCDs' L2s)
 // suffice; release/acquire orderings would add an L2 write-back /
@@ -660,6 +663,8 @@ __device__ __forceinline__ uint64_t fsst_lookback(uint32_t c, uint32_t n_chunks,
     uint64_t excl64     = 0;
     if (c == 0) {
       if (lane == 0) __hip_atomic_store(&status[0], ST_INC | T, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    } else if (spin_limit == 0) {  // no waiting allowed: report the look-back as failed
+      if (lane == 0) __hip_atomic_fetch_or(gbl(&st->error), 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     } else {
       if (lane == 0) __hip_atomic_store(&status[c], ST_AGG | T, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       uint32_t p = c - 1, spins = 0;

@@ -16,8 +16,6 @@
 
 namespace sdr {
 
-constexpr uint32_t BATCH_SPIN_LIMIT = FSST_SPIN_LIMIT;
-
 // Per-wave staging of the fixed-width codecs: 4 KiB of bitpacking or ALP
 // codes, or the ALP-RD left stream plus 4 KiB of its right stream.
 constexpr uint32_t BATCH_BP_STAGE    = 4096;
@@ -59,8 +57,9 @@ template <bool STRINGS, typename SM>
 __device__ __forceinline__ void batch_loop(const BatchItem* __restrict__ items, BatchQueue* __restrict__ q, SM& sm,
                                            uint32_t& s_idx)
 {
-  const uint32_t n = q->n_items;
-  const uint32_t w = wave_in_block();
+  const uint32_t n    = q->n_items;
+  const uint32_t spin = q->spin_limit;
+  const uint32_t w    = wave_in_block();
   for (;;) {
     if (threadIdx.x == 0) s_idx = atomicAdd(&q->claimed, 1u);
     __syncthreads();
@@ -75,7 +74,7 @@ __device__ __forceinline__ void batch_loop(const BatchItem* __restrict__ items, 
           const FsstChunk ch   = gload(col.chunks + it.chunk);
           fsst_chunk<FSST_OUT_BATCH>(ch, gload(col.info + it.chunk), col.tabs + (ch.seg == FSST_GAP ? 0u : ch.seg),
                                      it.chunk, col.n_chunks, col.state, col.offsets, col.chars, col.chars_cap,
-                                     col.total_rows, BATCH_SPIN_LIMIT, sm.fsst);
+                                     col.total_rows, spin, sm.fsst);
         }  // else unreachable: queues with VARCHAR entries run k_batch
         break;
       case BK_RLE:

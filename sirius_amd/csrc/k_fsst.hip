@@ -1,6 +1,7 @@
 // k_fsst.hip — per-column FSST decode kernel (one launch decodes every chunk
 // of a VARCHAR column; see dec_fsst.hpp for the chunk decoder).
 #include "dec_fsst.hpp"
+#include "sdr_tune.hpp"
 
 namespace sdr {
 
@@ -72,7 +73,7 @@ hipError_t launch_fsst(const FsstChunk* d_chunks, uint32_t first, uint32_t n, ui
   // runs after every launch of lower chunks on the same stream.
   (void)first;
   hipLaunchKernelGGL(k_fsst, dim3(n), dim3(FSST_THREADS), 0, s, d_chunks, n_chunks_total, d_info, d_tabs, d_state,
-                     d_offsets, d_chars, chars_cap, total_rows, FSST_SPIN_LIMIT);
+                     d_offsets, d_chars, chars_cap, total_rows, tune(SDR_TUNE_FSST_SPIN_LIMIT));
   return hipGetLastError();
 }
 

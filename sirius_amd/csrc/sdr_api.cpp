@@ -18,22 +18,21 @@
 namespace sdr {
 
 namespace {
-thread_local std::string g_err;
+// A fixed buffer: reporting an error never allocates.
+thread_local char g_err[512];
 }
 
 int fail(int code, const char* fmt, ...)
 {
-  char buf[512];
   va_list ap;
   va_start(ap, fmt);
-  std::vsnprintf(buf, sizeof(buf), fmt, ap);
+  std::vsnprintf(g_err, sizeof(g_err), fmt, ap);
   va_end(ap);
-  g_err = buf;
   return code;
 }
 
-void clear_error() { g_err.clear(); }
-const char* last_error() { return g_err.c_str(); }
+void clear_error() { g_err[0] = '\0'; }
+const char* last_error() { return g_err; }
 
 // The stream-ordered allocator keeps freed blocks cached instead of
 // returning them to the driver at every synchronisation.

@@ -23,6 +23,7 @@ struct sdr_batch {
     uint64_t cap;
     uint32_t total_rows;
     uint32_t item_first;  // first queue entry of this column
+    int status = SDR_OK;  // this column's outcome, set by batch_collect
   };
   std::vector<Col> cols;
   struct MaskJob {
@@ -166,9 +167,6 @@ int batch_add_strings(sdr_batch* b, sdr_string_column* col)
       if (col->data[r].segments[k].row_count) segs.push_back({&col->data[r].segments[k], col->data[r].codec});
   std::stable_sort(segs.begin(), segs.end(),
                    [](const auto& a, const auto& c) { return a.first->row_offset < c.first->row_offset; });
-  for (size_t i = 1; i < segs.size(); ++i)
-    if (segs[i].first->row_offset < segs[i - 1].first->row_offset + segs[i - 1].first->row_count)
-      return fail(SDR_E_BOUNDS, "string segments overlap at row %u", segs[i].first->row_offset);
   auto add_range = [&](const uint8_t* bytes, uint32_t size, uint32_t col_row, uint32_t rows, uint32_t codec) {
     bc.segs.push_back({bytes, size, rows, bc.n_chunks, col_row, codec, 0});
     bc.n_chunks += fsst_chunk_count(rows);
@@ -257,6 +255,7 @@ int batch_upload(sdr_batch* b, hipStream_t s)
   q.n_fsst_status = static_cast<uint32_t>(state_words);
   q.fsst_status   = reinterpret_cast<uint64_t*>(d + b->off_state);
   q.n_fsst_segs   = static_cast<uint32_t>(n_segs);
+  q.spin_limit    = tune(SDR_TUNE_FSST_SPIN_LIMIT);
   q.fsst_segs     = reinterpret_cast<const FsstSeg*>(d + b->off_segs);
   q.fsst_chunks   = reinterpret_cast<FsstChunk*>(d + b->off_chunks);
   q.fsst_info     = reinterpret_cast<FsstChunkInfo*>(d + off_info);
@@ -353,6 +352,9 @@ int batch_launch(sdr_batch* b, hipStream_t s)
     if (strings) HIP_TRY(launch_batch_fsst_prep(q, s));
     HIP_TRY(launch_batch(items, q, grid, strings, s));
   }
+  // a column of no rows has no chunk to write its closing offset
+  for (const auto& c : b->cols)
+    if (c.total_rows == 0 && c.offsets) HIP_TRY(launch_fill_u8(reinterpret_cast<uint8_t*>(c.offsets), 0, sizeof(int32_t), s));
   // validity masks (UNCOMPRESSED overlays) after the data decode
   auto* d_nulls = reinterpret_cast<unsigned long long*>(b->d_arena + b->off_nulls);
   if (!b->masks.empty()) {
@@ -402,16 +404,27 @@ int batch_collect(sdr_batch* b)
     const uint64_t total = st[w + 1];
     w += 2 + c.n_chunks;
     c.user->chars_used = total;
-    if (err && !rc) rc = fail(SDR_E_HIP, "string decode: offset look-back did not complete");
-    if (!rc && (total > 0x7FFFFFFFull || total > c.cap))
-      rc = fail(SDR_E_OVERFLOW, "total_chars (%llu) exceeds capacity (%llu) or int32 max",
-                static_cast<unsigned long long>(total), static_cast<unsigned long long>(c.cap));
+    c.status           = err ? SDR_E_HIP : (total > 0x7FFFFFFFull || total > c.cap) ? SDR_E_OVERFLOW : SDR_OK;
+    if (c.status && !rc)
+      rc = c.status == SDR_E_HIP
+             ? fail(SDR_E_HIP, "string decode: offset look-back did not complete")
+             : fail(SDR_E_OVERFLOW, "total_chars (%llu) exceeds capacity (%llu) or int32 max",
+                    static_cast<unsigned long long>(total), static_cast<unsigned long long>(c.cap));
   }
   for (size_t i = 0; i < b->masks.size(); ++i) {
     if (b->masks[i].fixed) b->masks[i].fixed->null_count = static_cast<int64_t>(nulls[i]);
     else b->masks[i].str->null_count = static_cast<int64_t>(nulls[i]);
   }
   return rc;
+}
+
+// Outcome of one string column of the last collected launch (SDR_OK for a
+// column the batch does not hold).
+int batch_column_status(const sdr_batch* b, const sdr_string_column* col)
+{
+  for (const auto& c : b->cols)
+    if (c.user == col) return c.status;
+  return SDR_OK;
 }
 
 // Wait for the last launch and write back string totals and null counts.

@@ -63,7 +63,7 @@ struct BatchQueue {
   uint32_t error;
   uint64_t* fsst_status;   // contiguous FsstState storage of all FSST columns
   uint32_t n_fsst_segs;    // row ranges of all FSST columns, for the pre-pass
-  uint32_t pad;
+  uint32_t spin_limit;     // look-back wait bound of the FSST chunks
   const FsstSeg* fsst_segs;      // their first_chunk index fsst_chunks and fsst_info
   FsstChunk* fsst_chunks;
   FsstChunkInfo* fsst_info;

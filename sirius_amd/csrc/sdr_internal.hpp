@@ -57,6 +57,7 @@ int batch_add_strings(sdr_batch* b, sdr_string_column* col);
 int batch_launch(sdr_batch* b, hipStream_t s);
 int batch_ready(sdr_batch* b);    // 1 finished, 0 running, < 0 error
 int batch_collect(sdr_batch* b);  // after batch_ready: write back totals and null counts
+int batch_column_status(const sdr_batch* b, const sdr_string_column* col);
 
 // Segments grouped by (codec, type size); [first, first + count) in the
 // descriptor array.

@@ -117,11 +117,17 @@ int allocate_outputs(sdr_server* sv, Req* q, hipStream_t s)
   return SDR_OK;
 }
 
-void complete(sdr_server* sv, Slot& sl, int rc)
+// Finish every request of a slot.  `rc` is the launch's own outcome (a HIP
+// failure applies to all of its requests); a persistent launch's string
+// requests then take their own column's outcome, so one request's overflow
+// never marks the others of its batch.
+void complete(sdr_server* sv, Slot& sl, int rc, bool timed = true)
 {
-  const double t_start = sv->gpu_ms(sl.ev_start), t_end = sv->gpu_ms(sl.ev_end);
+  const double t_start = timed ? sv->gpu_ms(sl.ev_start) : sv->now_ms();
+  const double t_end   = timed ? sv->gpu_ms(sl.ev_end) : t_start;
   for (Req* q : sl.reqs) {
     int st = rc;
+    if (st == SDR_OK && sl.batch && q->r.is_string) st = batch_column_status(sl.batch, &q->scol);
     if (q->r.is_string && st == SDR_OK && q->scol.chars_used > q->chars_cap) st = SDR_E_OVERFLOW;
     q->status = st;
     q->done   = true;
@@ -151,7 +157,7 @@ int collect(sdr_server* sv)
       const int r = batch_ready(sl.batch);
       if (r < 0) return r;
       if (r == 0) continue;
-      rc = batch_collect(sl.batch);
+      (void)batch_collect(sl.batch);  // per-column outcomes: batch_column_status
     } else {
       const hipError_t e = hipEventQuery(sl.ev_end);
       if (e == hipErrorNotReady) continue;
@@ -175,6 +181,11 @@ void admit(sdr_server* sv)
   }
 }
 
+int launch_slot(sdr_server* sv, Slot& sl);
+
+// Move admitted requests into a free slot and launch them.  A failure before
+// the launch is enqueued completes every request of the slot with that
+// error (their outputs go back to the pool), so the server keeps serving.
 int launch(sdr_server* sv, Slot& sl, uint32_t* n_launched)
 {
   const uint32_t cap = sv->cfg.mode == SDR_LAUNCH_PERSISTENT
@@ -184,6 +195,28 @@ int launch(sdr_server* sv, Slot& sl, uint32_t* n_launched)
     sl.reqs.push_back(sv->admitted.front());
     sv->admitted.pop_front();
   }
+  const int rc = launch_slot(sv, sl);
+  if (rc == SDR_OK) {
+    sl.busy = true;
+    ++sv->running;
+    if (n_launched) ++*n_launched;
+    return SDR_OK;
+  }
+  for (Req* q : sl.reqs) {
+    if (q->d_alloc) (void)hipFreeAsync(q->d_alloc, sl.s);
+    q->d_alloc        = nullptr;
+    q->col.d_out      = nullptr;
+    q->scol.d_offsets = nullptr;
+    q->scol.d_chars   = nullptr;
+  }
+  if (sl.batch) (void)sdr_batch_clear(sl.batch);
+  ++sv->running;  // complete() counts the slot down
+  complete(sv, sl, rc, false);
+  return SDR_OK;
+}
+
+int launch_slot(sdr_server* sv, Slot& sl)
+{
   for (Req* q : sl.reqs) {
     int rc = allocate_outputs(sv, q, sl.s);
     if (rc) return rc;
@@ -206,9 +239,6 @@ int launch(sdr_server* sv, Slot& sl, uint32_t* n_launched)
   }
   if (rc) return rc;
   HIP_TRY(hipEventRecord(sl.ev_end, sl.s));
-  sl.busy = true;
-  ++sv->running;
-  if (n_launched) ++*n_launched;
   return SDR_OK;
 }
 

@@ -33,6 +33,16 @@ int validate_string_column(const sdr_string_column* col)
     if (col->total_rows && !col->d_null_mask) return fail(SDR_E_INVALID_ARG, "d_null_mask is NULL");
   }
   if (col->total_rows && !col->d_offsets) return fail(SDR_E_INVALID_ARG, "d_offsets is NULL");
+  // segments may come in any order but must not share rows
+  std::vector<std::pair<uint32_t, uint32_t>> spans;  // row_offset, row_count
+  for (uint32_t r = 0; r < col->n_data; ++r)
+    for (uint32_t k = 0; k < col->data[r].n_segments; ++k)
+      if (col->data[r].segments[k].row_count)
+        spans.push_back({col->data[r].segments[k].row_offset, col->data[r].segments[k].row_count});
+  std::sort(spans.begin(), spans.end());
+  for (size_t i = 1; i < spans.size(); ++i)
+    if (spans[i].first < spans[i - 1].first + spans[i - 1].second)
+      return fail(SDR_E_BOUNDS, "string segments overlap at row %u", spans[i].first);
   return SDR_OK;
 }
 
@@ -59,9 +69,6 @@ int decode_strings(sdr_string_column* col, hipStream_t s, uint32_t mode)
       if (col->data[r].segments[k].row_count) segs.push_back({&col->data[r].segments[k], col->data[r].codec});
   std::stable_sort(segs.begin(), segs.end(),
                    [](const auto& a, const auto& b) { return a.first->row_offset < b.first->row_offset; });
-  for (size_t i = 1; i < segs.size(); ++i)
-    if (segs[i].first->row_offset < segs[i - 1].first->row_offset + segs[i - 1].first->row_count)
-      return fail(SDR_E_BOUNDS, "string segments overlap at row %u", segs[i].first->row_offset);
 
   // row ranges: the segments and the gaps between them; the pre-pass
   // writes their chunk lists on the device

@@ -1,4 +1,5 @@
 // sdr_tune.cpp — process-wide launch-shape knobs (sdr_tune_*).
+#include "sdr_fsst.hpp"
 #include "sdr_internal.hpp"
 #include "sdr_tune.hpp"
 
@@ -20,6 +21,7 @@ Knob g_knobs[] = {
   {"bp_groups_per_wave", 1, 16, 1, {1}},
   {"bp_chunk_rows", 512, 2048, 2048, {2048}},
   {"alp_vectors_per_wave", 1, 16, 1, {1}},
+  {"fsst_spin_limit", 0, 1u << 31, FSST_SPIN_LIMIT, {FSST_SPIN_LIMIT}},
 };
 constexpr uint32_t N_KNOBS = sizeof(g_knobs) / sizeof(g_knobs[0]);
 

@@ -18,7 +18,7 @@ import struct
 
 from . import _abi as A
 from . import lib as L
-from .duckdb_file import BLOCK_HEADER, Column, DuckDBFile
+from .duckdb_file import BLOCK_HEADER, Column, DuckDBFile, FormatError
 
 # Ranges closer than this share a read (the reference passes the block
 # header size: segments split only by a block's checksum coalesce).
@@ -74,6 +74,14 @@ def scan_column(f: DuckDBFile, table: str, column: str, device, stream=None, max
     col = next(c for c in t.columns if c.name == column)
     n = t.rows
     ts = 0 if col.type.is_string else col.type.type_size
+    # Validity decodes from UNCOMPRESSED bytes or a CONSTANT statistic only.
+    # Any other codec (ROARING, which the reference decodes on the host with
+    # DuckDB's own scan state, duckdb_native_decoder.cpp:194-243) is refused
+    # rather than read as all-valid.
+    for s in col.validity:
+        if s.codec not in (A.UNCOMPRESSED, A.CONSTANT):
+            raise FormatError(f"{column}: validity segment at row {s.row_start} uses {s.codec_name} "
+                              f"(codec {s.codec}), which this decoder does not read")
     # bytes from the file, and host-built tails: CONSTANT values, all-NULL validity
     file_segs = [s for s in col.segments if s.codec != 2] + [s for s in col.validity if s.codec == 1]
     tail = bytearray()

@@ -181,6 +181,8 @@ def main():
             ts = col.type.type_size
             name = f"tpcds_{column}"
             blob, tab, st = pack(f, col.segments, 0, ts, "string" if col.type.is_string else "fixed")
+            if any(s.codec not in (1, 2) for s in col.validity):  # only UNCOMPRESSED bytes / CONSTANT stats
+                raise ValueError(f"{table}.{column}: validity codecs {sorted({s.codec_name for s in col.validity})}")
             valid = [s for s in col.validity if s.codec == 1]
             vblob, vtab, _ = pack(f, valid, 0, ts, "validity")
             z[f"{name}/blob"], z[f"{name}/table"], z[f"{name}/stats"] = blob, tab, st

@@ -93,3 +93,91 @@ def test_admission_holds_budget(setup):
     finally:
         torch.cuda.synchronize()
         lib.sdr_server_destroy(sv)
+
+
+def _drain(lib, sv, n_expect, stream):
+    """Progress and poll until n_expect completions: {tag: Completion copy}."""
+    comps = (A.Completion * 64)()
+    n = C.c_uint32()
+    out = {}
+    while len(out) < n_expect:
+        L.check(lib.sdr_server_progress(sv, None))
+        L.check(lib.sdr_server_poll(sv, comps, 64, C.byref(n)))
+        for k in range(n.value):
+            c = A.Completion()
+            C.pointer(c)[0] = comps[k]
+            out[c.tag] = c
+    return out
+
+
+def test_overflow_is_reported_on_its_own_request(setup):
+    """One VARCHAR request with too small a chars_capacity, coalesced into one
+    persistent launch with good requests: only it reports SDR_E_OVERFLOW and
+    the others' outputs are bit-exact."""
+    lib, _, pool = setup
+    specs = S.make_specs(pool, 40, seed=31, max_rows=1 << 20)
+    strs = [s for s in specs if pool.src[s.kind].is_string]
+    assert strs
+    bad = strs[0]
+    cfg = A.ServerConfig(0, 1, A.LAUNCH_PERSISTENT, 0, 0)
+    sv = C.c_void_p()
+    L.check(lib.sdr_server_create(C.byref(cfg), C.byref(sv)))
+    stream = torch.cuda.current_stream()
+    try:
+        reqs = {s.tag: s.request(pool) for s in specs}
+        reqs[bad.tag].chars_capacity = 7
+        for s in specs:
+            L.check(lib.sdr_server_submit(sv, C.byref(reqs[s.tag])))
+        got = _drain(lib, sv, len(specs), stream)
+        for s in specs:
+            c = got[s.tag]
+            if s.tag == bad.tag:
+                assert c.status == A.SDR_E_OVERFLOW and c.chars_used > 7
+            else:
+                assert c.status == 0 and S.verify(pool, s, c), s.tag
+            L.check(lib.sdr_server_release(sv, s.tag, stream.cuda_stream))
+    finally:
+        torch.cuda.synchronize()
+        lib.sdr_server_destroy(sv)
+
+
+def test_overlapping_string_segments_refused_at_submit(setup):
+    lib, server, pool = setup
+    spec = next(s for s in S.make_specs(pool, 200, seed=41, max_rows=1 << 20)
+                if pool.src[s.kind].is_string and s.n_src >= 2)
+    r = spec.request(pool)
+    s0, s1 = r.data[0].segments[0], r.data[0].segments[1]
+    s1.row_offset = s0.row_offset + s0.row_count - 1  # its first row is segment 0's last
+    assert lib.sdr_server_submit(server, C.byref(r)) == A.SDR_E_BOUNDS
+    assert lib.sdr_server_pending(server) == 0
+
+
+@pytest.mark.parametrize("mode", [A.LAUNCH_PERSISTENT, A.LAUNCH_BATCHED], ids=["persistent", "per_request"])
+def test_zero_row_string_request(setup, mode):
+    """A VARCHAR request of no rows completes with offsets[0] == 0."""
+    lib, _, pool = setup
+    cfg = A.ServerConfig(0, 1, mode, 0, 0)
+    sv = C.c_void_p()
+    L.check(lib.sdr_server_create(C.byref(cfg), C.byref(sv)))
+    stream = torch.cuda.current_stream()
+    try:
+        # the pool recycles memory: fill a block first so stale bytes would show
+        junk = S.make_specs(pool, 50, seed=3, max_rows=1 << 16)
+        first = next(s for s in junk if pool.src[s.kind].is_string)
+        L.check(lib.sdr_server_submit(sv, C.byref(first.request(pool))))
+        c0 = _drain(lib, sv, 1, stream)[first.tag]
+        assert c0.status == 0
+        L.check(lib.sdr_server_release(sv, first.tag, stream.cuda_stream))
+        runs, keep = A.make_runs([(A.FSST, [])])
+        r = A.Request()
+        r.tag, r.is_string, r.total_rows, r.n_data, r.data, r.chars_capacity = 999_001, 1, 0, 1, runs, 16
+        L.check(lib.sdr_server_submit(sv, C.byref(r)))
+        c = _drain(lib, sv, 1, stream)[999_001]
+        assert c.status == 0 and c.chars_used == 0
+        off0 = S._view(c.d_out, 4).view(torch.int32)
+        torch.cuda.synchronize()
+        assert int(off0.item()) == 0
+        L.check(lib.sdr_server_release(sv, 999_001, stream.cuda_stream))
+    finally:
+        torch.cuda.synchronize()
+        lib.sdr_server_destroy(sv)

@@ -199,3 +199,27 @@ def test_string_shapes_match_oracle(shape, mode):
     np.testing.assert_array_equal(g_offs, o_offs)
     assert col.chars_used == len(o_chars)
     np.testing.assert_array_equal(g_chars[: col.chars_used], o_chars)
+
+
+@pytest.mark.parametrize("mode", [A.LAUNCH_BATCHED, A.LAUNCH_PERSISTENT], ids=["batched", "persistent"])
+def test_lookback_give_up_reaches_the_caller(mode):
+    """With the look-back's wait bound forced to zero from the host, every
+    chunk after the first gives up at once: the kernels exit normally, the
+    call reports SDR_E_HIP, and with the bound restored the same column
+    decodes bit-exactly."""
+    lib = L.load()
+    chars, offs, enc = _zipf(20_000, 0xC0DE0044)
+    d, base = upload(enc.blob)
+    runs = [(A.FSST, enc.segments(base))]
+    old = lib.sdr_tune_get(A.TUNE_FSST_SPIN_LIMIT)
+    try:
+        L.check(lib.sdr_tune_set(A.TUNE_FSST_SPIN_LIMIT, 0))
+        rc, col, _, _ = decode_strings_gpu(runs, enc.n_rows, len(chars) + 64, mode)
+        assert rc == A.SDR_E_HIP
+        assert b"look-back did not complete" in lib.sdr_last_error()
+    finally:
+        L.check(lib.sdr_tune_set(A.TUNE_FSST_SPIN_LIMIT, old))
+    rc, col, g_offs, g_chars = decode_strings_gpu(runs, enc.n_rows, len(chars) + 64, mode)
+    L.check(rc)
+    np.testing.assert_array_equal(g_offs, offs.astype(np.int32))
+    np.testing.assert_array_equal(g_chars[: col.chars_used], chars)

@@ -111,3 +111,21 @@ def test_stage_file_errors(tmp_path):
     assert lib.sdr_stage_file(str(p).encode(), ext, 1, 8, C.c_void_p(d.data_ptr()), 4096, sh) == 0
     torch.cuda.synchronize()
     np.testing.assert_array_equal(d[:1000].cpu().numpy(), np.frombuffer(p.read_bytes()[:1000], np.uint8))
+
+
+def test_scan_column_refuses_roaring_validity(tmp_path):
+    """A validity segment in a codec the decoder does not read (ROARING, 13)
+    is refused before any device work instead of being read as all-valid."""
+    from sirius_amd.duckdb_file import DuckDBFile, FormatError
+    from sirius_amd.stage import scan_column
+    from tests.duckdb_writer import write_database
+
+    tables = [{"name": "t", "rows": 4096, "columns": [{
+        "name": "k", "type_id": 14,
+        "segments": [{"row_start": 0, "rows": 4096, "codec": 2, "min": 5, "max": 5}],
+        "validity": [{"row_start": 0, "rows": 4096, "codec": 13, "has_null": True, "bytes": bytes(64)}]}]}]
+    path = write_database(tmp_path / "roaring.duckdb", tables)
+    with DuckDBFile(str(path)) as f:
+        assert [s.codec for s in f.tables["t"].columns[0].validity] == [13]
+        with pytest.raises(FormatError, match="validity segment at row 0"):
+            scan_column(f, "t", "k", "cpu")
