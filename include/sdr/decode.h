@@ -242,12 +242,16 @@ int sdr_batch_destroy(sdr_batch* batch);
 
 /* A decode server: requests from concurrent queries arrive over time and
  * decode segments that stay resident on the device.  The server
- *   - stages new segment bytes from host memory through a pinned buffer
- *     into device memory it keeps (sdr_server_stage),
+ *   - keeps one device memory pool for resident segments and request
+ *     outputs: slabs reserved when the pool grows, carved on the host, and
+ *     ranges released behind an event that rejoin the pool once it has
+ *     passed, so a warm server makes no allocation call,
+ *   - stages new segment bytes from host memory through a ring of pinned
+ *     buffers with asynchronous copies on its own stream
+ *     (sdr_server_stage); its launches wait for those copies on the device,
  *   - admits queued requests in arrival order while resident bytes plus the
  *     outputs of admitted, unreleased requests fit the HBM budget (a request
  *     larger than the budget is admitted alone, when nothing else is held),
- *   - allocates each admitted request's outputs from a HIP memory pool,
  *   - and, in SDR_LAUNCH_PERSISTENT mode, coalesces every admitted request
  *     into the next persistent graph launch while up to `max_inflight`
  *     launches run on their own streams: requests that arrive while others
@@ -256,7 +260,8 @@ int sdr_batch_destroy(sdr_batch* batch);
  *     SDR_LAUNCH_PER_SEGMENT decode one request per launch slot with the
  *     drop-in entries (the baselines).
  * No counterpart in the reference (it decodes one split per call:
- * src/op/scan/duckdb_native_decoder.cpp:972-1217); each request's output is
+ * src/op/scan/duckdb_native_decoder.cpp:972-1217, staging each split's
+ * segments with its own copies at :712-855); each request's output is
  * what sdr_decode_table / sdr_decode_strings_column return for it.  A server
  * is used from one thread. */
 typedef struct sdr_server sdr_server;
@@ -294,10 +299,38 @@ typedef struct sdr_completion {
   double submit_ms, start_ms, done_ms;
 } sdr_completion;
 
+/* Where a server's time went since it was created: host milliseconds spent
+ * in each phase of its calls on the calling thread, and the GPU time of its
+ * launches (start to end event; `gpu_union_ms` counts time during which at
+ * least one launch ran once).  Decoded bytes over gpu_union_ms is the rate
+ * the device sustained while it had work. */
+typedef struct sdr_server_stats {
+  uint64_t launches;
+  uint64_t requests;          /* completed                                       */
+  uint64_t queue_entries;     /* persistent-queue entries launched               */
+  double host_submit_ms;      /* sdr_server_submit                               */
+  double host_collect_ms;     /* polling and completing launches                 */
+  double host_alloc_ms;       /* output allocation                               */
+  double host_launch_ms;      /* queue build, upload and kernel enqueue          */
+  double host_release_ms;     /* sdr_server_release                              */
+  double host_stage_ms;       /* sdr_server_stage / sdr_server_unstage           */
+  double gpu_busy_ms;         /* sum of launch durations                         */
+  double gpu_union_ms;        /* union of launch intervals                       */
+} sdr_server_stats;
+
 int sdr_server_create(const sdr_server_config* config, sdr_server** out);
-/* Copy n host bytes to a new device allocation the server keeps until
- * sdr_server_unstage (counted against the budget); *d_out receives it. */
+/* Phase times and launch counts since creation. */
+int sdr_server_get_stats(const sdr_server* server, sdr_server_stats* out);
+/* Copy n host bytes into the server's pool, where they stay until
+ * sdr_server_unstage (counted against the budget); *d_out receives the
+ * device address.  The host buffer may be reused when the call returns; the
+ * device copy is asynchronous: the server's own launches are ordered after
+ * it, any other user of *d_out calls sdr_server_stage_wait first. */
 int sdr_server_stage(sdr_server* server, const void* host_bytes, uint64_t n, const uint8_t** d_out);
+/* Make `stream` wait (on the device) for every staging copy issued so far. */
+int sdr_server_stage_wait(sdr_server* server, sdr_stream_t stream);
+/* Give a staged range back: it rejoins the pool once the launches in flight
+ * have finished.  Requests still queued must not reference it. */
 int sdr_server_unstage(sdr_server* server, const uint8_t* d_ptr);
 /* Queue a request (host only); SDR_E_CAPACITY if its outputs exceed the
  * budget left after the resident segments. */
@@ -308,7 +341,8 @@ int sdr_server_progress(sdr_server* server, uint32_t* n_launched);
 /* Completions since the last poll, at most `cap`; *n receives the count. */
 int sdr_server_poll(sdr_server* server, sdr_completion* out, uint32_t cap, uint32_t* n);
 /* Return a completed request's outputs to the pool, ordered after the work
- * already enqueued on `stream` (the consumer's stream). */
+ * already enqueued on `stream` (the consumer's stream): the range is reused
+ * only once that work has finished.  No HIP call is made here. */
 int sdr_server_release(sdr_server* server, uint64_t tag, sdr_stream_t stream);
 /* Requests queued, admitted-not-launched and in flight. */
 int sdr_server_pending(const sdr_server* server);

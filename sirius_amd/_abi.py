@@ -135,6 +135,25 @@ class ServerConfig(C.Structure):
     ]
 
 
+class ServerStats(C.Structure):
+    _fields_ = [
+        ("launches", C.c_uint64),
+        ("requests", C.c_uint64),
+        ("queue_entries", C.c_uint64),
+        ("host_submit_ms", C.c_double),
+        ("host_collect_ms", C.c_double),
+        ("host_alloc_ms", C.c_double),
+        ("host_launch_ms", C.c_double),
+        ("host_release_ms", C.c_double),
+        ("host_stage_ms", C.c_double),
+        ("gpu_busy_ms", C.c_double),
+        ("gpu_union_ms", C.c_double),
+    ]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 class Request(C.Structure):
     _fields_ = [
         ("tag", C.c_uint64),

@@ -2,6 +2,15 @@
 // HBM-budget admission, pool-allocated outputs and continuous batching over
 // persistent graph launches (or one drop-in launch per request, the
 // baselines).  Runs on the calling thread; progress() does all device work.
+//
+// Device memory (resident segments and request outputs) comes from one
+// server-owned pool (DeviceArena): slabs reserved with hipMalloc when it
+// grows, carved on the host.  A range that device work may still use is
+// freed behind an event and rejoins the pool once the event has passed, so
+// steady-state serving makes no allocation call.  New segments are staged
+// through a ring of pinned buffers with hipMemcpyAsync on a staging stream;
+// launches wait for the staging copies on the device, never on the host.
+#include "sdr_arena.hpp"
 #include "sdr_internal.hpp"
 
 #include <algorithm>
@@ -15,7 +24,10 @@
 namespace {
 
 constexpr uint64_t DEFAULT_BUDGET = 200ull * 1000 * 1000 * 1000;
-constexpr size_t STAGE_CHUNK      = 64u << 20;  // pinned staging buffer
+constexpr size_t STAGE_CHUNK      = 16u << 20;  // one pinned staging buffer
+constexpr uint32_t STAGE_RING     = 4;          // staging buffers in flight
+constexpr uint64_t SLAB_MIN       = 64ull << 20;
+constexpr uint64_t SLAB_MAX       = 4ull << 30;
 
 struct Req {
   sdr_request r{};
@@ -25,8 +37,7 @@ struct Req {
   sdr_string_column scol{};
   uint64_t out_bytes  = 0;
   uint64_t chars_cap  = 0;
-  uint8_t* d_alloc    = nullptr;
-  hipStream_t stream  = nullptr;  // launch stream the outputs were allocated on
+  uint8_t* d_alloc    = nullptr;  // outputs, from the server's pool
   double t_submit     = 0;
   int status          = SDR_OK;
   bool done           = false;
@@ -39,18 +50,44 @@ struct Slot {
   sdr_batch* batch     = nullptr;
   bool busy            = false;
   int launch_status    = SDR_OK;  // synchronous status of a drop-in launch
+  uint64_t stage_seen  = 0;       // staging copies this slot's stream already waits for
   std::vector<Req*> reqs;
 };
 
+// Pool ranges waiting for an event before they can be reused.
+struct Fenced {
+  std::vector<uint8_t*> ptrs;
+  hipEvent_t ev;
+};
+
 using Clock = std::chrono::steady_clock;
+
+// Adds the host time of a scope to one of the server's phase counters.
+struct PhaseTimer {
+  double& acc;
+  Clock::time_point t = Clock::now();
+  explicit PhaseTimer(double& a) : acc(a) {}
+  ~PhaseTimer() { acc += std::chrono::duration<double, std::milli>(Clock::now() - t).count(); }
+};
 
 }  // namespace
 
 struct sdr_server {
   sdr_server_config cfg{};
   int device            = 0;
-  hipMemPool_t pool     = nullptr;
-  hipStream_t aux       = nullptr;
+  sdr::DeviceArena arena;
+  uint64_t slab_bytes   = SLAB_MIN;
+  std::vector<hipEvent_t> spare_events;
+  std::deque<Fenced> fenced;  // released ranges behind an event
+  std::unordered_map<hipStream_t, std::vector<uint8_t*>> open_release;  // released, event not recorded yet
+  hipStream_t aux       = nullptr;  // staging copies
+  hipStream_t fence_s   = nullptr;  // fences that wait for in-flight launches
+  uint8_t* ring[STAGE_RING] = {};
+  hipEvent_t ring_ev[STAGE_RING] = {};
+  bool ring_used[STAGE_RING] = {};
+  uint32_t ring_next    = 0;
+  hipEvent_t stage_done = nullptr;  // recorded on aux after the last staging copy
+  uint64_t stage_seq    = 0;
   hipEvent_t t0_event   = nullptr;
   double t0_gpu_ms      = 0;  // server clock at t0_event
   Clock::time_point t0;
@@ -60,10 +97,11 @@ struct sdr_server {
   std::unordered_map<uint64_t, std::unique_ptr<Req>> reqs;
   std::vector<sdr_completion> done;
   std::unordered_map<const uint8_t*, uint64_t> staged;
-  uint8_t* h_stage      = nullptr;
   uint64_t resident     = 0;
   uint64_t outputs_held = 0;
   uint32_t running      = 0;
+  sdr_server_stats stats{};
+  double union_end = 0;  // end of the launch intervals counted in gpu_union_ms
 
   double now_ms() const { return std::chrono::duration<double, std::milli>(Clock::now() - t0).count(); }
   double gpu_ms(hipEvent_t ev) const
@@ -77,22 +115,31 @@ struct sdr_server {
 
 sdr_server::~sdr_server()
 {
+  // everything that may still touch the pool's memory finishes first; the
+  // arena's destructor then returns the slabs
   for (auto& sl : slots) {
     if (sl.s) (void)hipStreamSynchronize(sl.s);
     if (sl.batch) (void)sdr_batch_destroy(sl.batch);
     if (sl.ev_start) (void)hipEventDestroy(sl.ev_start);
     if (sl.ev_end) (void)hipEventDestroy(sl.ev_end);
   }
-  for (auto& kv : reqs)
-    if (kv.second->d_alloc) (void)hipFreeAsync(kv.second->d_alloc, aux);
+  for (auto& f : fenced) {
+    (void)hipEventSynchronize(f.ev);
+    (void)hipEventDestroy(f.ev);
+  }
   if (aux) (void)hipStreamSynchronize(aux);
-  for (auto& kv : staged) (void)hipFree(const_cast<uint8_t*>(kv.first));
+  if (fence_s) (void)hipStreamSynchronize(fence_s);
+  for (hipEvent_t e : spare_events) (void)hipEventDestroy(e);
+  for (uint32_t i = 0; i < STAGE_RING; ++i) {
+    if (ring_ev[i]) (void)hipEventDestroy(ring_ev[i]);
+    if (ring[i]) (void)hipHostFree(ring[i]);
+  }
+  if (stage_done) (void)hipEventDestroy(stage_done);
   for (auto& sl : slots)
     if (sl.s) (void)hipStreamDestroy(sl.s);
   if (aux) (void)hipStreamDestroy(aux);
+  if (fence_s) (void)hipStreamDestroy(fence_s);
   if (t0_event) (void)hipEventDestroy(t0_event);
-  if (pool) (void)hipMemPoolDestroy(pool);
-  if (h_stage) (void)hipHostFree(h_stage);
 }
 
 namespace sdr {
@@ -100,12 +147,83 @@ namespace {
 
 uint64_t held(const sdr_server* sv) { return sv->resident + sv->outputs_held; }
 
-// Allocate the request's outputs from the pool on `s` and point its column
-// struct at them.
-int allocate_outputs(sdr_server* sv, Req* q, hipStream_t s)
+int get_event(sdr_server* sv, hipEvent_t* ev)
 {
-  HIP_TRY(hipMallocFromPoolAsync(reinterpret_cast<void**>(&q->d_alloc), std::max<uint64_t>(q->out_bytes, 256), sv->pool, s));
-  q->stream = s;
+  if (!sv->spare_events.empty()) {
+    *ev = sv->spare_events.back();
+    sv->spare_events.pop_back();
+    return SDR_OK;
+  }
+  HIP_TRY(hipEventCreateWithFlags(ev, hipEventDisableTiming));
+  return SDR_OK;
+}
+
+// Record one event per consumer stream behind the ranges released on it
+// since the last call (releases are grouped, so a release costs no HIP call).
+int close_releases(sdr_server* sv)
+{
+  for (auto& kv : sv->open_release) {
+    if (kv.second.empty()) continue;
+    hipEvent_t ev = nullptr;
+    int rc        = get_event(sv, &ev);
+    if (rc) return rc;
+    HIP_TRY(hipEventRecord(ev, kv.first));
+    sv->fenced.push_back({std::move(kv.second), ev});
+    kv.second.clear();
+  }
+  return SDR_OK;
+}
+
+// Return every fenced range whose event has passed to the pool; with
+// `wait_one`, first wait for the oldest fence.
+int reclaim(sdr_server* sv, bool wait_one)
+{
+  if (wait_one && !sv->fenced.empty()) HIP_TRY(hipEventSynchronize(sv->fenced.front().ev));
+  for (auto it = sv->fenced.begin(); it != sv->fenced.end();) {
+    const hipError_t e = hipEventQuery(it->ev);
+    if (e == hipErrorNotReady) {
+      ++it;
+      continue;
+    }
+    if (e != hipSuccess) return fail(SDR_E_HIP, "pool fence: %s", hipGetErrorString(e));
+    for (uint8_t* p : it->ptrs) (void)sv->arena.put(p);
+    sv->spare_events.push_back(it->ev);
+    it = sv->fenced.erase(it);
+  }
+  return SDR_OK;
+}
+
+// `bytes` of device memory from the pool.  Reuses released ranges first;
+// while the pool's reservation would pass the budget, waits for released
+// ranges still behind a fence rather than reserve more; otherwise reserves a
+// new slab.
+int pool_alloc(sdr_server* sv, uint64_t bytes, uint8_t** out)
+{
+  *out = sv->arena.take(bytes);
+  if (*out) return SDR_OK;
+  int rc = close_releases(sv);
+  if (!rc) rc = reclaim(sv, false);
+  if (rc) return rc;
+  while (!(*out = sv->arena.take(bytes)) && !sv->fenced.empty() &&
+         sv->arena.reserved() + std::max(bytes, sv->slab_bytes) > sv->cfg.budget_bytes) {
+    rc = reclaim(sv, true);
+    if (rc) return rc;
+  }
+  if (*out) return SDR_OK;
+  const hipError_t e = sv->arena.grow(bytes, sv->slab_bytes);
+  if (e != hipSuccess)
+    return fail(SDR_E_CAPACITY, "device pool: reserving %llu bytes: %s",
+                static_cast<unsigned long long>(std::max(bytes, sv->slab_bytes)), hipGetErrorString(e));
+  *out = sv->arena.take(bytes);
+  return *out ? SDR_OK : fail(SDR_E_CAPACITY, "device pool: no range after growth");
+}
+
+// Allocate the request's outputs from the pool and point its column struct
+// at them.
+int allocate_outputs(sdr_server* sv, Req* q)
+{
+  int rc = pool_alloc(sv, std::max<uint64_t>(q->out_bytes, 256), &q->d_alloc);
+  if (rc) return rc;
   if (q->r.is_string) {
     const uint64_t off_bytes = (uint64_t{q->r.total_rows} + 1u) * 4u;
     q->scol.d_offsets        = reinterpret_cast<int32_t*>(q->d_alloc);
@@ -125,6 +243,13 @@ void complete(sdr_server* sv, Slot& sl, int rc, bool timed = true)
 {
   const double t_start = timed ? sv->gpu_ms(sl.ev_start) : sv->now_ms();
   const double t_end   = timed ? sv->gpu_ms(sl.ev_end) : t_start;
+  if (timed) {
+    sv->stats.gpu_busy_ms += t_end - t_start;
+    const double from = std::max(t_start, sv->union_end);
+    if (t_end > from) sv->stats.gpu_union_ms += t_end - from;
+    sv->union_end = std::max(sv->union_end, t_end);
+  }
+  sv->stats.requests += sl.reqs.size();
   for (Req* q : sl.reqs) {
     int st = rc;
     if (st == SDR_OK && sl.batch && q->r.is_string) st = batch_column_status(sl.batch, &q->scol);
@@ -199,11 +324,12 @@ int launch(sdr_server* sv, Slot& sl, uint32_t* n_launched)
   if (rc == SDR_OK) {
     sl.busy = true;
     ++sv->running;
+    ++sv->stats.launches;
     if (n_launched) ++*n_launched;
     return SDR_OK;
   }
-  for (Req* q : sl.reqs) {
-    if (q->d_alloc) (void)hipFreeAsync(q->d_alloc, sl.s);
+  for (Req* q : sl.reqs) {  // nothing was enqueued on them
+    if (q->d_alloc) (void)sv->arena.put(q->d_alloc);
     q->d_alloc        = nullptr;
     q->col.d_out      = nullptr;
     q->scol.d_offsets = nullptr;
@@ -217,9 +343,20 @@ int launch(sdr_server* sv, Slot& sl, uint32_t* n_launched)
 
 int launch_slot(sdr_server* sv, Slot& sl)
 {
-  for (Req* q : sl.reqs) {
-    int rc = allocate_outputs(sv, q, sl.s);
-    if (rc) return rc;
+  {
+    PhaseTimer pt(sv->stats.host_alloc_ms);
+    for (Req* q : sl.reqs) {
+      int rc = allocate_outputs(sv, q);
+      if (rc) return rc;
+    }
+  }
+  PhaseTimer pt(sv->stats.host_launch_ms);
+  // outputs may be ranges other streams released: their fences have passed
+  // on the host (reclaim), so no device wait is needed for them; segments
+  // staged since this slot's last launch are waited for on the device
+  if (sv->stage_done && sl.stage_seen != sv->stage_seq) {
+    HIP_TRY(hipStreamWaitEvent(sl.s, sv->stage_done, 0));
+    sl.stage_seen = sv->stage_seq;
   }
   HIP_TRY(hipEventRecord(sl.ev_start, sl.s));
   int rc = SDR_OK;
@@ -230,6 +367,7 @@ int launch_slot(sdr_server* sv, Slot& sl)
       rc = q->r.is_string ? batch_add_strings(sl.batch, &q->scol) : batch_add_fixed(sl.batch, &q->col, 1);
     }
     if (!rc) rc = batch_launch(sl.batch, sl.s);
+    if (!rc) sv->stats.queue_entries += static_cast<uint64_t>(sdr_batch_num_entries(sl.batch));
   } else {
     Req* q = sl.reqs.front();
     if (q->r.is_string) rc = sdr_decode_strings_column_mode(&q->scol, sl.s, sv->cfg.mode);
@@ -262,20 +400,11 @@ int sdr_server_create(const sdr_server_config* config, sdr_server** out)
   if (sv->cfg.mode > SDR_LAUNCH_PERSISTENT) return fail(SDR_E_INVALID_ARG, "unknown launch mode %u", sv->cfg.mode);
   sv->t0 = Clock::now();
   HIP_TRY(hipGetDevice(&sv->device));
-  hipMemPoolProps props{};
-  props.allocType     = hipMemAllocationTypePinned;
-  props.location.type = hipMemLocationTypeDevice;
-  props.location.id   = sv->device;
-  HIP_TRY(hipMemPoolCreate(&sv->pool, &props));
-  uint64_t keep = UINT64_MAX;  // a warm pool never returns memory to the driver
-  HIP_TRY(hipMemPoolSetAttribute(sv->pool, hipMemPoolAttrReleaseThreshold, &keep));
-  // Outputs are freed on the launch stream they were allocated on (after a
-  // wait on the consumer's stream); memory is reused only in that stream's
-  // order, never opportunistically by another launch stream.
-  int no = 0;
-  HIP_TRY(hipMemPoolSetAttribute(sv->pool, hipMemPoolReuseAllowOpportunistic, &no));
-  HIP_TRY(hipMemPoolSetAttribute(sv->pool, hipMemPoolReuseAllowInternalDependencies, &no));
+  // slabs of 1/32 of the budget, between 64 MiB (or the whole budget, if
+  // smaller) and 4 GiB
+  sv->slab_bytes = std::min(SLAB_MAX, std::max(std::min(SLAB_MIN, sv->cfg.budget_bytes), sv->cfg.budget_bytes / 32));
   HIP_TRY(hipStreamCreateWithFlags(&sv->aux, hipStreamNonBlocking));
+  HIP_TRY(hipStreamCreateWithFlags(&sv->fence_s, hipStreamNonBlocking));
   sv->slots.resize(sv->cfg.max_inflight);
   for (auto& sl : sv->slots) {
     HIP_TRY(hipStreamCreateWithFlags(&sl.s, hipStreamNonBlocking));
@@ -299,25 +428,53 @@ int sdr_server_stage(sdr_server* sv, const void* host_bytes, uint64_t n, const u
   clear_error();
   if (!sv || !d_out || (n && !host_bytes)) return fail(SDR_E_INVALID_ARG, "NULL argument");
   *d_out = nullptr;
+  PhaseTimer pt(sv->stats.host_stage_ms);
   if (held(sv) + n > sv->cfg.budget_bytes)
     return fail(SDR_E_CAPACITY, "staging %llu bytes exceeds the budget", static_cast<unsigned long long>(n));
-  if (!sv->h_stage) HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&sv->h_stage), STAGE_CHUNK, hipHostMallocDefault));
   uint8_t* d = nullptr;
-  HIP_TRY(hipMalloc(reinterpret_cast<void**>(&d), std::max<uint64_t>(n, 16)));
+  int rc     = pool_alloc(sv, std::max<uint64_t>(n, 16), &d);
+  if (rc) return rc;
+  // chunks go through the ring of pinned buffers: the host copies into a
+  // buffer once the copy that last used it has finished (a wait only when
+  // the ring is full), the device copies on the staging stream
   const auto* src = static_cast<const uint8_t*>(host_bytes);
   for (uint64_t o = 0; o < n; o += STAGE_CHUNK) {
-    const size_t m = static_cast<size_t>(std::min<uint64_t>(STAGE_CHUNK, n - o));
-    std::memcpy(sv->h_stage, src + o, m);
-    hipError_t e = hipMemcpyAsync(d + o, sv->h_stage, m, hipMemcpyHostToDevice, sv->aux);
-    if (e == hipSuccess) e = hipStreamSynchronize(sv->aux);  // the staging buffer is reused
+    const size_t m   = static_cast<size_t>(std::min<uint64_t>(STAGE_CHUNK, n - o));
+    const uint32_t k = sv->ring_next;
+    sv->ring_next    = (k + 1) % STAGE_RING;
+    hipError_t e     = hipSuccess;
+    if (!sv->ring[k]) {
+      e = hipHostMalloc(reinterpret_cast<void**>(&sv->ring[k]), STAGE_CHUNK, hipHostMallocDefault);
+      if (e == hipSuccess) e = hipEventCreateWithFlags(&sv->ring_ev[k], hipEventDisableTiming);
+    } else if (sv->ring_used[k]) {
+      e = hipEventSynchronize(sv->ring_ev[k]);
+    }
+    if (e == hipSuccess) {
+      std::memcpy(sv->ring[k], src + o, m);
+      e = hipMemcpyAsync(d + o, sv->ring[k], m, hipMemcpyHostToDevice, sv->aux);
+    }
+    if (e == hipSuccess) e = hipEventRecord(sv->ring_ev[k], sv->aux);
     if (e != hipSuccess) {
-      (void)hipFree(d);
+      (void)hipStreamSynchronize(sv->aux);  // nothing may still write the range
+      (void)sv->arena.put(d);
       return fail(SDR_E_HIP, "stage: %s", hipGetErrorString(e));
     }
+    sv->ring_used[k] = true;
   }
+  if (!sv->stage_done) HIP_TRY(hipEventCreateWithFlags(&sv->stage_done, hipEventDisableTiming));
+  HIP_TRY(hipEventRecord(sv->stage_done, sv->aux));
+  ++sv->stage_seq;
   sv->staged[d] = n;
   sv->resident += n;
   *d_out = d;
+  return SDR_OK;
+}
+
+int sdr_server_stage_wait(sdr_server* sv, sdr_stream_t stream)
+{
+  clear_error();
+  if (!sv) return fail(SDR_E_INVALID_ARG, "server is NULL");
+  if (sv->stage_done) HIP_TRY(hipStreamWaitEvent(static_cast<hipStream_t>(stream), sv->stage_done, 0));
   return SDR_OK;
 }
 
@@ -327,9 +484,18 @@ int sdr_server_unstage(sdr_server* sv, const uint8_t* d_ptr)
   if (!sv) return fail(SDR_E_INVALID_ARG, "server is NULL");
   auto it = sv->staged.find(d_ptr);
   if (it == sv->staged.end()) return fail(SDR_E_NOT_FOUND, "pointer was not staged by this server");
+  PhaseTimer pt(sv->stats.host_stage_ms);
+  // the range rejoins the pool once every launch in flight (which may read
+  // it) and its own staging copy have finished: a fence on a stream that
+  // waits for both, so the host never blocks here
   for (auto& sl : sv->slots)
-    if (sl.busy) HIP_TRY(hipStreamSynchronize(sl.s));  // in-flight work may read it
-  HIP_TRY(hipFree(const_cast<uint8_t*>(d_ptr)));
+    if (sl.busy) HIP_TRY(hipStreamWaitEvent(sv->fence_s, sl.ev_end, 0));
+  if (sv->stage_done) HIP_TRY(hipStreamWaitEvent(sv->fence_s, sv->stage_done, 0));
+  hipEvent_t ev = nullptr;
+  int rc        = get_event(sv, &ev);
+  if (rc) return rc;
+  HIP_TRY(hipEventRecord(ev, sv->fence_s));
+  sv->fenced.push_back({{const_cast<uint8_t*>(d_ptr)}, ev});
   sv->resident -= it->second;
   sv->staged.erase(it);
   return SDR_OK;
@@ -339,6 +505,7 @@ int sdr_server_submit(sdr_server* sv, const sdr_request* r)
 {
   clear_error();
   if (!sv || !r) return fail(SDR_E_INVALID_ARG, "NULL argument");
+  PhaseTimer pt(sv->stats.host_submit_ms);
   if (sv->reqs.count(r->tag)) return fail(SDR_E_INVALID_ARG, "tag %llu is pending", static_cast<unsigned long long>(r->tag));
   if (r->n_data && !r->data) return fail(SDR_E_INVALID_ARG, "data is NULL");
   auto q = std::make_unique<Req>();
@@ -393,7 +560,13 @@ int sdr_server_progress(sdr_server* sv, uint32_t* n_launched)
   clear_error();
   if (!sv) return fail(SDR_E_INVALID_ARG, "server is NULL");
   if (n_launched) *n_launched = 0;
-  int rc = collect(sv);
+  int rc;
+  {
+    PhaseTimer pt(sv->stats.host_collect_ms);
+    rc = collect(sv);
+    if (!rc) rc = close_releases(sv);
+    if (!rc) rc = reclaim(sv, false);
+  }
   if (rc) return rc;
   admit(sv);
   for (auto& sl : sv->slots) {
@@ -423,20 +596,12 @@ int sdr_server_release(sdr_server* sv, uint64_t tag, sdr_stream_t stream)
   auto it = sv->reqs.find(tag);
   if (it == sv->reqs.end()) return fail(SDR_E_NOT_FOUND, "unknown tag %llu", static_cast<unsigned long long>(tag));
   Req* q = it->second.get();
+  PhaseTimer pt(sv->stats.host_release_ms);
   if (!q->done) return fail(SDR_E_STATE, "request %llu has not completed", static_cast<unsigned long long>(tag));
-  if (q->d_alloc) {
-    // free in the launch stream's order, after the consumer's work so far
-    hipStream_t consumer = static_cast<hipStream_t>(stream);
-    if (consumer != q->stream) {
-      hipEvent_t ev = nullptr;
-      HIP_TRY(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
-      hipError_t e = hipEventRecord(ev, consumer);
-      if (e == hipSuccess) e = hipStreamWaitEvent(q->stream, ev, 0);
-      (void)hipEventDestroy(ev);  // released once it has completed
-      if (e != hipSuccess) return fail(SDR_E_HIP, "release: %s", hipGetErrorString(e));
-    }
-    HIP_TRY(hipFreeAsync(q->d_alloc, q->stream));
-  }
+  // the range rejoins the pool behind an event on the consumer's stream,
+  // recorded once for all the releases on that stream before the next
+  // progress or allocation
+  if (q->d_alloc) sv->open_release[static_cast<hipStream_t>(stream)].push_back(q->d_alloc);
   sv->outputs_held -= q->out_bytes;
   sv->reqs.erase(it);
   return SDR_OK;
@@ -453,6 +618,14 @@ int sdr_server_pending(const sdr_server* sv)
 uint64_t sdr_server_held_bytes(const sdr_server* sv) { return sv ? held(sv) : 0; }
 
 double sdr_server_now_ms(const sdr_server* sv) { return sv ? sv->now_ms() : 0.0; }
+
+int sdr_server_get_stats(const sdr_server* sv, sdr_server_stats* out)
+{
+  clear_error();
+  if (!sv || !out) return fail(SDR_E_INVALID_ARG, "NULL argument");
+  *out = sv->stats;
+  return SDR_OK;
+}
 
 int sdr_server_destroy(sdr_server* sv)
 {

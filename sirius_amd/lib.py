@@ -25,7 +25,7 @@ DECLARED = [
     "sdr_tune_set", "sdr_tune_get", "sdr_tune_load",
     "sdr_server_create", "sdr_server_stage", "sdr_server_unstage", "sdr_server_submit", "sdr_server_progress",
     "sdr_server_poll", "sdr_server_release", "sdr_server_pending", "sdr_server_held_bytes", "sdr_server_now_ms",
-    "sdr_server_destroy",
+    "sdr_server_destroy", "sdr_server_get_stats", "sdr_server_stage_wait",
     "sdr_stage_plan", "sdr_stage_file", "sdr_rowids",
     "sdr_arrow_export",
 ]
@@ -109,6 +109,10 @@ def load() -> C.CDLL:
     lib.sdr_server_held_bytes.restype = C.c_uint64
     lib.sdr_server_now_ms.argtypes = [vp]
     lib.sdr_server_now_ms.restype = C.c_double
+    lib.sdr_server_stage_wait.argtypes = [vp, vp]
+    lib.sdr_server_stage_wait.restype = i32
+    lib.sdr_server_get_stats.argtypes = [vp, P(_abi.ServerStats)]
+    lib.sdr_server_get_stats.restype = i32
     lib.sdr_server_destroy.argtypes = [vp]
     lib.sdr_server_destroy.restype = i32
     lib.sdr_tune_set.argtypes = [u32, C.c_uint64]

@@ -5,8 +5,12 @@ Resident pool: for each codec of the mix, one source column of whole
 122880-row row groups (enough for the largest request) and a ladder of tail
 columns of log-spaced row counts below one row group, DuckDB-format
 segments (self-encoded), staged into the server's device memory through its
-pinned staging buffer.  Each staged column is decoded once into a reference
-copy and checked bit-exact against the generated values.
+pinned staging ring.  Each staged column is decoded once into a reference
+copy and checked bit-exact against the generated values.  The pool can hold
+`replicas` copies of every column at distinct device addresses: each
+request reads one copy, drawn at random, so a large pool makes requests read
+their segments from HBM rather than from a cache that earlier requests
+warmed.
 
 Requests: sizes drawn log-uniform from [min_rows, max_rows]; a request of R
 rows decodes consecutive source segments from a random start while they
@@ -84,20 +88,24 @@ class Column:
         """First row of each segment, then the row count (n_segments + 1)."""
         return np.append(self.enc.segs["row_offset"].astype(np.int64), self.enc.n_rows)
 
-    def seg(self, i: int, row_offset: int) -> A.Segment:
+    def seg(self, i: int, row_offset: int, base: int = 0) -> A.Segment:
         s = self.enc.segs[i]
-        return A.Segment(self.d_base + int(s["offset"]), int(s["bytes_size"]), row_offset, int(s["row_count"]),
-                         int(s["max_string_length"]))
+        return A.Segment((base or self.d_base) + int(s["offset"]), int(s["bytes_size"]), row_offset,
+                         int(s["row_count"]), int(s["max_string_length"]))
 
 
 class Pool:
     """Resident source and tail columns per codec, staged into `server`."""
 
-    def __init__(self, lib, server, kinds=tuple(MIX), max_rows=16 << 20, n_tails=48, seed=0xC0DE0005, log=None):
+    def __init__(self, lib, server, kinds=tuple(MIX), max_rows=16 << 20, n_tails=48, seed=0xC0DE0005, log=None,
+                 replicas=1, resident_bytes=0):
+        """`replicas` copies of every column, or, when `resident_bytes` is
+        given, as many as it takes to hold about that many bytes."""
         import torch
 
         self.lib, self.server = lib, server
         self.src, self.tails = {}, {}
+        self.copies = {}  # column id -> device bases of copies 1..replicas-1
         self.n_src_segs = -(-max_rows // SEG_ROWS)  # source rows: whole row groups covering max_rows
         self.staged_bytes = 0
         tail_sizes = np.unique(np.round(np.geomspace(2048, SEG_ROWS - 1, n_tails)).astype(np.int64))
@@ -113,19 +121,43 @@ class Pool:
             if log:
                 log(f"[serve] pool {kind}: source {self.n_src_segs} segments + {len(tail_sizes)} tails, "
                     f"{time.time() - t:.1f}s")
+        self.copy_bytes = self.staged_bytes
+        self.replicas = max(1, int(replicas))
+        if resident_bytes:
+            self.replicas = max(1, -(-int(resident_bytes) // self.copy_bytes))
+        t = time.time()
+        cols = [c for k in self.src for c in [self.src[k], *self.tails[k]]]
+        for col in cols:
+            self.copies[id(col)] = []
+        for _ in range(self.replicas - 1):
+            for col in cols:
+                self.copies[id(col)].append(self._stage_bytes(col.enc))
+        torch.cuda.synchronize()
+        if log and self.replicas > 1:
+            log(f"[serve] pool: {self.replicas} copies at distinct addresses, {self.staged_bytes / 1e9:.1f} GB "
+                f"resident, copies staged in {time.time() - t:.1f}s")
         self.tail_rows = [int(x) for x in tail_sizes]
+
+    def _stage_bytes(self, enc) -> int:
+        d = C.c_void_p()
+        L.check(self.lib.sdr_server_stage(self.server, enc.blob.ctypes.data, enc.blob.size, C.byref(d)))
+        self.staged_bytes += enc.blob.size
+        return d.value
+
+    def base(self, col: Column, replica: int) -> int:
+        """Device address of copy `replica` of a staged column."""
+        return col.d_base if replica == 0 else self.copies[id(col)][replica - 1]
 
     def _stage(self, kind, values) -> Column:
         import torch
 
         enc = _encode(kind, values)
-        d = C.c_void_p()
-        L.check(self.lib.sdr_server_stage(self.server, enc.blob.ctypes.data, enc.blob.size, C.byref(d)))
-        self.staged_bytes += enc.blob.size
-        col = Column(kind, enc, d.value)
+        col = Column(kind, enc, self._stage_bytes(enc))
         # reference decode of the whole column, checked against the values
+        # (on torch's stream, after the staging copy)
         dev = torch.device("cuda:0")
         sh = torch.cuda.current_stream().cuda_stream
+        L.check(self.lib.sdr_server_stage_wait(self.server, sh))
         runs, keep = A.make_runs([(enc.codec, [col.seg(i, int(s["row_offset"])) for i, s in enumerate(enc.segs)])])
         if col.is_string:
             chars, offs = values
@@ -162,6 +194,7 @@ class Spec:
     src_first: int  # first source segment
     n_src: int      # source segments
     tail: int       # tail column index or -1
+    replica: int = 0  # which copy of the pool's columns it reads
     arrival_ms: float = 0.0
     comp_bytes: int = 0
     keep: list = field(default_factory=list)
@@ -169,14 +202,16 @@ class Spec:
     def request(self, pool: Pool) -> A.Request:
         src = pool.src[self.kind]
         segs, row = [], 0
+        sb = pool.base(src, self.replica)
         for i in range(self.src_first, self.src_first + self.n_src):
-            seg = src.seg(i, row)
+            seg = src.seg(i, row, sb)
             segs.append(seg)
             row += seg.row_count
         if self.tail >= 0:
             tcol = pool.tails[self.kind][self.tail]
+            tb = pool.base(tcol, self.replica)
             for j in range(len(tcol.enc.segs)):
-                seg = tcol.seg(j, row)
+                seg = tcol.seg(j, row, tb)
                 segs.append(seg)
                 row += seg.row_count
         runs, keep = A.make_runs([(src.enc.codec, segs)])
@@ -215,7 +250,8 @@ def make_specs(pool: Pool, n: int, seed: int = 0xC0DE0005, min_rows: int = 2048,
         if last == first and tail < 0:
             tail = 0
         rows = int(cum[last] - cum[first]) + (pool.tail_rows[tail] if tail >= 0 else 0)
-        specs.append(Spec(i, k, rows, first, last - first, tail))
+        replica = int(rng.integers(0, pool.replicas)) if pool.replicas > 1 else 0
+        specs.append(Spec(i, k, rows, first, last - first, tail, replica))
     return specs
 
 
@@ -334,6 +370,9 @@ def run(lib, pool: Pool, specs: list, mode: int, max_inflight: int = 2, budget: 
     checks = []  # (tag, mismatch count on the device) per verified request
     n_launches = 0
     i = 0
+    st0, st1 = A.ServerStats(), A.ServerStats()
+    t_verify = 0.0
+    L.check(lib.sdr_server_get_stats(server, C.byref(st0)))
     try:
         t_base = lib.sdr_server_now_ms(server)
         if all(s.arrival_ms <= 0 for s in order):  # capacity: queue everything first
@@ -358,19 +397,25 @@ def run(lib, pool: Pool, specs: list, mode: int, max_inflight: int = 2, budget: 
                 bytes_out += (s.rows + 1) * 4 + c.chars_used if is_str[s.kind] else c.out_bytes
                 bytes_in += s.comp_bytes
                 if verify_every and s.tag % verify_every == 0 and c.status == 0:
+                    tv = time.perf_counter()
                     n_verified += 1
                     mm = mismatches(pool, s, c)
                     bad += mm
                     checks.append((s.tag, mm))
+                    t_verify += time.perf_counter() - tv
                 L.check(lib.sdr_server_release(server, c.tag, sh))
             if n_out.value == 0 and launched.value == 0:
                 nxt = order[i].arrival_ms if i < len(order) else None
                 if nxt is not None and nxt - now > 0.2:
                     time.sleep(min(0.001, (nxt - now) / 2000.0))
+        L.check(lib.sdr_server_get_stats(server, C.byref(st1)))
     finally:
         torch.cuda.synchronize()
         if own:
             lib.sdr_server_destroy(server)
+    d0, d1 = st0.as_dict(), st1.as_dict()
+    server_stats = {k: d1[k] - d0[k] for k in d1}
+    server_stats["driver_verify_ms"] = t_verify * 1e3
     for s in specs:
         s.keep = []
     first = max(min(s.arrival_ms for s in specs), 0.0)
@@ -390,4 +435,5 @@ def run(lib, pool: Pool, specs: list, mode: int, max_inflight: int = 2, budget: 
             "achieved_gbs": (bytes_out + bytes_in) / (span_ms * 1e-3) / 1e9,
             "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
             "mean_ms": float(lat.mean()), "errors": int(sum(1 for x in statuses if x != 0)),
-            "verified": n_verified, "verified_ok": n_verified if n_bad == 0 else -1, "mismatched_bytes": n_bad}
+            "verified": n_verified, "verified_ok": n_verified if n_bad == 0 else -1, "mismatched_bytes": n_bad,
+            "server_stats": server_stats}
