@@ -1,7 +1,9 @@
 // sdr_batch.cpp — host side of the persistent mixed-codec decode: builds the
-// device work queue (one arena: header, entries, FSST columns, chunks,
-// segments, look-back state), uploads it from pinned memory with one copy,
-// and launches the persistent kernel, optionally as a replayed hipGraph.
+// device work queue (one arena: header, entries grouped by decoder and
+// width, FSST columns, chunks, segments, look-back state), uploads it from
+// pinned memory with one copy, and launches the groups' persistent kernels,
+// optionally as a replayed hipGraph whose group kernels are parallel
+// branches.
 #include "sdr_batch.hpp"
 #include "sdr_internal.hpp"
 #include "sdr_tune.hpp"
@@ -14,6 +16,7 @@
 struct sdr_batch {
   // host-side description
   std::vector<sdr::BatchItem> items;
+  std::vector<uint8_t> item_group;  // BatchGroup of each entry
   struct Col {
     uint32_t n_chunks = 0;
     std::vector<sdr::FsstSeg> segs;  // row ranges: segments and the gaps between them
@@ -36,15 +39,17 @@ struct sdr_batch {
   size_t arena_bytes   = 0;
   uint8_t* h_stage     = nullptr;
   size_t stage_bytes   = 0;
+  hipEvent_t upload_done = nullptr;
+  bool upload_pending  = false;
   uint8_t* h_readback  = nullptr;  // pinned copy of the look-back state and null counters
   size_t readback_cap  = 0;
   size_t readback_bytes = 0;
   size_t used_bytes    = 0;
   size_t off_items = 0, off_cols = 0, off_chunks = 0, off_segs = 0, off_state = 0, off_nulls = 0;
   uint32_t n_status    = 0;
-  hipEvent_t upload_done = nullptr;
   hipEvent_t launch_done = nullptr;
-  bool upload_pending  = false;
+  size_t n_entries     = 0;                         // queue entries of the last upload
+  uint32_t entry_count[sdr::BG_COUNT] = {};         // ... per group
   // launch configuration
   // work units per queue entry, per BatchKind.  Each entry costs one claim
   // (an atomic on the queue's one counter and two barriers); on the
@@ -56,8 +61,9 @@ struct sdr_batch {
   hipGraphExec_t gexec = nullptr;
   hipGraph_t graph     = nullptr;
   uint8_t* graph_arena = nullptr;
-  uint32_t graph_grid  = 0;
-  bool graph_strings   = false;
+  uint32_t graph_mask  = 0;  // groups the recorded graph launches
+  uint32_t graph_gridopt = 0;
+  uint32_t group_cap[sdr::BG_COUNT] = {};  // resident blocks of each group's kernel (0 = not queried)
   hipStream_t last_stream = nullptr;
   bool launched           = false;  // a launch whose results are not collected yet
   ~sdr_batch();
@@ -97,6 +103,20 @@ BatchKind kind_of(uint32_t codec)
   }
 }
 
+// Queue group of an entry of kind k over values of ts bytes.
+BatchGroup group_of(BatchKind k, uint32_t ts)
+{
+  const uint32_t w = ts == 1 ? 0u : ts == 2 ? 1u : ts == 4 ? 2u : 3u;
+  switch (k) {
+    case BK_BITPACKING: return BatchGroup(BG_BP1 + w);
+    case BK_RLE: return BatchGroup(BG_RLE1 + w);
+    case BK_ALP: return ts == 4 ? BG_ALP4 : BG_ALP8;
+    case BK_ALPRD: return ts == 4 ? BG_ALPRD4 : BG_ALPRD8;
+    case BK_FSST: return BG_FSST;
+    default: return BG_COPY;
+  }
+}
+
 uint32_t units_in_segment(BatchKind k, uint32_t rows, uint32_t ts, uint32_t rle_slice)
 {
   switch (k) {
@@ -108,6 +128,7 @@ uint32_t units_in_segment(BatchKind k, uint32_t rows, uint32_t ts, uint32_t rle_
   }
 }
 
+// Wait until the staging copy is no longer read by a queued upload.
 int wait_upload(sdr_batch* b)
 {
   if (b->upload_pending) {
@@ -126,25 +147,23 @@ int batch_add_fixed(sdr_batch* b, sdr_column* cols, uint32_t n_cols)
     for (uint32_t r = 0; r < col.n_data; ++r) {
       const sdr_run& run = col.data[r];
       const BatchKind k  = kind_of(run.codec);
-      const uint32_t per = std::max<uint32_t>(1, b->units[k]);
       const uint32_t rle_slice = tune(SDR_TUNE_RLE_SLICE_ROWS);
       for (uint32_t s = 0; s < run.n_segments; ++s) {
         const sdr_segment& g = run.segments[s];
         if (!g.row_count) continue;
-        const uint32_t nu = units_in_segment(k, g.row_count, col.type_size, rle_slice);
-        for (uint32_t u = 0; u < nu; u += per) {
-          BatchItem it{};
-          it.seg        = g.d_bytes;
-          it.out        = static_cast<uint8_t*>(col.d_out) + uint64_t{g.row_offset} * col.type_size;
-          it.bytes_size = g.bytes_size;
-          it.row_count  = g.row_count;
-          it.unit0      = u;
-          it.n_units    = std::min(per, nu - u);
-          it.kind       = k;
-          it.ts         = static_cast<uint8_t>(col.type_size);
-          if (k == BK_RLE) it.reserved = static_cast<uint16_t>(rle_slice / 64u);
-          b->items.push_back(it);
-        }
+        // one work item per segment; upload cuts it into queue entries
+        BatchItem it{};
+        it.seg        = g.d_bytes;
+        it.out        = static_cast<uint8_t*>(col.d_out) + uint64_t{g.row_offset} * col.type_size;
+        it.bytes_size = g.bytes_size;
+        it.row_count  = g.row_count;
+        it.unit0      = 0;
+        it.n_units    = units_in_segment(k, g.row_count, col.type_size, rle_slice);
+        it.kind       = k;
+        it.ts         = static_cast<uint8_t>(col.type_size);
+        if (k == BK_RLE) it.reserved = static_cast<uint16_t>(rle_slice / 64u);
+        b->items.push_back(it);
+        b->item_group.push_back(static_cast<uint8_t>(group_of(k, col.type_size)));
       }
     }
     if (col.has_nulls && col.n_validity && col.total_rows) b->masks.push_back({&cols[c], nullptr});
@@ -187,15 +206,79 @@ int batch_add_strings(sdr_batch* b, sdr_string_column* col)
     it.chunk = k;
     it.unit0 = ci;  // resolved to a device column pointer at upload
     b->items.push_back(it);
+    b->item_group.push_back(static_cast<uint8_t>(BG_FSST));
   }
   if (col->has_nulls && col->d_null_mask) b->masks.push_back({nullptr, col});
   b->cols.push_back(std::move(bc));
   return SDR_OK;
 }
 
+// Resident blocks of the kernels of the groups in `mask` (queried once).
+int batch_group_caps(sdr_batch* b, uint32_t mask)
+{
+  int dev = 0, cus = 0;
+  for (uint32_t g = 0; g < BG_COUNT; ++g) {
+    if (!(mask >> g & 1u) || b->group_cap[g]) continue;
+    if (!cus) {
+      HIP_TRY(hipGetDevice(&dev));
+      HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    }
+    const void* f = nullptr;
+    uint32_t block = 0, per = 0;
+    HIP_TRY(batch_group_launch_params(g, &f, &block, &per));
+    b->group_cap[g] = static_cast<uint32_t>(cus) * per;
+  }
+  return SDR_OK;
+}
+
+// How the work items become queue entries.  A fixed-width group's items
+// are cut into entries of per[g] units: enough entries to give every
+// resident block of the group's kernel work (a multiple of the four
+// wavefronts for the wave-parallel codecs), at most the batch's units per
+// entry for the codec, whose claims then amortise.  An FSST item is one
+// entry.  Returns the entries per group and the groups present.
+int batch_plan(sdr_batch* b, uint32_t per[BG_COUNT], uint32_t entries[BG_COUNT], uint32_t* mask)
+{
+  uint64_t tot[BG_COUNT]  = {};
+  uint32_t maxu[BG_COUNT] = {};
+  *mask = 0;
+  for (size_t i = 0; i < b->items.size(); ++i) {
+    const uint32_t g = b->item_group[i];
+    *mask |= 1u << g;
+    tot[g] += b->items[i].n_units;
+    maxu[g] = std::max<uint32_t>(1, b->units[b->items[i].kind]);
+  }
+  int rc = batch_group_caps(b, *mask);
+  if (rc) return rc;
+  for (uint32_t g = 0; g < BG_COUNT; ++g) {
+    entries[g] = 0;
+    per[g]     = 1;
+    if (!tot[g] || g == BG_FSST) continue;
+    const uint64_t cap  = std::max<uint32_t>(1, b->group_cap[g]);
+    uint64_t want       = (tot[g] + cap - 1) / cap;
+    const bool wave_par = !(g >= BG_RLE1 && g <= BG_RLE8);
+    if (wave_par) want = (want + BATCH_WAVES - 1) / BATCH_WAVES * BATCH_WAVES;
+    per[g] = static_cast<uint32_t>(std::min<uint64_t>(std::max<uint64_t>(want, 1), maxu[g]));
+  }
+  for (size_t i = 0; i < b->items.size(); ++i) {
+    const uint32_t g = b->item_group[i];
+    entries[g] += g == BG_FSST ? 1u : (b->items[i].n_units + per[g] - 1) / per[g];
+  }
+  return SDR_OK;
+}
+
 // Lay out the arena, fill the pinned staging copy, upload it.
 int batch_upload(sdr_batch* b, hipStream_t s)
 {
+  uint32_t per[BG_COUNT], entries[BG_COUNT], mask = 0;
+  int prc = batch_plan(b, per, entries, &mask);
+  if (prc) return prc;
+  size_t n_entries = 0;
+  for (uint32_t g = 0; g < BG_COUNT; ++g) {
+    n_entries += entries[g];
+    b->entry_count[g] = entries[g];
+  }
+  b->n_entries = n_entries;
   auto al = [](size_t v) { return (v + 255u) & ~size_t{255}; };
   size_t n_chunks = 0, n_segs = 0;
   for (auto& c : b->cols) {
@@ -207,7 +290,7 @@ int batch_upload(sdr_batch* b, hipStream_t s)
   for (auto& c : b->cols) state_words += 2 + c.n_chunks;
   size_t off                  = al(sizeof(BatchQueue));
   b->off_items                = off;
-  off                         = al(off + b->items.size() * sizeof(BatchItem));
+  off                         = al(off + n_entries * sizeof(BatchItem));
   b->off_cols                 = off;
   off                         = al(off + b->cols.size() * sizeof(FsstColumn));
   b->off_segs                 = off;
@@ -233,6 +316,8 @@ int batch_upload(sdr_batch* b, hipStream_t s)
     HIP_TRY(hipMalloc(reinterpret_cast<void**>(&b->d_arena), off + off / 4));
     b->arena_bytes = off + off / 4;
   }
+  int wrc = wait_upload(b);
+  if (wrc) return wrc;
   if (upload_bytes > b->stage_bytes) {
     if (b->h_stage) HIP_TRY(hipHostFree(b->h_stage));
     b->h_stage     = nullptr;
@@ -251,7 +336,16 @@ int batch_upload(sdr_batch* b, hipStream_t s)
   uint8_t* h = b->h_stage;
   uint8_t* d = b->d_arena;
   BatchQueue q{};
-  q.n_items       = static_cast<uint32_t>(b->items.size());
+  // entries laid out group by group, in the order added within a group
+  uint32_t at[BG_COUNT] = {};
+  for (uint32_t g = 0, o = 0; g < BG_COUNT; ++g) {
+    q.count[g] = entries[g];
+    q.first[g] = o;
+    at[g]      = o;
+    o += entries[g];
+  }
+  q.n_items       = static_cast<uint32_t>(n_entries);
+  q.bp_chunk_rows = tune(SDR_TUNE_BP_CHUNK_ROWS);
   q.n_fsst_status = static_cast<uint32_t>(state_words);
   q.fsst_status   = reinterpret_cast<uint64_t*>(d + b->off_state);
   q.n_fsst_segs   = static_cast<uint32_t>(n_segs);
@@ -289,9 +383,22 @@ int batch_upload(sdr_batch* b, hipStream_t s)
   }
   if (!dcols.empty()) std::memcpy(h + b->off_cols, dcols.data(), dcols.size() * sizeof(FsstColumn));
   auto* hi = reinterpret_cast<BatchItem*>(h + b->off_items);
-  std::memcpy(hi, b->items.data(), b->items.size() * sizeof(BatchItem));
-  for (size_t i = 0; i < b->items.size(); ++i)
-    if (hi[i].kind == BK_FSST) hi[i].column = reinterpret_cast<const FsstColumn*>(d + b->off_cols) + hi[i].unit0;
+  for (size_t i = 0; i < b->items.size(); ++i) {
+    const uint32_t g  = b->item_group[i];
+    const BatchItem& it = b->items[i];
+    if (g == BG_FSST) {
+      BatchItem& dst = hi[at[g]++];
+      dst            = it;
+      dst.column     = reinterpret_cast<const FsstColumn*>(d + b->off_cols) + it.unit0;
+      continue;
+    }
+    for (uint32_t u = 0; u < it.n_units; u += per[g]) {
+      BatchItem& dst = hi[at[g]++];
+      dst            = it;
+      dst.unit0      = it.unit0 + u;
+      dst.n_units    = std::min(per[g], it.n_units - u);
+    }
+  }
   HIP_TRY(hipMemcpyAsync(d, h, upload_bytes, hipMemcpyHostToDevice, s));
   if (!b->upload_done) HIP_TRY(hipEventCreateWithFlags(&b->upload_done, hipEventDisableTiming));
   HIP_TRY(hipEventRecord(b->upload_done, s));
@@ -302,55 +409,81 @@ int batch_upload(sdr_batch* b, hipStream_t s)
 int batch_launch(sdr_batch* b, hipStream_t s)
 {
   clear_error();
-  int rc = wait_upload(b);
-  if (rc) return rc;
+  int rc = SDR_OK;
   if (b->launch_done && b->last_stream && b->last_stream != s) HIP_TRY(hipStreamWaitEvent(s, b->launch_done, 0));
   rc = batch_upload(b, s);
   if (rc) return rc;
   auto* q     = reinterpret_cast<BatchQueue*>(b->d_arena);
   auto* items = reinterpret_cast<const BatchItem*>(b->d_arena + b->off_items);
-  const bool strings = std::any_of(b->items.begin(), b->items.end(),
-                                   [](const BatchItem& it) { return it.kind == BK_FSST; });
-  uint32_t grid = b->grid;
-  if (grid == 0) {
-    int dev = 0, cus = 0;
-    HIP_TRY(hipGetDevice(&dev));
-    HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-    grid = static_cast<uint32_t>(cus * batch_occupancy_blocks_per_cu(strings));
+  // each group kernel's grid: its resident capacity (or the SDR_BATCH_GRID
+  // cap), so a block loops over many entries
+  uint32_t cnt[BG_COUNT] = {}, mask = 0;
+  for (uint32_t g = 0; g < BG_COUNT; ++g) {
+    cnt[g] = b->entry_count[g];
+    if (cnt[g]) mask |= 1u << g;
   }
-  grid = std::max<uint32_t>(1, std::min<uint32_t>(grid, std::max<uint32_t>(1, static_cast<uint32_t>(b->items.size()))));
-  if (b->use_graph) {
-    // the graph's arguments are the arena addresses; entries change through
-    // the upload above, so one instantiation replays every batch
-    if (!b->gexec || b->graph_arena != b->d_arena || b->graph_grid != grid || b->graph_strings != strings) {
+  uint32_t grid[BG_COUNT] = {};
+  for (uint32_t g = 0; g < BG_COUNT; ++g) {
+    if (!(mask >> g & 1u)) continue;
+    grid[g] = std::max<uint32_t>(1, b->grid ? std::min(b->grid, b->group_cap[g]) : b->group_cap[g]);
+  }
+  const bool strings = mask & (1u << BG_FSST);
+  if (b->use_graph && mask) {
+    // nodes: reset -> [FSST pre-pass] -> one persistent kernel per group, the
+    // group kernels parallel branches; node arguments are the arena
+    // addresses, entries change through the upload above, so one
+    // instantiation replays every batch with the same groups
+    if (!b->gexec || b->graph_arena != b->d_arena || b->graph_mask != mask || b->graph_gridopt != b->grid) {
       if (b->gexec) HIP_TRY(hipGraphExecDestroy(b->gexec));
       if (b->graph) HIP_TRY(hipGraphDestroy(b->graph));
       b->gexec = nullptr;
       b->graph = nullptr;
-      hipStream_t cs;
-      HIP_TRY(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
-      hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
-      if (e == hipSuccess) e = launch_batch_reset(q, cs);
-      if (e == hipSuccess && strings) e = launch_batch_fsst_prep(q, cs);
-      if (e == hipSuccess) e = launch_batch(items, q, grid, strings, cs);
-      hipGraph_t g = nullptr;
-      hipError_t e2 = hipStreamEndCapture(cs, &g);
-      (void)hipStreamDestroy(cs);
-      if (e != hipSuccess || e2 != hipSuccess) {
-        if (g) (void)hipGraphDestroy(g);
-        return fail(SDR_E_HIP, "batch graph capture: %s", hipGetErrorString(e != hipSuccess ? e : e2));
+      hipGraph_t gr = nullptr;
+      HIP_TRY(hipGraphCreate(&gr, 0));
+      b->graph = gr;
+      BatchQueue* qa = q;
+      const BatchItem* ia = items;
+      void* reset_args[] = {&qa};
+      hipKernelNodeParams kp{};
+      kp.func           = const_cast<void*>(batch_reset_kernel());
+      kp.gridDim        = dim3(BATCH_RESET_GRID);
+      kp.blockDim       = dim3(256);
+      kp.kernelParams   = reset_args;
+      hipGraphNode_t n_reset = nullptr, n_prep = nullptr;
+      HIP_TRY(hipGraphAddKernelNode(&n_reset, gr, nullptr, 0, &kp));
+      if (strings) {
+        kp.func     = const_cast<void*>(batch_fsst_prep_kernel());
+        kp.gridDim  = dim3(BATCH_PREP_GRID);
+        kp.blockDim = dim3(BATCH_PREP_THREADS);
+        HIP_TRY(hipGraphAddKernelNode(&n_prep, gr, &n_reset, 1, &kp));
       }
-      b->graph = g;
-      HIP_TRY(hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0));
-      b->graph_arena = b->d_arena;
-      b->graph_grid    = grid;
-      b->graph_strings = strings;
+      uint32_t gids[BG_COUNT];
+      for (uint32_t g = 0; g < BG_COUNT; ++g) {
+        if (!(mask >> g & 1u)) continue;
+        gids[g] = g;
+        const void* f = nullptr;
+        uint32_t block = 0, per = 0;
+        HIP_TRY(batch_group_launch_params(g, &f, &block, &per));
+        void* args[] = {&ia, &qa, &gids[g]};
+        kp.func         = const_cast<void*>(f);
+        kp.gridDim      = dim3(grid[g]);
+        kp.blockDim     = dim3(block);
+        kp.kernelParams = args;
+        hipGraphNode_t n = nullptr;
+        hipGraphNode_t dep = g == BG_FSST ? n_prep : n_reset;
+        HIP_TRY(hipGraphAddKernelNode(&n, gr, &dep, 1, &kp));
+      }
+      HIP_TRY(hipGraphInstantiate(&b->gexec, gr, nullptr, nullptr, 0));
+      b->graph_arena   = b->d_arena;
+      b->graph_mask    = mask;
+      b->graph_gridopt = b->grid;
     }
     HIP_TRY(hipGraphLaunch(b->gexec, s));
-  } else {
+  } else {  // also a batch with no entries: its columns' look-back state is cleared
     HIP_TRY(launch_batch_reset(q, s));
     if (strings) HIP_TRY(launch_batch_fsst_prep(q, s));
-    HIP_TRY(launch_batch(items, q, grid, strings, s));
+    for (uint32_t g = 0; g < BG_COUNT; ++g)
+      if (mask >> g & 1u) HIP_TRY(launch_batch_group(items, q, g, std::min(grid[g], cnt[g]), s));
   }
   // a column of no rows has no chunk to write its closing offset
   for (const auto& c : b->cols)
@@ -370,7 +503,8 @@ int batch_launch(sdr_batch* b, hipStream_t s)
   }
   // the host reads string totals and null counts from a pinned copy once
   // the launch has finished (batch_collect)
-  HIP_TRY(hipMemcpyAsync(b->h_readback, b->d_arena + b->off_state, b->readback_bytes, hipMemcpyDeviceToHost, s));
+  if (!b->cols.empty() || !b->masks.empty())  // otherwise there is nothing to write back
+    HIP_TRY(hipMemcpyAsync(b->h_readback, b->d_arena + b->off_state, b->readback_bytes, hipMemcpyDeviceToHost, s));
   if (!b->launch_done) HIP_TRY(hipEventCreateWithFlags(&b->launch_done, hipEventDisableTiming));
   HIP_TRY(hipEventRecord(b->launch_done, s));
   b->last_stream = s;
@@ -448,9 +582,9 @@ sdr_batch* thread_batch()
 int decode_table_persistent(sdr_column* cols, uint32_t n_cols, hipStream_t s)
 {
   sdr_batch* b = thread_batch();
-  int rc = wait_upload(b);
-  if (rc) return rc;
+  int rc = SDR_OK;
   b->items.clear();
+  b->item_group.clear();
   b->cols.clear();
   b->masks.clear();
   b->launched = false;  // results of a launch nobody collected are dropped
@@ -464,9 +598,9 @@ int decode_table_persistent(sdr_column* cols, uint32_t n_cols, hipStream_t s)
 int decode_strings_persistent(sdr_string_column* col, hipStream_t s)
 {
   sdr_batch* b = thread_batch();
-  int rc = wait_upload(b);
-  if (rc) return rc;
+  int rc = SDR_OK;
   b->items.clear();
+  b->item_group.clear();
   b->cols.clear();
   b->masks.clear();
   b->launched = false;  // results of a launch nobody collected are dropped
@@ -516,8 +650,6 @@ int sdr_batch_add_table(sdr_batch* b, sdr_column* cols, uint32_t n_cols)
   if (!b) return fail(SDR_E_INVALID_ARG, "batch is NULL");
   int rc = validate_fixed_columns(cols, n_cols);
   if (rc) return rc;
-  rc = wait_upload(b);
-  if (rc) return rc;
   for (uint32_t c = 0; c < n_cols; ++c) cols[c].null_count = 0;
   return batch_add_fixed(b, cols, n_cols);
 }
@@ -528,14 +660,21 @@ int sdr_batch_add_strings(sdr_batch* b, sdr_string_column* col)
   if (!b) return fail(SDR_E_INVALID_ARG, "batch is NULL");
   int rc = validate_string_column(col);
   if (rc) return rc;
-  rc = wait_upload(b);
-  if (rc) return rc;
   col->chars_used = 0;
   col->null_count = 0;
   return batch_add_strings(b, col);
 }
 
-int sdr_batch_num_entries(const sdr_batch* b) { return b ? static_cast<int>(b->items.size()) : SDR_E_INVALID_ARG; }
+int sdr_batch_num_entries(const sdr_batch* b)
+{
+  if (!b) return SDR_E_INVALID_ARG;
+  // entries the added work makes with the current options
+  uint32_t per[BG_COUNT], entries[BG_COUNT], mask = 0;
+  if (batch_plan(const_cast<sdr_batch*>(b), per, entries, &mask)) return SDR_E_HIP;
+  size_t n = 0;
+  for (uint32_t g = 0; g < BG_COUNT; ++g) n += entries[g];
+  return static_cast<int>(n);
+}
 
 int sdr_batch_launch(sdr_batch* b, sdr_stream_t stream)
 {
@@ -553,9 +692,8 @@ int sdr_batch_clear(sdr_batch* b)
 {
   clear_error();
   if (!b) return fail(SDR_E_INVALID_ARG, "batch is NULL");
-  int rc = wait_upload(b);
-  if (rc) return rc;
   b->items.clear();
+  b->item_group.clear();
   b->cols.clear();
   b->masks.clear();
   b->launched = false;  // results of a launch nobody collected are dropped

@@ -55,29 +55,60 @@ struct BatchItem {
 };
 static_assert(sizeof(BatchItem) == 64, "BatchItem layout");
 
-// Queue header read by the kernel: the item count and the claim counter.
+// Queue groups: entries of one decoder and value width, drained by one
+// specialised persistent kernel (k_batch.hip).  Entries are laid out group
+// by group; within a group in the order they were added.
+enum BatchGroup : uint32_t {
+  BG_FSST = 0,
+  BG_BP1, BG_BP2, BG_BP4, BG_BP8,
+  BG_RLE1, BG_RLE2, BG_RLE4, BG_RLE8,
+  BG_ALP4, BG_ALP8, BG_ALPRD4, BG_ALPRD8,
+  BG_COPY,  // UNCOMPRESSED and CONSTANT byte ranges
+  BG_COUNT
+};
+
+// A claim counter on a cache line of its own: the group kernels run at once
+// and each claims from its own counter.
+struct alignas(128) BatchCounter {
+  uint32_t v;
+};
+
+// Queue header read by the kernels.
 struct BatchQueue {
+  BatchCounter claimed[BG_COUNT];  // next unclaimed entry of each group (reset per launch)
+  uint32_t first[BG_COUNT];        // each group's entries: [first, first + count)
+  uint32_t count[BG_COUNT];
   uint32_t n_items;
-  uint32_t claimed;
-  uint32_t n_fsst_status;  // status words to clear in `fsst_status`
   uint32_t error;
-  uint64_t* fsst_status;   // contiguous FsstState storage of all FSST columns
-  uint32_t n_fsst_segs;    // row ranges of all FSST columns, for the pre-pass
+  uint32_t n_fsst_status;  // status words to clear in `fsst_status`
   uint32_t spin_limit;     // look-back wait bound of the FSST chunks
+  uint32_t n_fsst_segs;    // row ranges of all FSST columns, for the pre-pass
+  uint32_t bp_chunk_rows;  // most rows a bitpacking wavefront stages at once
+  uint64_t* fsst_status;   // contiguous FsstState storage of all FSST columns
   const FsstSeg* fsst_segs;      // their first_chunk index fsst_chunks and fsst_info
   FsstChunk* fsst_chunks;
   FsstChunkInfo* fsst_info;
   FsstSymTab* fsst_tabs;
 };
 
+// Fixed grids of the reset and FSST pre-pass kernels (both loop), so one
+// recorded launch serves every batch.
+constexpr uint32_t BATCH_RESET_GRID    = 64;
+constexpr uint32_t BATCH_PREP_GRID     = 512;
+constexpr uint32_t BATCH_PREP_THREADS  = 256;  // the FSST pre-pass workgroup
+constexpr uint32_t BATCH_WAVES         = 4;    // wavefronts of a fixed-width group block
+constexpr uint32_t BATCH_XCDS          = 8;    // XCDs workgroups are dispatched to round-robin;
+                                               // fixed-width group grids are a multiple
+
 hipError_t launch_batch_reset(BatchQueue* q, hipStream_t s);
-// FSST segment pre-pass over the queue's segments (a fixed grid that loops,
-// so one recorded launch serves every batch).
+// FSST segment pre-pass over the queue's segments.
 hipError_t launch_batch_fsst_prep(BatchQueue* q, hipStream_t s);
-// `strings`: the queue holds VARCHAR (BK_FSST) entries; without them a
-// kernel that carries no FSST state runs, at higher occupancy.
-hipError_t launch_batch(const BatchItem* items, BatchQueue* q, uint32_t grid, bool strings, hipStream_t s);
-// Blocks per CU the persistent kernel can keep resident.
-int batch_occupancy_blocks_per_cu(bool strings);
+// The persistent kernel of group g over `grid` blocks.
+hipError_t launch_batch_group(const BatchItem* items, BatchQueue* q, uint32_t g, uint32_t grid, hipStream_t s);
+// Kernel, block size and resident blocks per CU of group g (for graph nodes
+// and grid sizing).
+hipError_t batch_group_launch_params(uint32_t g, const void** func, uint32_t* block, uint32_t* blocks_per_cu);
+const void* batch_fsst_prep_kernel();
+const void* batch_reset_kernel();
 
 }  // namespace sdr

@@ -278,15 +278,16 @@ int collect(sdr_server* sv)
   for (auto& sl : sv->slots) {
     if (!sl.busy) continue;
     int rc = SDR_OK;
+    // the end event is the slot's last work (after a batch's read-back), so
+    // once it has passed the launch is done and its times are readable
+    const hipError_t e = hipEventQuery(sl.ev_end);
+    if (e == hipErrorNotReady) continue;
+    if (e != hipSuccess) return fail(SDR_E_HIP, "server launch: %s", hipGetErrorString(e));
     if (sl.batch) {
       const int r = batch_ready(sl.batch);
       if (r < 0) return r;
-      if (r == 0) continue;
       (void)batch_collect(sl.batch);  // per-column outcomes: batch_column_status
     } else {
-      const hipError_t e = hipEventQuery(sl.ev_end);
-      if (e == hipErrorNotReady) continue;
-      if (e != hipSuccess) return fail(SDR_E_HIP, "server launch: %s", hipGetErrorString(e));
       rc = sl.launch_status;
     }
     complete(sv, sl, rc);
