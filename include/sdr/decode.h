@@ -404,6 +404,25 @@ int sdr_decode_strings_column(sdr_string_column* col, sdr_stream_t stream);
 int sdr_decode_strings_column_mode(sdr_string_column* col, sdr_stream_t stream,
                                    uint32_t launch_mode);
 
+/* Bytes of the fixed per-device scratch array of
+ * sdr_decode_strings_column_noalloc. */
+#define SDR_NOALLOC_SCRATCH_BYTES (16u << 20)
+
+/* sdr_decode_strings_column that allocates nothing.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Outputs are the caller's, as in sdr_decode_strings_column;
+ * the decode's own device memory (range descriptors, chunk descriptors,
+ * symbol tables, look-back state, null counter) lives in one array of
+ * SDR_NOALLOC_SCRATCH_BYTES in the library's code object, which each GPU the
+ * library runs on holds until the process ends.  A column needing more
+ * (about 2.3 KiB per segment plus 56 bytes per 1024 rows) returns
+ * SDR_E_CAPACITY before any device work.  Every call on a GPU shares that
+ * GPU's array: calls that overlap on one GPU, on any streams or threads,
+ * race and must be serialised by the caller.  Synchronises `stream` once,
+ * as sdr_decode_strings_column does. */
+int sdr_decode_strings_column_noalloc(sdr_string_column* col, sdr_stream_t stream);
+
 /* ---- row ids --------------------------------------------------------------- */
 
 /* Rows [first_row, first_row + count) of a table, written as int64 to

@@ -11,9 +11,39 @@
 
 #include <algorithm>
 #include <cstdint>
+#include <mutex>
 #include <vector>
 
+// Device memory of the allocation-free string decode
+// (sdr_decode_strings_column_noalloc): a fixed array in the code object, one
+// copy per GPU, held until the process ends.  Host code never uses its name
+// as an address: the per-device address comes from hipGetSymbolAddress.
+static __device__ __align__(256) uint8_t scratch[SDR_NOALLOC_SCRATCH_BYTES];
+
 namespace sdr {
+
+namespace {
+constexpr int SCRATCH_DEVICES = 64;
+std::mutex g_scratch_mu;
+uint8_t* g_scratch_ptr[SCRATCH_DEVICES] = {};
+
+// This device's copy of `scratch`, looked up once per device (with that
+// device current).
+int scratch_address(uint8_t** out)
+{
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  if (dev < 0 || dev >= SCRATCH_DEVICES) return fail(SDR_E_HIP, "device %d out of range", dev);
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_scratch_ptr[dev] = static_cast<uint8_t*>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return SDR_OK;
+}
+}  // namespace
 
 int validate_string_column(const sdr_string_column* col)
 {
@@ -46,7 +76,11 @@ int validate_string_column(const sdr_string_column* col)
   return SDR_OK;
 }
 
-int decode_strings(sdr_string_column* col, hipStream_t s, uint32_t mode)
+// `scratch_mem`: null to take the call's device memory from the
+// stream-ordered pool, else the per-device scratch array (capacity
+// SDR_NOALLOC_SCRATCH_BYTES), in which case nothing is allocated: the range
+// descriptors go up with a plain copy and every clear is a fill kernel.
+int decode_strings(sdr_string_column* col, hipStream_t s, uint32_t mode, uint8_t* scratch_mem = nullptr)
 {
   clear_error();
   if (mode != SDR_LAUNCH_BATCHED && mode != SDR_LAUNCH_PER_SEGMENT && mode != SDR_LAUNCH_PERSISTENT)
@@ -58,7 +92,7 @@ int decode_strings(sdr_string_column* col, hipStream_t s, uint32_t mode)
   col->null_count = 0;
   const uint32_t n = col->total_rows;
   if (n == 0) {
-    if (col->d_offsets) HIP_TRY(hipMemsetAsync(col->d_offsets, 0, sizeof(int32_t), s));
+    if (col->d_offsets) HIP_TRY(launch_fill_u8(reinterpret_cast<uint8_t*>(col->d_offsets), 0, sizeof(int32_t), s));
     return SDR_OK;
   }
 
@@ -96,8 +130,18 @@ int decode_strings(sdr_string_column* col, hipStream_t s, uint32_t mode)
   const size_t state_bytes = al(sizeof(FsstState) + sizeof(uint64_t) * n_chunks);
   const size_t total_bytes = seg_bytes + chunk_bytes + info_bytes + tab_bytes + state_bytes + sizeof(unsigned long long);
   uint8_t* d = nullptr;
-  rc = device_alloc(reinterpret_cast<void**>(&d), total_bytes, s);
-  if (rc) return rc;
+  if (scratch_mem) {
+    if (total_bytes > SDR_NOALLOC_SCRATCH_BYTES)
+      return fail(SDR_E_CAPACITY, "string decode needs %llu bytes of descriptors; the no-allocation scratch holds %u",
+                  static_cast<unsigned long long>(total_bytes), SDR_NOALLOC_SCRATCH_BYTES);
+    d = scratch_mem;
+  } else {
+    rc = device_alloc(reinterpret_cast<void**>(&d), total_bytes, s);
+    if (rc) return rc;
+  }
+  auto release = [&]() {
+    if (!scratch_mem) device_free(d, s);
+  };
   auto* d_segs   = reinterpret_cast<FsstSeg*>(d);
   auto* d_chunks = reinterpret_cast<FsstChunk*>(d + seg_bytes);
   auto* d_info   = reinterpret_cast<FsstChunkInfo*>(d + seg_bytes + chunk_bytes);
@@ -107,10 +151,15 @@ int decode_strings(sdr_string_column* col, hipStream_t s, uint32_t mode)
   auto* d_nulls  = reinterpret_cast<unsigned long long*>(d_rest + state_bytes);
 
   hipError_t e = hipSuccess;
-  rc = upload_async(d_segs, hsegs.data(), hsegs.size() * sizeof(FsstSeg), s);
-  if (rc) {
-    device_free(d, s);
-    return rc;
+  if (scratch_mem) {
+    e = hipMemcpyAsync(d_segs, hsegs.data(), hsegs.size() * sizeof(FsstSeg), hipMemcpyHostToDevice, s);
+    if (e != hipSuccess) return fail(SDR_E_HIP, "string decode: %s", hipGetErrorString(e));
+  } else {
+    rc = upload_async(d_segs, hsegs.data(), hsegs.size() * sizeof(FsstSeg), s);
+    if (rc) {
+      release();
+      return rc;
+    }
   }
   e = launch_fsst_reset(d_state, n_chunks, s);
   if (e == hipSuccess)
@@ -131,7 +180,7 @@ int decode_strings(sdr_string_column* col, hipStream_t s, uint32_t mode)
     if (e == hipSuccess) {
       rc = build_null_mask(col->validity, col->n_validity, n, col->d_null_mask, d_nulls, s);
       if (rc) {
-        device_free(d, s);
+        release();
         return rc;
       }
     }
@@ -144,7 +193,7 @@ int decode_strings(sdr_string_column* col, hipStream_t s, uint32_t mode)
   if (e == hipSuccess) e = hipMemcpyAsync(&h, d_state, sizeof(h), hipMemcpyDeviceToHost, s);
   if (e == hipSuccess && masked) e = hipMemcpyAsync(&h_nulls, d_nulls, sizeof(h_nulls), hipMemcpyDeviceToHost, s);
   if (e == hipSuccess) e = hipStreamSynchronize(s);
-  device_free(d, s);
+  release();
   if (e != hipSuccess) return fail(SDR_E_HIP, "string decode: %s", hipGetErrorString(e));
   if (h.error) return fail(SDR_E_HIP, "string decode: offset look-back did not complete");
   col->chars_used = h.total;
@@ -184,6 +233,15 @@ int sdr_decode_strings_column_mode(sdr_string_column* col, sdr_stream_t stream, 
 int sdr_decode_strings_column(sdr_string_column* col, sdr_stream_t stream)
 {
   return decode_strings(col, static_cast<hipStream_t>(stream), SDR_LAUNCH_BATCHED);
+}
+
+int sdr_decode_strings_column_noalloc(sdr_string_column* col, sdr_stream_t stream)
+{
+  clear_error();
+  uint8_t* mem = nullptr;
+  const int rc = scratch_address(&mem);
+  if (rc) return rc;
+  return decode_strings(col, static_cast<hipStream_t>(stream), SDR_LAUNCH_BATCHED, mem);
 }
 
 }  // extern "C"

@@ -18,6 +18,7 @@ DECLARED = [
     "sdr_decode_table", "sdr_decode_table_mode",
     "sdr_decode_bitpacking_data", "sdr_decode_rle_data", "sdr_decode_alp_data", "sdr_decode_alprd_data",
     "sdr_strings_upper_bound", "sdr_decode_strings_column", "sdr_decode_strings_column_mode",
+    "sdr_decode_strings_column_noalloc",
     "sdr_plan_create", "sdr_plan_launch", "sdr_plan_launch_codec", "sdr_plan_num_launches",
     "sdr_plan_null_counts", "sdr_plan_destroy",
     "sdr_batch_create", "sdr_batch_set_option", "sdr_batch_add_table", "sdr_batch_add_strings",
@@ -77,6 +78,8 @@ def load() -> C.CDLL:
     lib.sdr_decode_strings_column.restype = i32
     lib.sdr_decode_strings_column_mode.argtypes = [P(_abi.StringColumn), vp, u32]
     lib.sdr_decode_strings_column_mode.restype = i32
+    lib.sdr_decode_strings_column_noalloc.argtypes = [P(_abi.StringColumn), vp]
+    lib.sdr_decode_strings_column_noalloc.restype = i32
     lib.sdr_plan_create.argtypes = [P(_abi.Column), u32, u32, vp, P(vp)]
     lib.sdr_plan_create.restype = i32
     lib.sdr_plan_launch.argtypes = [vp, vp]
