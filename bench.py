@@ -1,21 +1,25 @@
 #!/usr/bin/env python3
 """Decode throughput of DuckDB storage segments on one MI355X.
 
-Headline workload ("int_date_mix_64M", the integer/date mix the metric is
-quoted on): 64 Mi rows of three columns in DuckDB-format segments
-(self-encoded; DuckDB is not installed): an INT64 column of runs (geometric
-length, mean 16, values in [0, 2^40)) encoded RLE, an INT32 quantity column
-(1..50) and a DATE column (7 years, sorted in 2048-row groups), both
-BITPACKING.  Inputs are resident in HBM before timing starts.
+Headline workload ("served_mixed_10k", the served mixed-codec stream the
+target is quoted on): 10,000 decode requests of log-uniform size (2 Ki to 16
+Mi rows; 40% BITPACKING INT32, 20% RLE INT64, 25% ALP DOUBLE, 15% FSST
+VARCHAR) over DuckDB-format segments (self-encoded; DuckDB is not installed)
+resident in the decode server's device pool, Poisson arrivals at 0.8x the
+measured capacity, a 200 GB admission budget.  A step is one pass of the
+whole stream through the server (persistent group-kernel launches, continuous
+batching); `value` is decoded bytes per second over the timed passes.
 
-A step is one decode of the three columns through sdr_decode_table (the
-drop-in entry, batched launches).  Besides the headline the run reports the
-same columns through the persistent batch and the plain one-launch-per-
-segment path (p50/p99 step latency of each), per-kernel durations measured
-with HIP events inside a timed loop on the launch stream, the roofline of
-the dominant kernel, the other single-GPU workloads (8M-row bitpacking,
-ALP/ALP-RD doubles, FSST strings) and the CPU baseline.  Prints one JSON
-line.
+Beside it the run reports the rate sweep (0.5x, 0.95x), the baselines (one
+drop-in launch per request, one launch per segment, each at 0.8x its own
+capacity), configs 1-4 (8M-row bitpacking; the 64M-row integer/date mix;
+ALP/ALP-RD doubles; FSST strings) through the batched, persistent and
+one-launch-per-segment paths with per-kernel durations, the PCIe-inclusive
+staging leg, and the CPU baseline per config.  The default run measures the
+served stream and the other legs in two child processes, each with a
+runtime and device memory of its own; this process only merges their
+results.  `--workload NAME` makes one of configs 1-4 the headline instead.
+Prints one JSON line.
 """
 from __future__ import annotations
 
@@ -34,7 +38,9 @@ sys.path.insert(0, str(ROOT))
 
 METRIC = "decoded GB/s and % HBM roofline vs DuckDB CPU; p50/p99 latency batched vs plain"
 HBM_PEAK_GBS = 8000.0
-WORKLOADS = ("int_date_mix_64M", "copy_256M", "bp_int32_8M", "alp_double_32M", "fsst_varchar_16M")
+SERVED = "served_mixed_10k"
+WORKLOADS = (SERVED, "int_date_mix_64M", "copy_256M", "bp_int32_8M", "alp_double_32M", "fsst_varchar_16M")
+CONFIGS = ("bp_int32_8M", "int_date_mix_64M", "alp_double_32M", "fsst_varchar_16M")  # configs 1-4
 
 
 def log(*a):
@@ -278,7 +284,7 @@ def rounded(x, nd=4):
     if isinstance(x, float):
         return round(x, nd)
     if isinstance(x, dict):
-        return {k: rounded(v, nd) for k, v in x.items()}
+        return {k: rounded(v, nd) for k, v in x.items() if k != "latencies_ms"}
     return x
 
 
@@ -288,7 +294,7 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--rows", type=int, default=None)
-    ap.add_argument("--workload", choices=WORKLOADS, default="int_date_mix_64M")
+    ap.add_argument("--workload", choices=WORKLOADS, default=SERVED)
     ap.add_argument("--only", choices=("all", "batched", "served"), default="all",
                     help="batched: the drop-in steps and the per-kernel loop only; served: the served "
                          "mixed-codec stream only (both for profiler runs)")
@@ -300,12 +306,18 @@ def main() -> int:
                     help="resident pool size of the served stream (copies of its columns at distinct addresses)")
     ap.add_argument("--served-requests", type=int, default=10000,
                     help="requests of the served mixed-codec stream (0: skip it)")
+    ap.add_argument("--leg", choices=("served", "extra"), default=None,
+                    help="run one leg of the default run and write its JSON to --leg-out")
+    ap.add_argument("--leg-out", default=None)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported: the decode path runs on a "
               f"single MI355X. Across GPUs it shards by placement (one process per GPU decoding its own "
               f"segments; decode never communicates)", file=sys.stderr)
         return 2
+
+    if args.leg is None and args.workload == SERVED and args.only == "all":
+        return run_legs(args)
 
     import torch
 
@@ -315,10 +327,13 @@ def main() -> int:
     lib = L.load()
     tuning = apply_tuning(lib)
     dev = torch.device("cuda:0")
-    if args.only == "served":
-        sr = served(lib, args.served_requests, baselines=not args.no_served_baselines,
-                    resident_gb=args.served_resident_gb)
-        print(json.dumps({"workload": "served", **sr}), flush=True)
+    if args.leg is not None:
+        res = (served_leg if args.leg == "served" else extra_leg)(args, lib, tuning, dev)
+        Path(args.leg_out).write_text(json.dumps(res))
+        return 0
+    if args.workload == SERVED or args.only == "served":
+        s = served_leg(args, lib, tuning, dev)
+        print(json.dumps({"workload": SERVED, **s["served"], "cpu_baseline": s["cpu"]}), flush=True)
         return 0
     work = build_workload(args.workload, args.rows)
 
@@ -358,13 +373,9 @@ def main() -> int:
     if not args.no_extra:
         staged_r = staged(lib, work, dev, max(5, args.steps // 5))
 
-    served_r = None
-    if args.served_requests and args.workload == "int_date_mix_64M":
-        served_r = served(lib, args.served_requests)
-
     cpu = None
     if not args.no_cpu_baseline:
-        cpu = cpu_baseline(work)
+        cpu = cpu_configs({args.workload: work})
 
     cols = {"int_date_mix_64M": ["RLE INT64 runs(mean 16)", "BITPACKING INT32 1..50", "BITPACKING DATE sorted/2048"],
             "copy_256M": ["UNCOMPRESSED INT64"]}.get(args.workload, [n for n, _, _ in work])
@@ -399,7 +410,6 @@ def main() -> int:
         "bit_exact": r["bit_exact"] and r["plain"]["bit_exact"] and r["persistent"]["bit_exact"],
         "tuning": tuning,
         "workloads": {k: rounded({kk: vv for kk, vv in v.items() if kk != "kernels"}) for k, v in extra.items()},
-        "served": served_r,
         "staged_from_file": staged_r,
         "roofline": None if dom is None else {
             "bound": "hbm", "kernel": dom_name, "achieved": round(dom["gbs"], 1), "peak": HBM_PEAK_GBS,
@@ -409,6 +419,130 @@ def main() -> int:
     }
     print(json.dumps(line), flush=True)
     return 0
+
+
+def served_leg(args, lib, tuning, dev) -> dict:
+    """The served mixed-codec stream (config 5) and its CPU baseline."""
+    sr = served(lib, args.served_requests, steps=args.steps, warmup=args.warmup,
+                baselines=not args.no_served_baselines, resident_gb=args.served_resident_gb)
+    pool, specs = sr.pop("_pool"), sr.pop("_specs")
+    cpu = None if args.no_cpu_baseline else cpu_served(pool, specs)
+    return {"served": sr, "cpu": cpu, "tuning": tuning}
+
+
+def extra_leg(args, lib, tuning, dev) -> dict:
+    """Configs 1-4 (batched, persistent and one-launch-per-segment paths, per-kernel durations), the
+    PCIe-inclusive staging leg and the configs' CPU baselines."""
+    extra, works, staged_r = {}, {}, None
+    if not args.no_extra:
+        for wl in CONFIGS:
+            log(f"[bench] measuring {wl}")
+            w = build_workload(wl)
+            er, eres = measure(lib, w, dev, max(5, args.steps // 2), max(5, args.warmup), 3)
+            del eres
+            extra[wl] = er
+            works[wl] = w
+            log(f"[bench] {wl}: batched {er['ms_per_step']:.3f} ms/step ({er['decoded_gbs']:.0f} GB/s), persistent "
+                f"{er['persistent']['ms_per_step']:.3f} (p50 {er['persistent']['p50_ms']:.3f}), plain "
+                f"{er['plain']['ms_per_step']:.3f}, bit-exact={er['bit_exact']}")
+        staged_r = staged(lib, works["int_date_mix_64M"], dev, max(5, args.steps // 5))
+    elif not args.no_cpu_baseline:
+        works = {wl: build_workload(wl) for wl in CONFIGS}
+    cpu = None if args.no_cpu_baseline else cpu_configs(works)
+    return {"workloads": {k: rounded(v) for k, v in extra.items()}, "staged_from_file": staged_r, "cpu": cpu,
+            "tuning": tuning}
+
+
+def run_legs(args) -> int:
+    """The default run: the served stream, then configs 1-4 with the staging leg, each in a child
+    process of its own (this process never touches the GPU), merged into one JSON line."""
+    import subprocess
+    import tempfile
+
+    common = ["--steps", str(args.steps), "--warmup", str(args.warmup),
+              "--served-requests", str(args.served_requests), "--served-resident-gb", str(args.served_resident_gb)]
+    for flag in ("no_served_baselines", "no_cpu_baseline", "no_extra"):
+        if getattr(args, flag):
+            common.append("--" + flag.replace("_", "-"))
+    legs = {}
+    for leg in ("served", "extra"):
+        if leg == "extra" and args.no_extra and args.no_cpu_baseline:
+            continue
+        fd, path = tempfile.mkstemp(prefix=f"sdr_bench_{leg}_", suffix=".json")
+        os.close(fd)
+        try:
+            log(f"[bench] leg {leg}: child process")
+            rc = subprocess.run([sys.executable, "-u", str(Path(__file__).resolve()), "--leg", leg,
+                                 "--leg-out", path, *common], stdout=sys.stderr).returncode
+            legs[leg] = json.loads(Path(path).read_text()) if rc == 0 else {"rc": rc}
+        finally:
+            os.unlink(path)
+        if rc != 0:
+            log(f"[bench] leg {leg} exited with status {rc}")
+            if leg == "served":
+                return 1
+    print(json.dumps(headline_line(args, legs["served"], legs.get("extra"))), flush=True)
+    return 0
+
+
+def headline_line(args, s: dict, x: dict | None) -> dict:
+    """The JSON line: the served stream is the value, the other legs ride along."""
+    sr = s["served"]
+    main = sr["persistent"]
+    st = main["server_stats"]
+    alg = main["decoded_bytes"] + main["compressed_bytes"]
+    launch_ms = st["gpu_union_ms"] / max(1, st["launches"])
+    x = x or {}
+    ik = (x.get("workloads") or {}).get("int_date_mix_64M", {}).get("kernels", {})
+    dom = max(ik, key=lambda k: ik[k]["us"]) if ik else None
+    per = {**((x.get("cpu") or {}).get("per_config") or {}), **((s.get("cpu") or {}).get("per_config") or {})}
+    cpu = None
+    if s.get("cpu"):
+        cpu = {**s["cpu"], "per_config": per,
+               "sample": s["cpu"]["sample"] + ("; configs 1-4: each whole config decoded repeatedly for about "
+                                               f"{CPU_BUDGET_S:.0f} s" if x.get("cpu") else "")}
+    return {
+        "metric": METRIC,
+        "value": round(main["decoded_gbs"], 2),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": sr["passes"],
+        "warmup": args.warmup,
+        "ms_per_step": round(main["span_ms"] / sr["passes"], 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32+int64+f64+varchar",
+        "data": "synthetic (DuckDB-format segments, self-encoded; DuckDB absent)",
+        "config": {"workload": SERVED, "requests": main["requests"] // sr["passes"],
+                   "rows": "log-uniform 2Ki..16Mi per request",
+                   "mix": "40% BITPACKING INT32, 20% RLE INT64, 25% ALP DOUBLE, 15% FSST VARCHAR",
+                   "arrivals": f"Poisson at {sr['rate_req_per_s']} req/s (0.8x measured capacity)",
+                   "budget_bytes": 200 * 10 ** 9, "resident_gb": sr["resident_gb"],
+                   "path": "sdr_server, persistent group-kernel launches on 2 slot streams (continuous batching)"},
+        "job_achieved_gbs": round(main["achieved_gbs"], 2),
+        "job_frac_hbm": sr["frac_hbm"],
+        "latency_ms": {"served": {"p50": round(main["p50_ms"], 3), "p99": round(main["p99_ms"], 3)},
+                       **{k: {"p50": v["p50_ms"], "p99": v["p99_ms"]} for k, v in sr["baselines"].items()}},
+        "bit_exact": main["verified_ok"] == main["verified"] and main["mismatched_requests"] == 0,
+        "tuning": s["tuning"],
+        "served": sr,
+        "workloads": x.get("workloads"),
+        "staged_from_file": x.get("staged_from_file"),
+        "legs": {"extra": "ok" if x and "rc" not in x else (f"exit status {x['rc']}" if x else "skipped")},
+        "roofline": {
+            "bound": "hbm", "kernel": "persistent batch launch (group kernels), served stream",
+            "achieved": round(alg / (st["gpu_union_ms"] * 1e-3) / 1e9, 1) if st["gpu_union_ms"] > 0 else None,
+            "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(alg / (st["gpu_union_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4) if st["gpu_union_ms"] > 0 else None,
+            "traffic": None,
+            "alg_bytes_per_launch": round(alg / max(1, st["launches"])), "avg_ms_per_launch": round(launch_ms, 4),
+            "note": "algorithmic bytes (compressed read + decoded written) over the time at least one launch ran, "
+                    "from each launch's start/end HIP events on its slot stream"},
+        "roofline_int_date_mix_64M": None if dom is None else {
+            "kernel": dom, **{k: round(v, 2) if isinstance(v, float) else v for k, v in ik[dom].items()}},
+        "cpu_baseline": cpu,
+    }
 
 
 def staged(lib, work, dev, steps):
@@ -484,14 +618,19 @@ def staged(lib, work, dev, steps):
             "note": "file in the page cache; host-clock timing around sdr_stage_file + sdr_decode_table"}
 
 
-def served(lib, n_requests, baselines=True, resident_gb=32.0):
+def served(lib, n_requests, steps=0, warmup=0, baselines=True, resident_gb=32.0):
     """The served mixed-codec stream: requests of log-uniform size
     (2 Ki..16 Mi rows), 40% BITPACKING INT32 / 20% RLE INT64 / 25% ALP DOUBLE
-    / 15% FSST VARCHAR over segments resident in the server's pool, Poisson
-    arrivals at 0.8x the measured capacity of the persistent path, 200 GB
-    admission budget.  Baselines (one drop-in launch per request, one
-    launch per segment) serve the first requests of the same stream at the
-    same rate."""
+    / 15% FSST VARCHAR over segments resident in the server's pool (copies at
+    distinct addresses, about `resident_gb` GB), 200 GB admission budget.
+
+    Capacity is the throughput with a prefix of the stream queued at once.
+    The headline serves the whole stream with Poisson arrivals at 0.8x that
+    capacity: `warmup` untimed passes, then `steps` timed passes (a step is
+    one pass of the stream), every 16th request checked bit-exact on the
+    device.  A sweep serves a prefix at 0.5x and 0.95x.  Each baseline (one
+    drop-in launch per request, one launch per segment) is served at 0.8x
+    of its own capacity, so its p50/p99 are those of a load it sustains."""
     import copy
 
     import torch
@@ -504,97 +643,261 @@ def served(lib, n_requests, baselines=True, resident_gb=32.0):
     cfg = A.ServerConfig(200 * 10 ** 9, 2, A.LAUNCH_PERSISTENT, 0, 0)
     server = C.c_void_p()
     L.check(lib.sdr_server_create(C.byref(cfg), C.byref(server)))
+
+    def capacity(mode, m, srv=None, verify_every=16):
+        sp = copy.deepcopy(specs[:m])
+        S.poisson_arrivals(sp, 0)
+        r = S.run(lib, pool, sp, mode, server=srv, verify_every=verify_every)
+        return r, m / (r["span_ms"] * 1e-3)
+
     try:
         pool = S.Pool(lib, server, max_rows=16 << 20, n_tails=48, log=log, resident_bytes=int(resident_gb * 1e9))
         specs = S.make_specs(pool, n_requests)
-        log(f"[bench] served: pool staged ({pool.staged_bytes / 1e9:.2f} GB resident) in {time.time() - t:.1f}s")
-        # capacity: a prefix of the stream, everything queued at once
+        log(f"[bench] served: pool staged ({pool.staged_bytes / 1e9:.2f} GB resident, {pool.replicas} copies) "
+            f"in {time.time() - t:.1f}s")
         m = min(len(specs), 1000)
-        cap_specs = copy.deepcopy(specs[:m])
-        S.poisson_arrivals(cap_specs, 0)
-        for _ in range(2):  # the first pass reserves the pool's output slabs
-            cap = S.run(lib, pool, cap_specs, A.LAUNCH_PERSISTENT, server=server, verify_every=16)
-        log(f"[bench] served capacity stats: {json.dumps(rounded(cap['server_stats'], 2))}")
-        cap_rps = m / (cap["span_ms"] * 1e-3)
-        mean_bytes = cap["decoded_bytes"] / m
+        capacity(A.LAUNCH_PERSISTENT, m, server)  # reserves the pool's output slabs
+        cap, cap_rps = capacity(A.LAUNCH_PERSISTENT, m, server)
         rate = 0.8 * cap_rps
-        # bound the timed stream to about a minute of arrivals
-        n = min(len(specs), max(m, int(60.0 * rate)))
-        run_specs = specs[:n]
+        log(f"[bench] served: capacity {cap_rps:.1f} req/s ({cap['decoded_gbs']:.0f} GB/s decoded, "
+            f"{cap['achieved_gbs'] / HBM_PEAK_GBS:.3f} of HBM); serving {len(specs)} requests at {rate:.1f} req/s, "
+            f"{warmup} warm-up + {steps} timed passes")
+        run_specs = copy.deepcopy(specs)
         S.poisson_arrivals(run_specs, rate)
-        log(f"[bench] served: capacity {cap_rps:.1f} req/s ({cap['decoded_gbs']:.0f} GB/s); "
-            f"serving {n} requests at {rate:.1f} req/s")
-        main = S.run(lib, pool, run_specs, A.LAUNCH_PERSISTENT, server=server, verify_every=8)
-        log(f"[bench] served: {main['decoded_gbs']:.0f} GB/s, p50 {main['p50_ms']:.2f} ms, "
-            f"p99 {main['p99_ms']:.2f} ms, verified {main['verified_ok']}/{main['verified']}")
+        for _ in range(warmup):
+            S.run(lib, pool, run_specs, A.LAUNCH_PERSISTENT, server=server, verify_every=16)
+        passes = []
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(max(1, steps)):
+            passes.append(S.run(lib, pool, run_specs, A.LAUNCH_PERSISTENT, server=server, verify_every=16))
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        main = S.merge_runs(passes)
+        log(f"[bench] served: {main['decoded_gbs']:.0f} GB/s decoded, {main['achieved_gbs'] / HBM_PEAK_GBS:.3f} "
+            f"of HBM, p50 {main['p50_ms']:.2f} ms, p99 {main['p99_ms']:.2f} ms, verified "
+            f"{main['verified_ok']}/{main['verified']}; GPU busy {main['server_stats']['gpu_union_ms']:.1f} of "
+            f"{main['span_ms']:.1f} ms")
         for f in main["failed"]:
             log(f"[bench] served MISMATCH {f}")
+        sweep = {}
+        n_sweep = min(len(specs), 3000)
+        for fr in (0.5, 0.95):
+            sp = copy.deepcopy(specs[:n_sweep])
+            S.poisson_arrivals(sp, fr * cap_rps)
+            sr = S.run(lib, pool, sp, A.LAUNCH_PERSISTENT, server=server, verify_every=16)
+            sweep[f"{fr:.2f}"] = {k: sr[k] for k in ("decoded_gbs", "achieved_gbs", "p50_ms", "p99_ms", "requests")}
+            log(f"[bench] served at {fr:.2f}x capacity: {sr['decoded_gbs']:.0f} GB/s, p50 {sr['p50_ms']:.2f} ms, "
+                f"p99 {sr['p99_ms']:.2f} ms")
+        sweep["0.80"] = {k: main[k] for k in ("decoded_gbs", "achieved_gbs", "p50_ms", "p99_ms", "requests")}
         base = {}
-        for name, mode, nb in ((("one_launch_per_request", A.LAUNCH_BATCHED, n),
-                                ("one_launch_per_segment", A.LAUNCH_PER_SEGMENT, n)) if baselines else ()):
-            bs = copy.deepcopy(run_specs[:nb])
-            S.poisson_arrivals(bs, rate)
-            base[name] = S.run(lib, pool, bs, mode, verify_every=8)
-            log(f"[bench] served baseline {name}: {base[name]['decoded_gbs']:.0f} GB/s, "
-                f"p50 {base[name]['p50_ms']:.1f} ms, p99 {base[name]['p99_ms']:.1f} ms")
+        for name, mode in (("one_launch_per_request", A.LAUNCH_BATCHED),
+                           ("one_launch_per_segment", A.LAUNCH_PER_SEGMENT)) if baselines else ():
+            mb = min(len(specs), 200)
+            bcap, bcap_rps = capacity(mode, mb, None)
+            nb = min(len(specs), 1000)
+            bs = copy.deepcopy(specs[:nb])
+            S.poisson_arrivals(bs, 0.8 * bcap_rps)
+            br = S.run(lib, pool, bs, mode, verify_every=16)
+            base[name] = {**rounded(br), "capacity_req_per_s": round(bcap_rps, 2),
+                          "capacity_gbs": round(bcap["decoded_gbs"], 2), "rate_req_per_s": round(0.8 * bcap_rps, 2)}
+            log(f"[bench] served baseline {name}: capacity {bcap['decoded_gbs']:.0f} GB/s; at 0.8x it "
+                f"{br['decoded_gbs']:.0f} GB/s, p50 {br['p50_ms']:.1f} ms, p99 {br['p99_ms']:.1f} ms")
         torch.cuda.synchronize()
     finally:
         lib.sdr_server_destroy(server)
-    return {
+    st = main["server_stats"]
+    alg = main["decoded_bytes"] + main["compressed_bytes"]
+    union_gbs = alg / (st["gpu_union_ms"] * 1e-3) / 1e9 if st["gpu_union_ms"] > 0 else 0.0
+    out = {
         "workload": "served mixed-codec stream (40% BP INT32, 20% RLE INT64, 25% ALP DOUBLE, 15% FSST VARCHAR; "
-                    "rows log-uniform 2Ki..16Mi; Poisson arrivals; 200 GB budget)",
+                    "rows log-uniform 2Ki..16Mi; Poisson arrivals at 0.8x capacity; 200 GB budget)",
+        "resident_gb": round(pool.staged_bytes / 1e9, 2), "pool_copies": pool.replicas,
         "capacity_req_per_s": round(cap_rps, 2), "capacity_gbs": round(cap["decoded_gbs"], 2),
-        "mean_request_bytes": round(mean_bytes), "rate_req_per_s": round(rate, 2),
+        "capacity_frac_hbm": round(cap["achieved_gbs"] / HBM_PEAK_GBS, 4),
+        "mean_request_bytes": round(cap["decoded_bytes"] / m), "rate_req_per_s": round(rate, 2),
+        "passes": len(passes), "wall_s": round(wall, 3),
         "persistent": rounded(main),
         "frac_hbm": round(main["achieved_gbs"] / HBM_PEAK_GBS, 4),
-        "baselines": {k: rounded(v) for k, v in base.items()},
-        **({"vs_one_launch_per_request": round(main["decoded_gbs"] / base["one_launch_per_request"]["decoded_gbs"], 3),
-            "vs_one_launch_per_segment": round(main["decoded_gbs"] / base["one_launch_per_segment"]["decoded_gbs"], 3)}
-           if baselines else {}),
+        "gpu_busy_gbs": round(union_gbs, 1),
+        "sweep": rounded(sweep),
+        "baselines": base,
     }
+    for name, b in base.items():
+        out[f"vs_{name}"] = {"capacity_ratio": round(cap["decoded_gbs"] / b["capacity_gbs"], 3),
+                             "p99_ms_at_0.8": [round(main["p99_ms"], 3), round(b["p99_ms"], 3)]}
+    out["_pool"], out["_specs"] = pool, specs  # host data for the CPU baseline (not printed)
+    return out
 
 
-def cpu_baseline(work):
-    """The oracle (C restatement of the decoders, 'port') on the host cores,
-    on a bounded sample of the same workload: the first segments of each
-    column up to 8 Mi rows, repeated for about 10 s."""
-    import sirius_amd._abi as A
-    from oracle import oracle as O
+def host_cores():
+    """Logical CPUs, physical cores (from /proc/cpuinfo) and the CPUs this
+    process may run on."""
+    logical = os.cpu_count() or 1
+    phys = set()
+    try:
+        cur = {}
+        for line in open("/proc/cpuinfo"):
+            if ":" in line:
+                k, v = (x.strip() for x in line.split(":", 1))
+                cur[k] = v
+            elif cur:
+                phys.add((cur.get("physical id"), cur.get("core id")))
+                cur = {}
+        if cur:
+            phys.add((cur.get("physical id"), cur.get("core id")))
+    except OSError:
+        pass
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = logical
+    return logical, len(phys) if phys and None not in next(iter(phys)) else None, avail
 
-    lib = O.load()
-    threads = min(16, os.cpu_count() or 1)
-    sample_rows = 8 << 20
-    cols, keep, nbytes = [], [], 0
-    for _, values, enc in work:
-        sel = enc.segs[enc.segs["row_offset"] < sample_rows]
-        segs = [A.Segment(enc.blob.ctypes.data + int(s["offset"]), int(s["bytes_size"]), int(s["row_offset"]),
-                          int(s["row_count"]), 0) for s in sel]
-        n = int(max(int(s["row_offset"]) + int(s["row_count"]) for s in sel))
+
+CPU_BUDGET_S = 2.0  # seconds of CPU decoding per config and thread count
+
+
+class _CpuDecoder:
+    """The oracle's multithreaded C decoders (a restatement of the
+    reference's decoders, labelled 'port': DuckDB is not installed) over
+    host copies of the segments, with all the CPUs this process may use (at
+    most 16, the host share of one GPU here) or with one thread."""
+
+    def __init__(self):
+        from oracle import oracle as O
+
+        self.lib = O.load()
+        self.logical, self.physical, self.avail = host_cores()
+        self.n_all = max(1, min(self.avail, 16))
+
+    def fixed_col(self, enc, segs, n):
+        import sirius_amd._abi as A
+
         out = np.empty(n * enc.type_size, np.uint8)
         runs, k = A.make_runs([(enc.codec, segs)])
         col = A.Column()
         col.type_size, col.total_rows, col.data, col.n_data, col.d_out = enc.type_size, n, runs, 1, out.ctypes.data
-        keep += [k, out]
-        cols.append(col)
-        nbytes += n * enc.type_size
-    reps, elapsed = 0, 0.0
-    t0 = time.perf_counter()
-    while elapsed < 10.0 and reps < 2000:
-        for col in cols:  # per column: the columns of the sample differ in row count
-            rc = lib.oracle_decode_table_mt(C.byref(col), 1, threads)
-            assert rc == 0
-        reps += 1
-        elapsed = time.perf_counter() - t0
-    gbs = nbytes * reps / elapsed / 1e9
-    try:
-        import duckdb  # noqa: F401
+        return col, [k, out], n * enc.type_size
 
-        duck = "duckdb importable on this host (not timed here)"
-    except Exception:
-        duck = "duckdb not installed: baseline is the repo's C restatement (not DuckDB)"
-    log(f"[bench] cpu baseline {gbs:.2f} GB/s on {threads} threads ({reps} reps, {elapsed:.1f}s)")
-    return {"value": round(gbs, 3), "unit": "GB/s", "cores": threads, "kind": "port",
-            "sample": f"first {sample_rows} rows of each column of the workload, decoded {reps}x; {duck}"}
+    def str_col(self, enc, segs, n, n_chars):
+        import sirius_amd._abi as A
+
+        offs = np.empty(n + 1, np.int32)
+        chars = np.empty(max(1, n_chars) + 64, np.uint8)
+        runs, k = A.make_runs([(enc.codec, segs)])
+        col = A.StringColumn()
+        col.total_rows, col.data, col.n_data = n, runs, 1
+        col.d_offsets, col.d_chars, col.chars_capacity = offs.ctypes.data, chars.ctypes.data, len(chars)
+        return col, [k, offs, chars], 4 * (n + 1) + n_chars
+
+    @staticmethod
+    def host_segs(enc, row0=0):
+        import sirius_amd._abi as A
+
+        b = enc.blob.ctypes.data
+        return [A.Segment(b + int(s["offset"]), int(s["bytes_size"]), row0 + int(s["row_offset"]),
+                          int(s["row_count"]), int(s["max_string_length"])) for s in enc.segs]
+
+    def decode(self, col, is_str, threads):
+        if is_str:
+            rc = self.lib.oracle_decode_strings_column_mt(C.byref(col), threads)
+        else:
+            rc = self.lib.oracle_decode_table_mt(C.byref(col), 1, threads)
+        assert rc == 0, rc
+
+    def summary(self, per: dict, head: str, sample: str) -> dict:
+        return {"value": per[head]["gbs_all"], "unit": "GB/s", "cores": self.n_all, "kind": "port",
+                "sample": sample, "threads_all": self.n_all, "logical_cpus": self.logical,
+                "physical_cores": self.physical, "usable_cpus": self.avail, "per_config": per}
+
+
+def cpu_configs(works, budget_s=CPU_BUDGET_S):
+    """CPU baseline per config: the whole config decoded from host memory
+    (working sets of 0.05-1 GB), repeated for about `budget_s`, on all
+    usable CPUs and on one."""
+    import sirius_amd._abi as A
+
+    cd = _CpuDecoder()
+
+    def timed(jobs, threads):
+        elapsed, nbytes = 0.0, 0
+        t0 = time.perf_counter()
+        while elapsed < budget_s:
+            for col, is_str, nb in jobs:
+                cd.decode(col, is_str, threads)
+                nbytes += nb
+            elapsed = time.perf_counter() - t0
+        return nbytes / elapsed / 1e9
+
+    per = {}
+    for name, work in works.items():
+        jobs, keep = [], []
+        for _, values, enc in work:
+            if enc.codec == A.FSST:
+                col, k, nb = cd.str_col(enc, cd.host_segs(enc), enc.n_rows, len(values[0]))
+                jobs.append((col, True, nb))
+            else:
+                col, k, nb = cd.fixed_col(enc, cd.host_segs(enc), enc.n_rows)
+                jobs.append((col, False, nb))
+            keep.append(k)
+        g_all, g_one = timed(jobs, cd.n_all), timed(jobs, 1)
+        per[name] = {"gbs_all": round(g_all, 3), "gbs_1": round(g_one, 3),
+                     "decoded_bytes": int(sum(nb for _, _, nb in jobs))}
+        log(f"[bench] cpu {name}: {g_all:.2f} GB/s on {cd.n_all} threads, {g_one:.2f} GB/s on 1")
+    return cd.summary(per, next(iter(per)), f"each whole config decoded repeatedly for about {budget_s:.0f} s; "
+                      "duckdb not installed, so the repo's multithreaded C restatement of the decoders")
+
+
+def cpu_served(pool, specs, budget_s=CPU_BUDGET_S):
+    """CPU baseline of the served stream: its requests decoded in order
+    from host copies of the pool's segments for about `budget_s`, on all
+    usable CPUs and on one."""
+    import sirius_amd._abi as A
+
+    cd = _CpuDecoder()
+    jobs, keep = [], []
+    for sp in specs[:400]:
+        src = pool.src[sp.kind]
+        segs, row = [], 0
+        for i in range(sp.src_first, sp.src_first + sp.n_src):
+            s_ = src.enc.segs[i]
+            segs.append(A.Segment(src.enc.blob.ctypes.data + int(s_["offset"]), int(s_["bytes_size"]), row,
+                                  int(s_["row_count"]), int(s_["max_string_length"])))
+            row += int(s_["row_count"])
+        if sp.tail >= 0:
+            tenc = pool.tails[sp.kind][sp.tail].enc
+            segs += cd.host_segs(tenc, row0=row)
+            row += tenc.n_rows
+        if src.is_string:
+            cum = src.cum
+            r0, r1 = int(cum[sp.src_first]), int(cum[sp.src_first + sp.n_src])
+            n_chars = int(src.ref_offs[r1] - src.ref_offs[r0])
+            if sp.tail >= 0:
+                n_chars += int(pool.tails[sp.kind][sp.tail].ref_offs[-1])
+            col, k, _ = cd.str_col(src.enc, segs, row, n_chars)
+            jobs.append((col, True, None))
+        else:
+            col, k, nb = cd.fixed_col(src.enc, segs, row)
+            jobs.append((col, False, nb))
+        keep.append(k)
+
+    def timed(threads):
+        nbytes, t0, i = 0, time.perf_counter(), 0
+        while time.perf_counter() - t0 < budget_s and i < len(jobs):
+            col, is_str, nb = jobs[i]
+            cd.decode(col, is_str, threads)
+            nbytes += (4 * (col.total_rows + 1) + col.chars_used) if is_str else nb
+            i += 1
+        return nbytes / (time.perf_counter() - t0) / 1e9, i
+
+    g_all, n_req = timed(cd.n_all)
+    g_one, n_req1 = timed(1)
+    log(f"[bench] cpu served stream: {g_all:.2f} GB/s on {cd.n_all} threads ({n_req} requests), "
+        f"{g_one:.2f} GB/s on 1 ({n_req1})")
+    per = {SERVED: {"gbs_all": round(g_all, 3), "gbs_1": round(g_one, 3), "requests": n_req, "requests_1": n_req1}}
+    return cd.summary(per, SERVED, f"served stream: its requests decoded in order for about {budget_s:.0f} s; "
+                      "duckdb not installed, so the repo's multithreaded C restatement of the decoders")
 
 
 if __name__ == "__main__":

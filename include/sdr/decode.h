@@ -253,7 +253,8 @@ int sdr_batch_destroy(sdr_batch* batch);
  *     outputs of admitted, unreleased requests fit the HBM budget (a request
  *     larger than the budget is admitted alone, when nothing else is held),
  *   - and, in SDR_LAUNCH_PERSISTENT mode, coalesces every admitted request
- *     into the next persistent graph launch while up to `max_inflight`
+ *     into the next persistent launch (the batch's group kernels, enqueued
+ *     directly on the slot's stream) while up to `max_inflight`
  *     launches run on their own streams: requests that arrive while others
  *     decode join the next launch as soon as a launch slot and budget are
  *     free (continuous batching).  SDR_LAUNCH_BATCHED and

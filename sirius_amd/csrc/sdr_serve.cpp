@@ -1,7 +1,6 @@
 // sdr_serve.cpp — the decode server (sdr_server_*): resident segments,
 // HBM-budget admission, pool-allocated outputs and continuous batching over
-// persistent graph launches (or one drop-in launch per request, the
-// baselines).  Runs on the calling thread; progress() does all device work.
+// persistent launches (or one drop-in launch per request, the baselines).  Runs on the calling thread; progress() does all device work.
 //
 // Device memory (resident segments and request outputs) comes from one
 // server-owned pool (DeviceArena): slabs reserved with hipMalloc when it
@@ -413,6 +412,11 @@ int sdr_server_create(const sdr_server_config* config, sdr_server** out)
     HIP_TRY(hipEventCreate(&sl.ev_end));
     if (sv->cfg.mode == SDR_LAUNCH_PERSISTENT) {
       int rc = sdr_batch_create(&sl.batch);
+      // a slot's launches enqueue the group kernels directly: consecutive
+      // launches differ in the groups they hold, and a graph would be
+      // destroyed and instantiated again right after the last launch of
+      // the previous one completed
+      if (!rc) rc = sdr_batch_set_option(sl.batch, SDR_BATCH_USE_GRAPH, 0);
       if (rc) return rc;
     }
   }

@@ -297,44 +297,59 @@ def _view(ptr: int, nbytes: int):
     return t
 
 
+def _differs(out, ref):
+    """0-dim bool device tensor: some element of `out` differs from `ref`
+    (equal shapes; no host synchronisation)."""
+    return (out != ref).any()
+
+
+def _word_view(t, ts: int):
+    """Byte tensor as ts-byte words where ts is 4 or 8 (fewer elements to
+    compare); other widths stay bytes."""
+    import torch
+
+    return t.view({4: torch.int32, 8: torch.int64}[ts]) if ts in (4, 8) else t
+
+
 def mismatches(pool: Pool, spec: Spec, comp: A.Completion):
-    """Device tensor counting the bytes of the request's output that differ
-    from the pool's reference decode (enqueued on torch's current stream,
-    no host synchronisation)."""
+    """0-dim bool device tensor, True when the request's output differs
+    from the pool's reference decode anywhere (enqueued on torch's current
+    stream, no host synchronisation).  The source rows and the tail rows
+    are compared as two slices of the output, in words of the value width."""
     import torch
 
     src = pool.src[spec.kind]
     cum = src.cum
     r0 = int(cum[spec.src_first])
     ns = int(cum[spec.src_first + spec.n_src]) - r0
+    tcol = pool.tails[spec.kind][spec.tail] if spec.tail >= 0 else None
     if src.is_string:
         ro_h = src.ref_offs
         ro, rc = src.ref
         c0, c1 = int(ro_h[r0]), int(ro_h[r0 + ns])
-        exp_o = [ro[r0:r0 + ns + 1] - c0]
-        exp_c = [rc[c0:c1]]
-        if spec.tail >= 0:
-            tcol = pool.tails[spec.kind][spec.tail]
-            to, tc = tcol.ref
-            exp_o.append(to[1:] + (c1 - c0))
-            exp_c.append(tc[: int(tcol.ref_offs[-1])])
-        eo = torch.cat(exp_o)
-        ec = torch.cat(exp_c)
+        n_tail_chars = int(tcol.ref_offs[-1]) if tcol is not None else 0
+        if comp.chars_used != (c1 - c0) + n_tail_chars:
+            return torch.ones((), dtype=torch.bool, device="cuda:0")
         offs = _view(comp.d_out, (spec.rows + 1) * 4).view(torch.int32)
-        bad = (offs != eo).sum()
-        if comp.chars_used != ec.numel():
-            return bad + 1
-        return bad + (_view(comp.d_chars, comp.chars_used) != ec).sum()
+        chars = _view(comp.d_chars, comp.chars_used)
+        d = _differs(offs[:ns + 1] + c0, ro[r0:r0 + ns + 1])
+        d = d | _differs(chars[:c1 - c0], rc[c0:c1])
+        if tcol is not None:
+            to, tc = tcol.ref
+            d = d | _differs(offs[ns:] - (c1 - c0), to)
+            d = d | _differs(chars[c1 - c0:], tc[:n_tail_chars])
+        return d
     ts = src.enc.type_size
-    parts = [src.ref[r0 * ts:(r0 + ns) * ts]]
-    if spec.tail >= 0:
-        parts.append(pool.tails[spec.kind][spec.tail].ref)
-    return (_view(comp.d_out, spec.rows * ts) != torch.cat(parts)).sum()
+    out = _view(comp.d_out, spec.rows * ts)
+    d = _differs(_word_view(out[:ns * ts], ts), _word_view(src.ref[r0 * ts:(r0 + ns) * ts], ts))
+    if tcol is not None:
+        d = d | _differs(_word_view(out[ns * ts:], ts), _word_view(tcol.ref, ts))
+    return d
 
 
 def verify(pool: Pool, spec: Spec, comp: A.Completion, stream=None) -> bool:
     """True when the request's output equals the pool's reference decode."""
-    return int(mismatches(pool, spec, comp).item()) == 0
+    return not bool(mismatches(pool, spec, comp).item())
 
 
 def run(lib, pool: Pool, specs: list, mode: int, max_inflight: int = 2, budget: int = 0,
@@ -344,9 +359,12 @@ def run(lib, pool: Pool, specs: list, mode: int, max_inflight: int = 2, budget: 
 
     Request descriptors are built before the clock starts; each request is
     submitted when its arrival time has passed (all at once when every
-    arrival is 0, which measures capacity).  Sampled outputs are compared
-    with the reference on torch's stream without host synchronisation and
-    released after the comparison in stream order."""
+    arrival is 0, which measures capacity).  The outputs of every
+    `verify_every`-th request are held until the last request has
+    completed, then compared with the reference on torch's stream (so the
+    comparison does not share the GPU with the timed stream) and released
+    after the comparison in stream order; `verified_ok` counts the sampled
+    requests whose output equals the reference."""
     import torch
 
     own = server is None
@@ -366,8 +384,8 @@ def run(lib, pool: Pool, specs: list, mode: int, max_inflight: int = 2, budget: 
     lat, done_ms, statuses = [], [], []
     bytes_out = bytes_in = 0
     n_verified = 0
-    bad = torch.zeros((), dtype=torch.int64, device="cuda:0")
-    checks = []  # (tag, mismatch count on the device) per verified request
+    checks = []  # (tag, 0-dim bool device tensor: output differs) per verified request
+    sampled = []  # (spec, completion) of the requests to verify, outputs held until then
     n_launches = 0
     i = 0
     st0, st1 = A.ServerStats(), A.ServerStats()
@@ -397,18 +415,22 @@ def run(lib, pool: Pool, specs: list, mode: int, max_inflight: int = 2, budget: 
                 bytes_out += (s.rows + 1) * 4 + c.chars_used if is_str[s.kind] else c.out_bytes
                 bytes_in += s.comp_bytes
                 if verify_every and s.tag % verify_every == 0 and c.status == 0:
-                    tv = time.perf_counter()
-                    n_verified += 1
-                    mm = mismatches(pool, s, c)
-                    bad += mm
-                    checks.append((s.tag, mm))
-                    t_verify += time.perf_counter() - tv
+                    kept = A.Completion()
+                    C.pointer(kept)[0] = c
+                    sampled.append((s, kept))  # checked and released once the stream is served
+                    continue
                 L.check(lib.sdr_server_release(server, c.tag, sh))
             if n_out.value == 0 and launched.value == 0:
                 nxt = order[i].arrival_ms if i < len(order) else None
                 if nxt is not None and nxt - now > 0.2:
                     time.sleep(min(0.001, (nxt - now) / 2000.0))
         L.check(lib.sdr_server_get_stats(server, C.byref(st1)))
+        tv = time.perf_counter()
+        for s, c in sampled:
+            checks.append((s.tag, mismatches(pool, s, c)))
+            L.check(lib.sdr_server_release(server, c.tag, sh))
+        n_verified = len(sampled)
+        t_verify = time.perf_counter() - tv
     finally:
         torch.cuda.synchronize()
         if own:
@@ -421,19 +443,38 @@ def run(lib, pool: Pool, specs: list, mode: int, max_inflight: int = 2, budget: 
     first = max(min(s.arrival_ms for s in specs), 0.0)
     span_ms = max(done_ms) - first
     lat = np.asarray(lat)
-    n_bad = int(bad.item())
+    flags = torch.stack([m for _, m in checks]).cpu().numpy() if checks else np.zeros(0, bool)
+    n_bad = int(flags.sum())
     failed = []
-    if n_bad:
-        counts = torch.stack([m for _, m in checks]).cpu().numpy()
-        for (tag, _), cnt in zip(checks, counts):
-            if cnt:
-                sp = by_tag[tag]
-                failed.append({"tag": tag, "kind": sp.kind, "rows": sp.rows, "src_first": sp.src_first,
-                               "n_src": sp.n_src, "tail": sp.tail, "bad_bytes": int(cnt)})
+    for (tag, _), f in zip(checks, flags):
+        if f:
+            sp = by_tag[tag]
+            failed.append({"tag": tag, "kind": sp.kind, "rows": sp.rows, "src_first": sp.src_first,
+                           "n_src": sp.n_src, "tail": sp.tail})
     return {"failed": failed[:20], "requests": len(specs), "launches": n_launches, "decoded_bytes": int(bytes_out),
             "compressed_bytes": int(bytes_in), "span_ms": span_ms, "decoded_gbs": bytes_out / (span_ms * 1e-3) / 1e9,
             "achieved_gbs": (bytes_out + bytes_in) / (span_ms * 1e-3) / 1e9,
             "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
             "mean_ms": float(lat.mean()), "errors": int(sum(1 for x in statuses if x != 0)),
-            "verified": n_verified, "verified_ok": n_verified if n_bad == 0 else -1, "mismatched_bytes": n_bad,
-            "server_stats": server_stats}
+            "verified": n_verified, "verified_ok": n_verified - n_bad, "mismatched_requests": n_bad,
+            "server_stats": server_stats, "latencies_ms": lat}
+
+
+def merge_runs(runs: list) -> dict:
+    """One result over several passes of run(): bytes, spans and server
+    phase times add up, percentiles are over every request of every pass."""
+    lat = np.concatenate([r["latencies_ms"] for r in runs])
+    span = sum(r["span_ms"] for r in runs)
+    dec = sum(r["decoded_bytes"] for r in runs)
+    comp = sum(r["compressed_bytes"] for r in runs)
+    stats = {k: sum(r["server_stats"][k] for r in runs) for k in runs[0]["server_stats"]}
+    verified = sum(r["verified"] for r in runs)
+    return {"failed": [f for r in runs for f in r["failed"]][:20], "requests": sum(r["requests"] for r in runs),
+            "launches": sum(r["launches"] for r in runs), "decoded_bytes": dec, "compressed_bytes": comp,
+            "span_ms": span, "decoded_gbs": dec / (span * 1e-3) / 1e9,
+            "achieved_gbs": (dec + comp) / (span * 1e-3) / 1e9,
+            "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+            "mean_ms": float(lat.mean()), "errors": sum(r["errors"] for r in runs), "verified": verified,
+            "verified_ok": sum(r["verified_ok"] for r in runs),
+            "mismatched_requests": sum(r["mismatched_requests"] for r in runs),
+            "server_stats": stats}

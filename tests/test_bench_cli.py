@@ -34,3 +34,37 @@ def test_string_workload_decodes_to_its_values():
     got_offs, got_chars = O.decode_strings(enc, threads=4)
     np.testing.assert_array_equal(got_offs.astype(np.int64), np.asarray(offs, dtype=np.int64))
     np.testing.assert_array_equal(got_chars[:int(offs[-1])], chars[:int(offs[-1])])
+
+
+def _served_leg():
+    main = {"decoded_gbs": 600.0, "achieved_gbs": 800.0, "span_ms": 300.0, "requests": 20000,
+            "decoded_bytes": 180 * 10 ** 9, "compressed_bytes": 60 * 10 ** 9, "p50_ms": 10.0, "p99_ms": 30.0,
+            "verified": 1250, "verified_ok": 1250, "mismatched_requests": 0,
+            "server_stats": {"gpu_union_ms": 250.0, "launches": 100}}
+    sr = {"persistent": main, "passes": 2, "rate_req_per_s": 30000.0, "resident_gb": 32.0, "frac_hbm": 0.1,
+          "baselines": {"one_launch_per_request": {"p50_ms": 200.0, "p99_ms": 400.0}}}
+    cpu = {"value": 5.0, "unit": "GB/s", "cores": 16, "kind": "port", "sample": "served stream",
+           "per_config": {"served_mixed_10k": {"gbs_all": 5.0, "gbs_1": 0.5}}}
+    return {"served": sr, "cpu": cpu, "tuning": {}}
+
+
+def test_headline_line_takes_the_served_leg_as_value_and_merges_the_other_leg():
+    args = B.argparse.Namespace(warmup=5)
+    x = {"workloads": {"int_date_mix_64M": {"kernels": {"RLE": {"us": 90.0, "alg_bytes": 10, "gbs": 5000.0}}}},
+         "staged_from_file": {"ms_per_step": 4.0},
+         "cpu": {"per_config": {"bp_int32_8M": {"gbs_all": 9.0, "gbs_1": 1.0}}}}
+    line = B.headline_line(args, _served_leg(), x)
+    assert line["value"] == 600.0 and line["unit"] == "GB/s" and line["n_gpus"] == 1
+    assert line["config"]["workload"] == "served_mixed_10k" and line["steps"] == 2
+    assert line["roofline"]["achieved"] == round(240e9 / 0.25 / 1e9, 1)
+    assert line["roofline"]["frac"] == round(960.0 / 8000.0, 4)
+    assert line["roofline_int_date_mix_64M"]["kernel"] == "RLE"
+    assert set(line["cpu_baseline"]["per_config"]) == {"served_mixed_10k", "bp_int32_8M"}
+    assert line["cpu_baseline"]["value"] == 5.0 and line["legs"]["extra"] == "ok"
+    B.json.dumps(line)
+
+
+def test_headline_line_reports_a_failed_other_leg():
+    line = B.headline_line(B.argparse.Namespace(warmup=5), _served_leg(), {"rc": -11})
+    assert line["value"] == 600.0 and line["workloads"] is None
+    assert line["legs"]["extra"] == "exit status -11"

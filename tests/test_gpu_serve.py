@@ -181,3 +181,49 @@ def test_zero_row_string_request(setup, mode):
     finally:
         torch.cuda.synchronize()
         lib.sdr_server_destroy(sv)
+
+
+def test_verify_detects_a_changed_byte(setup):
+    """The served-output check itself: after one byte of a completed
+    request's output changes, in its source rows or in its tail rows
+    (values, or chars and offsets of a VARCHAR request), verify() fails;
+    restored, it passes again."""
+    lib, _, pool = setup
+    pick = {}
+    for s in S.make_specs(pool, 400, seed=41, max_rows=1 << 21):
+        if s.tail >= 0 and s.n_src > 0:
+            pick.setdefault(pool.src[s.kind].is_string, s)
+    assert set(pick) == {False, True}
+    cfg = A.ServerConfig(0, 1, A.LAUNCH_PERSISTENT, 0, 0)
+    sv = C.c_void_p()
+    L.check(lib.sdr_server_create(C.byref(cfg), C.byref(sv)))
+    stream = torch.cuda.current_stream()
+    try:
+        reqs = {s.tag: s.request(pool) for s in pick.values()}
+        for r in reqs.values():
+            L.check(lib.sdr_server_submit(sv, C.byref(r)))
+        got = _drain(lib, sv, len(reqs), stream)
+        for is_str, s in pick.items():
+            c = got[s.tag]
+            assert c.status == 0 and S.verify(pool, s, c)
+            src = pool.src[s.kind]
+            ns = int(src.cum[s.src_first + s.n_src] - src.cum[s.src_first])
+            if is_str:
+                c_src = int(src.ref_offs[int(src.cum[s.src_first]) + ns] - src.ref_offs[int(src.cum[s.src_first])])
+                chars = S._view(c.d_chars, c.chars_used)
+                offs = S._view(c.d_out, (s.rows + 1) * 4)
+                targets = [(chars, 0), (chars, c_src), (offs, 4 * (ns + 1))]
+            else:
+                ts = src.enc.type_size
+                out = S._view(c.d_out, s.rows * ts)
+                targets = [(out, 0), (out, ns * ts + ts - 1)]
+            assert S._ZERO_COPY
+            for t, i in targets:
+                t[i] ^= 1
+                assert not S.verify(pool, s, c), (s.kind, i)
+                t[i] ^= 1
+                assert S.verify(pool, s, c), (s.kind, i)
+            L.check(lib.sdr_server_release(sv, s.tag, stream.cuda_stream))
+    finally:
+        torch.cuda.synchronize()
+        lib.sdr_server_destroy(sv)
