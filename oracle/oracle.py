@@ -31,6 +31,7 @@ def load():
         lib.oracle_decode_strings_column_mt.argtypes = [C.POINTER(A.StringColumn), C.c_int]
         lib.oracle_fsst_lengths.argtypes = [vp, u32, u32, vp]
         lib.oracle_fsst_decode.argtypes = [vp, u32, u32, vp, vp, C.c_uint64]
+        lib.oracle_dict_fsst_decode.argtypes = [vp, u32, u32, vp, vp, C.c_uint64, vp]
         _lib = lib
     return _lib
 
@@ -84,3 +85,42 @@ def decode_strings(enc, threads: int = 1):
     if rc != 0:
         raise RuntimeError(f"oracle rc {A.STATUS_NAMES.get(rc, rc)}")
     return offs, chars[: col.chars_used]
+
+
+def decode_strings_masked(enc, threads: int = 1):
+    """String Encoded column, decoded with a null mask (no validity runs, so
+    only DICT_FSST inline NULLs clear bits) -> (offsets int32[n+1], chars
+    uint8, valid bool[n], null_count)."""
+    n = enc.n_rows
+    segs = enc.segments(enc.blob.ctypes.data)
+    runs, keep = A.make_runs([(enc.codec, segs)])
+    offs = np.zeros(n + 1, np.int32)
+    cap = int(sum(int(s["row_count"]) * int(s["max_string_length"]) for s in enc.segs)) or 1
+    chars = np.zeros(cap, np.uint8)
+    mask = np.full((n + 7) // 8, 0xFF, np.uint8)  # left all valid when no NULL is found
+    col = A.StringColumn()
+    col.total_rows = n
+    col.data = runs
+    col.n_data = 1
+    col.d_offsets = offs.ctypes.data
+    col.d_chars = chars.ctypes.data
+    col.chars_capacity = cap
+    col.d_null_mask = mask.ctypes.data
+    rc = load().oracle_decode_strings_column_mt(C.byref(col), threads)
+    if rc != 0:
+        raise RuntimeError(f"oracle rc {A.STATUS_NAMES.get(rc, rc)}")
+    valid = np.unpackbits(mask, bitorder="little")[:n].astype(bool)
+    return offs, chars[: col.chars_used], valid, int(col.null_count)
+
+
+def dict_fsst_segment(seg: bytes | np.ndarray, rows: int, cap: int = 1 << 20):
+    """One DICT_FSST segment -> (lengths uint32[rows], chars uint8, inline_null)."""
+    b = np.frombuffer(bytes(seg), np.uint8) if not isinstance(seg, np.ndarray) else np.ascontiguousarray(seg)
+    lengths = np.zeros(max(1, rows), np.uint32)
+    chars = np.zeros(cap, np.uint8)
+    flag = np.zeros(1, np.uint8)
+    rc = load().oracle_dict_fsst_decode(b.ctypes.data, b.size, rows, lengths.ctypes.data, chars.ctypes.data, cap,
+                                        flag.ctypes.data)
+    if rc != 0:
+        raise RuntimeError(f"oracle rc {A.STATUS_NAMES.get(rc, rc)}")
+    return lengths[:rows], chars[: int(lengths[:rows].sum())], bool(flag[0])

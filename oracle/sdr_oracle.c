@@ -699,6 +699,204 @@ static void plain_decode(uint32_t codec, const uint8_t* seg, uint32_t size, uint
 }
 
 /* ------------------------------------------------------------------------ */
+/* DICT_FSST strings                                                         */
+/* ------------------------------------------------------------------------ */
+
+/* DICT_FSST follows src/cuda/scan/strings/dict_fsst.cu:17-38 (layout and
+ * modes), :64-72 (16-byte header: dict_size, dict_count, mode,
+ * string_lengths_width, dictionary_indices_width, pad, symbol_table_size),
+ * :516-525 (region offsets, all 8-byte aligned: dict at 16, symbol table at
+ * align8(16 + dict_size), string lengths at 16 + align8(dict_size) in mode 0
+ * or align8(symtab + symtab_size), indices at align8(lengths + their bytes);
+ * a segment is valid if it has rows, 16 header bytes, mode <= 2 and, unless
+ * mode 2, its index region starts inside it), :160-185 (entry k of the
+ * dictionary is [byte_off[k], byte_off[k+1]) of the dict region, byte_off
+ * the scanned bit-packed string lengths), :190-235 (decoded offsets: mode 0,
+ * or a dictionary of at most one entry, copies byte_off; the FSST modes give
+ * entry 0 (NULL) length 0 and every other entry the length its compressed
+ * bytes walk to, an escape counting one byte and skipping the next even when
+ * it is the entry's last byte), :238-243 (a segment has inline NULLs when it
+ * is not mode 2, has more than one entry and entry 0 is empty), :246-265 (row
+ * lengths: index = the bit-packed dictionary index, or row + 1 in mode 2;
+ * length dec_off[idx+1] - dec_off[idx] when 0 < idx < dict_count, else 0),
+ * :268-320 (mode 1 predecodes each entry once; a trailing escape writes no
+ * byte), :324-400 (gather: mode 0 copies raw entry bytes, modes 1 and 2 the
+ * decoded entry) and :426-446 (with inline NULLs anywhere in the column,
+ * every row of a segment that is not mode 2, invalid segments included, whose
+ * index is 0 is NULL).  Arithmetic on the header is 32-bit as in the
+ * reference.
+ *
+ * Where this build differs, malformed input only:
+ *  - a lengths or indices width above 32 makes the segment invalid (the
+ *    reference reads a 64-bit window there);
+ *  - the byte a trailing escape counts is written as 0 (the reference leaves
+ *    it unwritten); parity unpinned: DuckDB's encoder never ends an entry
+ *    with an escape;
+ *  - a row whose index is >= dict_count decodes empty in both passes (the
+ *    reference's gather copies the bytes of whatever offsets follow), and an
+ *    entry whose end is below its start is empty;
+ *  - bytes past the segment read as zero. */
+typedef struct dfsst_seg {
+  int ok, has_fsst, inline_null;
+  uint32_t mode, dict_count, diw, off_dict, off_didx;
+  uint32_t* E;         /* dict_count + 1 entry offsets into src */
+  const uint8_t* src;  /* dict region (mode 0 / no table) or decoded entries */
+  uint64_t src_size;
+  uint8_t* pre;        /* decoded entries, owned */
+} dfsst_seg;
+
+static uint32_t al8(uint32_t v) { return (v + 7u) & ~7u; }
+
+static void dfsst_free(dfsst_seg* d)
+{
+  free(d->E);
+  free(d->pre);
+  memset(d, 0, sizeof(*d));
+}
+
+static int dfsst_prepare(const uint8_t* seg, uint32_t size, uint32_t rows, dfsst_seg* d)
+{
+  memset(d, 0, sizeof(*d));
+  if (rows == 0 || size < 16u) return SDR_OK;
+  uint32_t dict_size = ld32(seg), dict_count = ld32(seg + 4), symtab_size = ld32(seg + 12);
+  uint32_t mode = seg[8], slw = seg[9], diw = seg[10];
+  if (mode > 2u || slw > 32u || diw > 32u) return SDR_OK;
+  uint32_t off_dict   = 16u;
+  uint32_t off_symtab = al8(off_dict + dict_size);
+  uint32_t off_slens  = mode == 0u ? off_dict + al8(dict_size) : al8(off_symtab + symtab_size);
+  uint32_t slens_bits = dict_count * slw;
+  uint32_t off_didx   = al8(off_slens + (slens_bits + 7u) / 8u);
+  if (off_didx > size && mode != 2u) return SDR_OK;
+  d->E = (uint32_t*)calloc((size_t)dict_count + 1u, sizeof(uint32_t));
+  uint32_t* bo = (uint32_t*)calloc((size_t)dict_count + 1u, sizeof(uint32_t));
+  if (!d->E || !bo) { free(bo); dfsst_free(d); return SDR_E_CAPACITY; }
+  d->ok = 1;
+  d->mode = mode;
+  d->dict_count = dict_count;
+  d->diw = diw;
+  d->off_dict = off_dict;
+  d->off_didx = off_didx;
+  uint64_t sl_avail = off_slens < size ? size - off_slens : 0u;
+  for (uint32_t k = 0; k < dict_count; ++k)
+    bo[k + 1] = bo[k] + (uint32_t)read_bits(seg + (sl_avail ? off_slens : 0u), sl_avail, (uint64_t)k * slw, slw);
+  d->has_fsst = mode != 0u && dict_count > 1u;
+  d->inline_null = mode != 2u && dict_count > 1u && bo[1] == 0u;
+  uint64_t dict_avail = off_dict < size ? size - off_dict : 0u;
+  const uint8_t* dict = seg + off_dict;
+#define DICT_AT(i) ((uint64_t)(i) < dict_avail ? dict[(i)] : 0u)
+  if (!d->has_fsst) {
+    memcpy(d->E, bo, sizeof(uint32_t) * ((size_t)dict_count + 1u));
+    d->src = dict;
+    d->src_size = dict_avail;
+    free(bo);
+    return SDR_OK;
+  }
+  fsst_table* t = (fsst_table*)malloc(sizeof(fsst_table));
+  if (!t) { free(bo); dfsst_free(d); return SDR_E_CAPACITY; }
+  if (off_symtab < size) fsst_import(seg, size, off_symtab, t);
+  else memset(t, 0, sizeof(*t)); /* no table bytes: every code decodes to nothing */
+  d->E[0] = 0;
+  d->E[1] = 0; /* entry 0 is NULL */
+  for (uint32_t k = 1; k < dict_count; ++k) {
+    uint32_t c0 = bo[k], n = bo[k + 1] >= bo[k] ? bo[k + 1] - bo[k] : 0u;
+    uint32_t pos = 0, dl = 0;
+    while (pos < n) {
+      uint32_t code = DICT_AT((uint64_t)c0 + pos);
+      ++pos;
+      if (code < FSST_ESC) dl += t->len[code];
+      else { ++pos; ++dl; }
+    }
+    d->E[k + 1] = d->E[k] + dl;
+  }
+  uint64_t total = d->E[dict_count];
+  d->pre = (uint8_t*)calloc(total ? total : 1u, 1);
+  if (!d->pre) { free(t); free(bo); dfsst_free(d); return SDR_E_CAPACITY; }
+  for (uint32_t k = 1; k < dict_count; ++k) {
+    uint32_t c0 = bo[k], n = bo[k + 1] >= bo[k] ? bo[k + 1] - bo[k] : 0u;
+    uint32_t pos = 0;
+    uint64_t o = d->E[k], lim = d->E[k + 1] >= d->E[k] ? d->E[k + 1] : d->E[k];
+    if (lim > total) lim = total; /* offsets that wrapped (malformed lengths) */
+    while (pos < n) {
+      uint32_t code = DICT_AT((uint64_t)c0 + pos);
+      ++pos;
+      if (code < FSST_ESC) {
+        for (uint32_t j = 0; j < t->len[code] && o < lim; ++j) d->pre[o++] = (uint8_t)(t->sym[code] >> (8u * j));
+      } else if (pos < n) {
+        if (o < lim) d->pre[o++] = (uint8_t)DICT_AT((uint64_t)c0 + pos);
+        ++pos;
+      }
+    }
+  }
+#undef DICT_AT
+  d->src = d->pre;
+  d->src_size = total;
+  free(t);
+  free(bo);
+  return SDR_OK;
+}
+
+/* Dictionary index of segment row i (0, which is NULL, for an invalid segment). */
+static uint32_t dfsst_index(const dfsst_seg* d, const uint8_t* seg, uint32_t size, uint32_t i)
+{
+  if (!d->ok) return 0;
+  if (d->mode == 2u) return i + 1u;
+  uint64_t avail = d->off_didx < size ? size - d->off_didx : 0u;
+  return (uint32_t)read_bits(seg + (avail ? d->off_didx : 0u), avail, (uint64_t)i * d->diw, d->diw);
+}
+
+static uint32_t dfsst_row(const dfsst_seg* d, const uint8_t* seg, uint32_t size, uint32_t i, uint32_t* start)
+{
+  uint32_t idx = dfsst_index(d, seg, size, i);
+  *start = 0;
+  if (!d->ok || idx == 0u || idx >= d->dict_count || d->E[idx + 1] < d->E[idx]) return 0;
+  *start = d->E[idx];
+  return d->E[idx + 1] - d->E[idx];
+}
+
+int oracle_dict_fsst_decode(const uint8_t* seg, uint32_t bytes_size, uint32_t rows, uint32_t* lengths,
+                            uint8_t* chars, uint64_t chars_cap, uint8_t* inline_null)
+{
+  dfsst_seg d;
+  int rc = dfsst_prepare(seg, bytes_size, rows, &d);
+  if (rc) return rc;
+  uint64_t o = 0;
+  for (uint32_t i = 0; i < rows; ++i) {
+    uint32_t st, l = dfsst_row(&d, seg, bytes_size, i, &st);
+    lengths[i] = l;
+    if (chars)
+      for (uint32_t k = 0; k < l && o < chars_cap; ++k, ++o)
+        chars[o] = (uint64_t)st + k < d.src_size ? d.src[st + k] : 0u;
+  }
+  if (inline_null) *inline_null = (uint8_t)d.inline_null;
+  dfsst_free(&d);
+  return SDR_OK;
+}
+
+/* Rows of a DICT_FSST segment that are NULL when the column has inline
+ * NULLs: index 0 outside mode 2 (every row of an invalid segment). */
+static void dfsst_mark_nulls(const uint8_t* seg, uint32_t size, uint32_t rows, uint32_t row_offset, uint8_t* mask)
+{
+  dfsst_seg d;
+  if (dfsst_prepare(seg, size, rows, &d)) return;
+  if (!d.ok || d.mode != 2u)
+    for (uint32_t i = 0; i < rows; ++i)
+      if (dfsst_index(&d, seg, size, i) == 0u) {
+        uint32_t r = row_offset + i;
+        mask[r >> 3] &= (uint8_t)~(1u << (r & 7u));
+      }
+  dfsst_free(&d);
+}
+
+static int dfsst_has_inline_nulls(const uint8_t* seg, uint32_t size, uint32_t rows)
+{
+  dfsst_seg d;
+  if (dfsst_prepare(seg, size, rows, &d)) return 0;
+  int r = d.ok && d.inline_null;
+  dfsst_free(&d);
+  return r;
+}
+
+/* ------------------------------------------------------------------------ */
 /* UNCOMPRESSED / CONSTANT / validity                                        */
 /* follows src/cuda/scan/gpu_native_decode.cu:154-170 (CONSTANT), :221-286   */
 /* (UNCOMPRESSED), :396-476 (validity overlay), :519-538 (all-valid mask)    */
@@ -930,7 +1128,19 @@ static void* str_worker(void* arg)
     if (i >= p->n) break;
     str_job* j = &p->jobs[i];
     const sdr_segment* g = j->g;
-    if (j->codec != SDR_CODEC_FSST) {
+    if (j->codec == SDR_CODEC_DICT_FSST) {
+      if (p->phase == 0) {
+        oracle_dict_fsst_decode(g->d_bytes, g->bytes_size, g->row_count, j->lengths + g->row_offset, NULL, 0, NULL);
+      } else {
+        uint64_t base = (uint32_t)j->offsets[g->row_offset];
+        uint64_t end = (uint32_t)j->offsets[g->row_offset + g->row_count];
+        uint32_t* tmp = (uint32_t*)malloc(sizeof(uint32_t) * (g->row_count ? g->row_count : 1));
+        if (tmp) {
+          oracle_dict_fsst_decode(g->d_bytes, g->bytes_size, g->row_count, tmp, j->chars + base, end - base, NULL);
+          free(tmp);
+        }
+      }
+    } else if (j->codec != SDR_CODEC_FSST) {
       if (p->phase == 0) {
         plain_lengths(j->codec, g->d_bytes, g->bytes_size, g->row_count, j->lengths + g->row_offset);
       } else {
@@ -979,7 +1189,7 @@ int oracle_decode_strings_column_mt(sdr_string_column* col, int n_threads)
   if (n > 0x7FFFFFFFu) return SDR_E_BOUNDS;
   for (uint32_t r = 0; r < col->n_data; ++r)
     if (col->data[r].codec != SDR_CODEC_FSST && col->data[r].codec != SDR_CODEC_DICTIONARY &&
-        col->data[r].codec != SDR_CODEC_UNCOMPRESSED)
+        col->data[r].codec != SDR_CODEC_UNCOMPRESSED && col->data[r].codec != SDR_CODEC_DICT_FSST)
       return SDR_E_UNSUPPORTED_CODEC;
   int rc = check_runs_bounds(col->data, col->n_data, n);
   if (rc) return rc;
@@ -991,6 +1201,15 @@ int oracle_decode_strings_column_mt(sdr_string_column* col, int n_threads)
   }
   if (n == 0) return SDR_OK;
   if (!col->d_offsets) return SDR_E_INVALID_ARG;
+  /* DICT_FSST inline NULLs need a mask to land in (checked before any output) */
+  int any_inline = 0;
+  for (uint32_t r = 0; r < col->n_data; ++r)
+    if (col->data[r].codec == SDR_CODEC_DICT_FSST)
+      for (uint32_t s = 0; s < col->data[r].n_segments; ++s) {
+        const sdr_segment* g = &col->data[r].segments[s];
+        if (g->row_count && dfsst_has_inline_nulls(g->d_bytes, g->bytes_size, g->row_count)) any_inline = 1;
+      }
+  if (any_inline && !col->d_null_mask) return SDR_E_INVALID_ARG;
   uint32_t* lengths = (uint32_t*)calloc((size_t)n + 1u, sizeof(uint32_t));
   size_t nj = 0;
   for (uint32_t r = 0; r < col->n_data; ++r) nj += col->data[r].n_segments;
@@ -1022,8 +1241,19 @@ int oracle_decode_strings_column_mt(sdr_string_column* col, int n_threads)
   run_str_pool(&p, 1, n_threads);
   free(lengths);
   free(p.jobs);
-  if (col->has_nulls && col->d_null_mask) {
-    col->null_count = build_mask(col->validity, col->n_validity, n, col->d_null_mask);
+  if ((col->has_nulls && col->d_null_mask) || any_inline) {
+    col->null_count = build_mask(col->validity, col->has_nulls ? col->n_validity : 0u, n, col->d_null_mask);
+    if (any_inline) {
+      for (uint32_t r = 0; r < col->n_data; ++r)
+        if (col->data[r].codec == SDR_CODEC_DICT_FSST)
+          for (uint32_t s = 0; s < col->data[r].n_segments; ++s) {
+            const sdr_segment* g = &col->data[r].segments[s];
+            if (g->row_count) dfsst_mark_nulls(g->d_bytes, g->bytes_size, g->row_count, g->row_offset, col->d_null_mask);
+          }
+      int64_t nulls = 0;
+      for (uint32_t i = 0; i < n; ++i) nulls += !((col->d_null_mask[i >> 3] >> (i & 7u)) & 1u);
+      col->null_count = nulls;
+    }
   }
   return SDR_OK;
 }

@@ -35,6 +35,11 @@ int oracle_fsst_lengths(const uint8_t* seg, uint32_t bytes_size, uint32_t row_co
 int oracle_fsst_decode(const uint8_t* seg, uint32_t bytes_size, uint32_t row_count,
                        uint32_t* lengths, uint8_t* chars, uint64_t chars_cap);
 
+/* DICT_FSST: decode one segment (lengths[row_count]; `chars` may be NULL
+ * for lengths only); *inline_null receives the segment's inline-NULL flag. */
+int oracle_dict_fsst_decode(const uint8_t* seg, uint32_t bytes_size, uint32_t row_count,
+                            uint32_t* lengths, uint8_t* chars, uint64_t chars_cap, uint8_t* inline_null);
+
 /* Multithreaded column decode used as the timed CPU baseline: segments of
  * every run are spread over `n_threads` threads (1 = scalar). */
 int oracle_decode_table_mt(sdr_column* cols, uint32_t n_cols, int n_threads);

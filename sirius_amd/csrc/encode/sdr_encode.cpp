@@ -872,12 +872,149 @@ void fsst_encode(sdr_encoded* e, const uint8_t* chars, const uint64_t* offs, uin
   }
 }
 
+// ---------------------------------------------------------------------------
+// DICT_FSST — DuckDB's dictionary codec over FSST-compressed entries
+// (layout of src/cuda/scan/strings/dict_fsst.cu:17-38 and the reference's
+// test synthesizer, test/cpp/scan/strings_synth.hpp:437-549): a 16-byte
+// header {dict_size, dict_count, mode, string_lengths_width,
+// dictionary_indices_width, pad, symbol_table_size}, the dictionary entries
+// front to back from byte 16 (entry 0 is the empty NULL slot), the FSST
+// symbol table (modes 1 and 2), the bit-packed entry lengths and, except in
+// mode 2, the bit-packed row indices, every region 8-byte aligned.
+// Mode 0 keeps entries raw, mode 1 FSST-compresses them, mode 2 has one
+// entry per row (row i is entry i + 1) and no index region.
+// ---------------------------------------------------------------------------
+
+// Values of `w` bits LSB-first into exactly ceil(n * w / 8) bytes at `dst`.
+void pack_tight(uint8_t* dst, const std::vector<uint32_t>& v, uint32_t w)
+{
+  if (w == 0) return;
+  uint64_t bit = 0;
+  for (uint32_t x : v) {
+    for (uint32_t b = 0; b < w; ++b, ++bit)
+      if ((x >> b) & 1u) dst[bit >> 3] |= static_cast<uint8_t>(1u << (bit & 7u));
+  }
+}
+
+inline uint32_t al8(uint64_t v) { return static_cast<uint32_t>((v + 7u) & ~uint64_t{7}); }
+
+// One segment over rows [r0, r1); NULL rows (valid[r] == 0) select entry 0.
+std::vector<uint8_t> dict_fsst_segment(const uint8_t* chars, const uint64_t* offs, const uint8_t* valid,
+                                       uint64_t r0, uint64_t r1, uint32_t mode, uint64_t seed, uint32_t* max_len)
+{
+  std::vector<std::string> dict(1);  // entry 0: the NULL slot
+  std::vector<uint32_t> sel;
+  std::unordered_map<std::string, uint32_t> at;
+  *max_len = 0;
+  for (uint64_t r = r0; r < r1; ++r) {
+    std::string v(reinterpret_cast<const char*>(chars + offs[r]), offs[r + 1] - offs[r]);
+    *max_len = std::max<uint32_t>(*max_len, static_cast<uint32_t>(v.size()));
+    if (mode == 2) {
+      dict.push_back(std::move(v));
+      continue;
+    }
+    if (valid && !valid[r]) {
+      sel.push_back(0);
+      continue;
+    }
+    auto it = at.find(v);
+    if (it == at.end()) {
+      at.emplace(v, static_cast<uint32_t>(dict.size()));
+      sel.push_back(static_cast<uint32_t>(dict.size()));
+      dict.push_back(std::move(v));
+    } else {
+      sel.push_back(it->second);
+    }
+  }
+  const uint32_t dict_count = static_cast<uint32_t>(dict.size());
+  std::vector<std::vector<uint8_t>> entry(dict_count);
+  std::vector<uint8_t> symtab;
+  if (mode != 0 && dict_count > 1) {
+    Rng rng(seed);
+    std::vector<const uint8_t*> sp;
+    std::vector<uint32_t> sl;
+    uint64_t budget = 16384;
+    for (uint32_t t = 0; t < 4096 && budget > 0; ++t) {
+      const std::string& e = dict[1 + rng.next() % (dict_count - 1)];
+      sp.push_back(reinterpret_cast<const uint8_t*>(e.data()));
+      sl.push_back(static_cast<uint32_t>(e.size()));
+      budget = budget > e.size() ? budget - e.size() : 0;
+    }
+    SymTable st = fsst_build(sp, sl);
+    symtab      = fsst_export(st);
+    for (uint32_t k = 1; k < dict_count; ++k)
+      fsst_compress_row(st, reinterpret_cast<const uint8_t*>(dict[k].data()), dict[k].size(), entry[k]);
+  } else {
+    for (uint32_t k = 0; k < dict_count; ++k) entry[k].assign(dict[k].begin(), dict[k].end());
+  }
+  uint32_t max_entry = 0;
+  uint64_t dict_size = 0;
+  std::vector<uint32_t> lens(dict_count);
+  for (uint32_t k = 0; k < dict_count; ++k) {
+    lens[k] = static_cast<uint32_t>(entry[k].size());
+    max_entry = std::max(max_entry, lens[k]);
+    dict_size += lens[k];
+  }
+  const uint32_t rows = static_cast<uint32_t>(r1 - r0);
+  const uint32_t slw  = std::max(1u, bit_width_u64(max_entry));
+  const uint32_t diw  = std::max(1u, bit_width_u64(dict_count - 1u));
+  const uint32_t off_dict   = 16;
+  const uint32_t off_symtab = mode == 0 ? 0u : al8(off_dict + dict_size);
+  const uint32_t off_slens  = mode == 0 ? off_dict + al8(dict_size) : al8(uint64_t{off_symtab} + symtab.size());
+  const uint32_t off_didx   = al8(off_slens + (uint64_t{dict_count} * slw + 7u) / 8u);
+  const uint32_t total      = mode == 2 ? off_didx : al8(off_didx + (uint64_t{rows} * diw + 7u) / 8u);
+  std::vector<uint8_t> seg(total, 0);
+  put<uint32_t>(seg, 0, static_cast<uint32_t>(dict_size));
+  put<uint32_t>(seg, 4, dict_count);
+  seg[8]  = static_cast<uint8_t>(mode);
+  seg[9]  = static_cast<uint8_t>(slw);
+  seg[10] = static_cast<uint8_t>(diw);
+  put<uint32_t>(seg, 12, static_cast<uint32_t>(symtab.size()));
+  uint64_t cur = off_dict;
+  for (uint32_t k = 0; k < dict_count; ++k) {
+    if (!entry[k].empty()) std::memcpy(seg.data() + cur, entry[k].data(), entry[k].size());
+    cur += entry[k].size();
+  }
+  if (!symtab.empty()) std::memcpy(seg.data() + off_symtab, symtab.data(), symtab.size());
+  pack_tight(seg.data() + off_slens, lens, slw);
+  if (mode != 2) pack_tight(seg.data() + off_didx, sel, diw);
+  return seg;
+}
+
+void dict_fsst_encode(sdr_encoded* e, const uint8_t* chars, const uint64_t* offs, const uint8_t* valid, uint64_t n,
+                      uint32_t mode, uint32_t seg_rows, uint32_t block, uint64_t seed)
+{
+  uint64_t r = 0;
+  while (r < n) {
+    uint64_t want = std::min<uint64_t>(seg_rows, n - r);
+    for (;;) {
+      uint32_t ml = 0;
+      std::vector<uint8_t> seg = dict_fsst_segment(chars, offs, valid, r, r + want, mode, seed ^ r, &ml);
+      if (seg.size() <= block || want == 1) {
+        add_segment(e, seg, r, static_cast<uint32_t>(want), ml);
+        r += want;
+        break;
+      }
+      want = std::max<uint64_t>(1, want / 2);
+    }
+  }
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
 // C ABI
 // ---------------------------------------------------------------------------
 extern "C" {
+
+sdr_encoded* sdr_encode_dict_fsst(const uint8_t* chars, const uint64_t* offsets, const uint8_t* valid, uint64_t n,
+                                  uint32_t mode, uint32_t seg_rows, uint32_t block_size, uint64_t seed)
+{
+  if (mode > 2 || seg_rows == 0) return nullptr;
+  auto* e = new sdr_encoded();
+  dict_fsst_encode(e, chars, offsets, valid, n, mode, seg_rows, block_size, seed);
+  return e;
+}
 
 sdr_encoded* sdr_encode_bitpacking(const void* values, uint64_t n, uint32_t ts, int is_signed,
                                    uint32_t rg_rows, uint32_t block, int force_mode)

@@ -54,6 +54,14 @@ sdr_encoded* sdr_encode_uncompressed(const void* values, uint64_t n, uint32_t ty
 sdr_encoded* sdr_encode_fsst(const uint8_t* chars, const uint64_t* offsets, uint64_t n,
                              uint32_t row_group_rows, uint32_t block_size, uint64_t seed);
 
+/* DICT_FSST segments of up to seg_rows rows (halved until a segment fits
+ * block_size): mode 0 raw dictionary, 1 FSST-compressed dictionary, 2 one
+ * FSST entry per row.  valid (n bytes, or NULL for all valid): rows with 0
+ * select entry 0, the NULL slot (modes 0 and 1). */
+sdr_encoded* sdr_encode_dict_fsst(const uint8_t* chars, const uint64_t* offsets, const uint8_t* valid,
+                                  uint64_t n, uint32_t mode, uint32_t seg_rows, uint32_t block_size,
+                                  uint64_t seed);
+
 const uint8_t* sdr_encoded_bytes(const sdr_encoded* e, uint64_t* size);
 const sdr_enc_segment* sdr_encoded_segments(const sdr_encoded* e, uint32_t* n);
 void sdr_encoded_free(sdr_encoded* e);

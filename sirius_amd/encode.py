@@ -37,8 +37,9 @@ def _load():
     lib.sdr_encode_alprd.argtypes = [vp, u64, u32, u32, u32]
     lib.sdr_encode_uncompressed.argtypes = [vp, u64, u32, u32, u32]
     lib.sdr_encode_fsst.argtypes = [vp, vp, u64, u32, u32, u64]
+    lib.sdr_encode_dict_fsst.argtypes = [vp, vp, vp, u64, u32, u32, u32, u64]
     for f in ("sdr_encode_bitpacking", "sdr_encode_rle", "sdr_encode_alp", "sdr_encode_alprd",
-              "sdr_encode_uncompressed", "sdr_encode_fsst"):
+              "sdr_encode_uncompressed", "sdr_encode_fsst", "sdr_encode_dict_fsst"):
         getattr(lib, f).restype = vp
     lib.sdr_encoded_bytes.argtypes = [vp, C.POINTER(u64)]
     lib.sdr_encoded_bytes.restype = vp
@@ -141,6 +142,19 @@ def encode_fsst(chars: np.ndarray, offsets: np.ndarray, seed: int = 0x5EED, rg_r
     n = o.size - 1
     h = _load().sdr_encode_fsst(pc, po, n, rg_rows, block, seed)
     return _take(h, _abi.FSST, 0, n)
+
+
+def encode_dict_fsst(chars: np.ndarray, offsets: np.ndarray, mode: int, valid: np.ndarray | None = None,
+                     seg_rows: int = 4096, block: int = BLOCK_SIZE, seed: int = 0x5EED) -> Encoded:
+    """DICT_FSST segments (mode 0 raw dictionary, 1 FSST-compressed
+    dictionary, 2 one FSST entry per row) of up to `seg_rows` rows each;
+    rows whose `valid` byte is 0 select entry 0, the NULL slot."""
+    c, pc = _c(chars.astype(np.uint8, copy=False))
+    o, po = _c(offsets.astype(np.uint64, copy=False))
+    n = o.size - 1
+    v, pv = (None, None) if valid is None else _c(np.asarray(valid, dtype=np.uint8))
+    h = _load().sdr_encode_dict_fsst(pc, po, pv, n, mode, seg_rows, block, seed)
+    return _take(h, _abi.DICT_FSST, 0, n)
 
 
 # ---- generators -------------------------------------------------------------
