@@ -225,6 +225,9 @@ typedef enum sdr_batch_option {
 int sdr_batch_create(sdr_batch** out);
 int sdr_batch_set_option(sdr_batch* batch, uint32_t key, int64_t value);
 int sdr_batch_add_table(sdr_batch* batch, sdr_column* cols, uint32_t n_cols);
+/* A column with DICT_FSST segments is refused (SDR_E_UNSUPPORTED_CODEC):
+ * its dictionaries are sized by host round trips a queued launch cannot
+ * make; sdr_decode_strings_column decodes it. */
 int sdr_batch_add_strings(sdr_batch* batch, sdr_string_column* col);
 int sdr_batch_num_entries(const sdr_batch* batch);
 /* Upload the queue and enqueue the persistent decode on `stream`. */
@@ -397,10 +400,17 @@ uint64_t sdr_strings_upper_bound(const sdr_run* runs, uint32_t n_runs);
 /* Decode one VARCHAR column.  Replaces gpu_decode_strings_column
  * (src/include/cuda/scan/gpu_decode_strings.cuh:71-73,
  * src/cuda/scan/gpu_decode_strings.cu:100-402).  Runs may mix FSST,
- * DICTIONARY and UNCOMPRESSED segments (src/cuda/scan/strings/fsst.cu,
- * dictionary.cu, uncompressed.cu); DICT_FSST returns
- * SDR_E_UNSUPPORTED_CODEC.  Synchronises `stream` once to read back
- * chars_used. */
+ * DICTIONARY, UNCOMPRESSED and DICT_FSST segments (src/cuda/scan/strings/
+ * fsst.cu, dictionary.cu, uncompressed.cu, dict_fsst.cu).  Synchronises
+ * `stream` once to read back chars_used; a column with DICT_FSST segments
+ * synchronises twice more, for their headers and then their dictionaries'
+ * decoded sizes (the reference's prepare_dict_fsst makes the same two
+ * round trips, dict_fsst.cu:471-690), and decodes through the per-column
+ * passes in every launch mode.  DICT_FSST rows that select entry 0 are
+ * NULL when any of the column's DICT_FSST segments has inline NULLs
+ * (dict_fsst.cu:238-243, :426-446): the mask is then built (validity runs,
+ * if any, and those rows) and d_null_mask is required, else the call
+ * returns SDR_E_INVALID_ARG before writing anything. */
 int sdr_decode_strings_column(sdr_string_column* col, sdr_stream_t stream);
 int sdr_decode_strings_column_mode(sdr_string_column* col, sdr_stream_t stream,
                                    uint32_t launch_mode);
@@ -421,7 +431,8 @@ int sdr_decode_strings_column_mode(sdr_string_column* col, sdr_stream_t stream,
  * SDR_E_CAPACITY before any device work.  Every call on a GPU shares that
  * GPU's array: calls that overlap on one GPU, on any streams or threads,
  * race and must be serialised by the caller.  Synchronises `stream` once,
- * as sdr_decode_strings_column does. */
+ * as sdr_decode_strings_column does.  DICT_FSST segments, whose
+ * dictionaries are sized at run time, are refused (SDR_E_UNSUPPORTED_CODEC). */
 int sdr_decode_strings_column_noalloc(sdr_string_column* col, sdr_stream_t stream);
 
 /* ---- row ids --------------------------------------------------------------- */

@@ -716,6 +716,7 @@ __device__ __forceinline__ void fsst_offsets(const FsstChunk& ch, uint32_t c, ui
 
 struct FsstPrepSmem {
   __attribute__((aligned(16))) uint8_t st[FSST_SYMSTAGE + 48];
+  uint8_t len[256];  // code lengths of a DICT_FSST dictionary's table
   uint32_t sum[FSST_THREADS];
   uint32_t huge[FSST_THREADS];
   uint32_t wave[FSST_WAVES];
@@ -804,6 +805,49 @@ __device__ __forceinline__ void plain_prep_segment(const FsstSeg& sg, FsstChunkI
   }
 }
 
+// ---- DICT_FSST strings ----
+//
+// strings/dict_fsst.cu:17-38: a 16-byte header, the dictionary entries front
+// to back from byte 16 (entry 0 is the NULL slot), in the FSST modes (1, 2)
+// a symbol table, the bit-packed entry lengths and, except in mode 2, the
+// bit-packed row indices; mode 0 keeps entries raw, mode 1 FSST-compresses
+// them, mode 2 has one entry per row (row i is entry i + 1).  The host reads
+// the headers and fills DictFsstSeg (region offsets, :516-525); a pre-pass
+// per segment builds the entry offsets (:160-235), a predecode pass writes
+// the FSST entries once (:268-320), and the rows copy their entry like
+// DICTIONARY rows (:246-265, :324-360).  Bytes past the segment read as
+// zero; the byte a trailing escape counts is written as 0.
+
+// Dictionary index of segment row sr (row + 1 in mode 2).
+__device__ __forceinline__ uint32_t dfsst_index(const DictFsstSeg& d, uint32_t sr)
+{
+  if (d.mode == 2u) return sr + 1u;
+  return stream_bits32(d.bytes + d.off_didx, d.bytes + d.bytes_size, uint64_t{sr} * d.diw, d.diw);
+}
+
+// Decoded length of one compressed entry of n bytes at p (bytes at or past
+// `end` read as zero, i.e. as code 0): a code adds its symbol's length, an
+// escape adds one and skips the next byte, even past the entry's end.
+__device__ __forceinline__ uint32_t dfsst_entry_len(const uint8_t* p, uint32_t n, const uint8_t* end,
+                                                    const uint8_t* len)
+{
+  const uint64_t in = p < end ? static_cast<uint64_t>(end - p) : 0u;
+  const uint32_t n_in = n < in ? n : static_cast<uint32_t>(in);
+  uint32_t pos = 0, dl = 0;
+  while (pos < n_in) {
+    const uint32_t code = *gbl(p + pos);
+    ++pos;
+    if (code < FSST_ESC) {
+      dl += len[code];
+    } else {
+      ++pos;
+      ++dl;
+    }
+  }
+  if (pos < n) dl += (n - pos) * len[0];  // the zero bytes past the segment
+  return dl;
+}
+
 // Block sum of one u64 per thread (every thread gets it).
 template <class SM>
 __device__ __forceinline__ uint64_t block_sum_u64(uint64_t v, SM& sm)
@@ -887,8 +931,15 @@ __device__ __attribute__((noinline)) void plain_chunk(const uint8_t* seg, uint32
   const uint32_t tid  = threadIdx.x;
   const uint8_t* end  = seg + bytes_size;
   const bool dict     = flags & FI_DICT;
+  const bool dfsst    = flags & FI_DFSST;
   const uint32_t w    = flags & FI_WIDTH;
   const FsstChunk ch{0u, row_begin, rows, col_row};
+  DictFsstSeg dd{};
+  if (dfsst) {  // seg is the segment's descriptor; rows copy from its entries
+    dd  = gload(reinterpret_cast<const DictFsstSeg*>(seg));
+    seg = dd.src;
+    end = dd.src + dd.src_size;
+  }
   __syncthreads();  // the previous user of sm is done
   uint32_t T   = 0;
   uint64_t own = 0;
@@ -899,7 +950,14 @@ __device__ __attribute__((noinline)) void plain_chunk(const uint8_t* seg, uint32
     if (r < rows) {
       const uint32_t sr = ch.row_begin + r;
       uint32_t e0 = 0, e1 = 0;
-      if (dict) {
+      if (dfsst) {
+        const uint32_t idx = dfsst_index(dd, sr);
+        if (idx != 0u && idx < dd.dict_count) {
+          e0 = gbl(dd.e)[idx];
+          e1 = gbl(dd.e)[idx + 1u];
+        }
+        dict_end = e1 >= e0 ? e1 : 0u;  // the entries are front to back: a row starts at e0
+      } else if (dict) {
         const uint32_t sel = stream_bits32(seg + 20, end, uint64_t{sr} * w, w);
         if (sel != 0u && sel < ibc) {
           e1 = ld_u32_any(seg + ibo + 4ull * sel);
@@ -913,7 +971,7 @@ __device__ __attribute__((noinline)) void plain_chunk(const uint8_t* seg, uint32
       }
       if (e1 <= dict_end && e1 >= e0) {
         len = e1 - e0;
-        src = dict_end - e1;
+        src = dfsst ? e0 : dict_end - e1;
       }
       sm.comp[r] = src;
     }
@@ -934,44 +992,20 @@ __device__ __attribute__((noinline)) void plain_chunk(const uint8_t* seg, uint32
   plain_emit_rows(sm, seg, end, rows, T, b, chars, chars_cap);
 }
 
-// Range `si` with the calling 256-thread workgroup: its chunk descriptors
-// into chunks[first_chunk + k]; for a segment, header checks
-// (fsst.cu:79-87), the symbol table import, one code per thread
-// (detail/fsst.cuh:50-96), into `tab`, and for each chunk k its
-// compressed-length prefix bounds (fsst.cu:132-188) and flags into
-// info[first_chunk + k].  Bytes past the segment read as zero; no symbol
-// table parse reads past byte 2057.
-__device__ __forceinline__ void fsst_prep_segment(const FsstSeg sg, uint32_t si, FsstChunk* __restrict__ chunks,
-                                                  FsstChunkInfo* __restrict__ info, FsstSymTab* __restrict__ tab,
-                                                  FsstPrepSmem& sm)
+// Import the symbol table serialised at seg + symoff (at most
+// FSST_SYMSTAGE bytes inside the segment are read; the rest read as zero)
+// into `tab`, one code per thread (detail/fsst.cuh:50-96); returns the
+// length of the calling thread's code.  A table whose version word does not
+// match decodes every code to nothing.  Every thread of the block must call
+// it.
+__device__ __forceinline__ uint32_t import_symtab(const uint8_t* seg, uint32_t bytes_size, uint32_t symoff, bool ok,
+                                                  FsstSymTab* __restrict__ tab, FsstPrepSmem& sm)
 {
   const uint32_t tid = threadIdx.x;
-  const uint8_t* seg = sg.bytes;
-  const uint8_t* end = seg + sg.bytes_size;
-  const bool gap     = seg == nullptr;
-  for (uint32_t k = tid; k * FSST_CHUNK_ROWS < sg.row_count; k += FSST_THREADS) {
-    const uint32_t r0 = k * FSST_CHUNK_ROWS;
-    const uint32_t n  = sg.row_count - r0 < FSST_CHUNK_ROWS ? sg.row_count - r0 : FSST_CHUNK_ROWS;
-    const FsstChunk fc{gap ? FSST_GAP : si, gap ? 0u : r0, n, sg.col_row + r0};
-    *gbl(reinterpret_cast<u32x4*>(chunks + sg.first_chunk + k)) = __builtin_bit_cast(u32x4, fc);
-  }
-  if (gap) return;  // uniform: nothing else of a gap is read
-  if (sg.codec == SDR_CODEC_DICTIONARY || sg.codec == SDR_CODEC_UNCOMPRESSED) {
-    plain_prep_segment(sg, info);
-    return;  // uniform per segment
-  }
-  bool ok = false;
-  uint32_t dict_end = 0, width = 0, symoff = 0;
-  if (sg.bytes_size >= 16u) {
-    dict_end = su32(seg + 4);
-    width    = su32(seg + 8);
-    symoff   = su32(seg + 12);
-    ok       = dict_end <= sg.bytes_size && symoff < dict_end && width <= 32u;
-  }
   __syncthreads();  // the previous segment's use of sm is done
   uint32_t sym_n = 0, sym_h = 0;
   if (ok) {
-    sym_n = sg.bytes_size - symoff;
+    sym_n = bytes_size - symoff;
     if (sym_n > FSST_SYMSTAGE) sym_n = FSST_SYMSTAGE;
     sym_h = block_stage<FSST_THREADS>(reinterpret_cast<uint32_t*>(sm.st), seg + symoff, sym_n);
   }
@@ -1013,6 +1047,150 @@ __device__ __forceinline__ void fsst_prep_segment(const FsstSeg sg, uint32_t si,
   }
   gbl(tab->sym)[tid] = sym;
   gbl(tab->len)[tid] = static_cast<uint8_t>(len);
+  return len;
+}
+
+// Dictionary pre-pass of one DICT_FSST segment with the calling block: the
+// entry byte offsets e_byte[0..dict_count] (scanned bit-packed lengths), for
+// the FSST modes with more than one entry the symbol table and the decoded
+// entry offsets (entry 0 empty), otherwise the byte offsets again, into
+// e_dec; out = {decoded total, inline-NULL flag}.  32-bit sums, as the
+// reference's scans.
+__device__ __forceinline__ void dfsst_prep_segment(const DictFsstSeg& d, uint32_t* __restrict__ e_byte,
+                                                   uint32_t* __restrict__ e_dec, FsstSymTab* __restrict__ tab,
+                                                   uint32_t* __restrict__ out, FsstPrepSmem& sm)
+{
+  const uint32_t tid = threadIdx.x;
+  if (!d.valid) {  // uniform per block
+    if (tid == 0) out[0] = out[1] = 0;
+    return;
+  }
+  const uint8_t* seg  = d.bytes;
+  const uint8_t* end  = seg + d.bytes_size;
+  const bool fsst     = d.mode != 0u && d.dict_count > 1u;
+  const uint32_t mylen = import_symtab(seg, d.bytes_size, d.off_symtab, fsst && d.off_symtab < d.bytes_size, tab, sm);
+  sm.len[tid] = static_cast<uint8_t>(mylen);
+  __syncthreads();
+  const uint8_t* slens = seg + d.off_slens;
+  const uint8_t* dict  = seg + d.off_dict;
+  uint32_t carry_b = 0, carry_d = 0, first_len = 0;
+  for (uint32_t k0 = 0; k0 < d.dict_count; k0 += 4u * FSST_THREADS) {
+    uint32_t v[4], dl[4], sb = 0, sd = 0;
+#pragma unroll
+    for (uint32_t i = 0; i < 4; ++i) {
+      const uint32_t k = k0 + 4u * tid + i;
+      v[i] = k < d.dict_count ? stream_bits32(slens, end, uint64_t{k} * d.slw, d.slw) : 0u;
+      sb += v[i];
+    }
+    uint32_t tb = 0;
+    uint32_t run = carry_b + block_excl_scan(sb, sm.wave, &tb);
+#pragma unroll
+    for (uint32_t i = 0; i < 4; ++i) {
+      const uint32_t k = k0 + 4u * tid + i;
+      if (k < d.dict_count) e_byte[d.e_byte + k] = run;
+      dl[i] = !fsst ? v[i] : (k == 0u || k >= d.dict_count) ? 0u : dfsst_entry_len(dict + run, v[i], end, sm.len);
+      if (k == 0u) first_len = v[i];
+      run += v[i];
+      sd += dl[i];
+    }
+    uint32_t td = 0;
+    uint32_t rd = carry_d + block_excl_scan(sd, sm.wave, &td);
+#pragma unroll
+    for (uint32_t i = 0; i < 4; ++i) {
+      const uint32_t k = k0 + 4u * tid + i;
+      if (k < d.dict_count) e_dec[d.e_byte + k] = rd;
+      rd += dl[i];
+    }
+    carry_b += tb;
+    carry_d += td;
+  }
+  if (tid == 0) {
+    e_byte[d.e_byte + d.dict_count] = carry_b;
+    e_dec[d.e_byte + d.dict_count]  = carry_d;
+    out[0] = carry_d;
+    out[1] = d.mode != 2u && d.dict_count > 1u && first_len == 0u;
+  }
+}
+
+// Predecode the FSST entries k >= 1 of one segment into d.src at d.e[k]
+// (exactly d.e[k + 1] - d.e[k] bytes; the byte of a trailing escape and
+// bytes past the decoded ones are 0), entries strided over `stride`
+// threads starting at `t`.
+__device__ __forceinline__ void dfsst_predecode(const DictFsstSeg& d, const uint32_t* __restrict__ e_byte,
+                                                const uint64_t* sym, const uint8_t* len, uint32_t t, uint32_t stride)
+{
+  const uint8_t* end  = d.bytes + d.bytes_size;
+  const uint8_t* dict = d.bytes + d.off_dict;
+  uint8_t* out        = const_cast<uint8_t*>(d.src);
+  for (uint32_t k = 1u + t; k < d.dict_count; k += stride) {
+    const uint32_t c0 = gbl(e_byte)[d.e_byte + k], c1 = gbl(e_byte)[d.e_byte + k + 1u];
+    const uint32_t n  = c1 >= c0 ? c1 - c0 : 0u;
+    uint64_t o        = gbl(d.e)[k];
+    uint64_t lim      = gbl(d.e)[k + 1u];
+    if (lim < o) lim = o;
+    if (lim > d.src_size) lim = d.src_size;
+    const uint8_t* p  = dict + c0;
+    uint32_t pos = 0;
+    while (pos < n && o < lim) {
+      const uint32_t code = p + pos < end ? *gbl(p + pos) : 0u;
+      ++pos;
+      if (code < FSST_ESC) {
+        const uint64_t v = sym[code];
+        for (uint32_t j = 0; j < len[code] && o < lim; ++j) *gbl(out + o++) = static_cast<uint8_t>(v >> (8u * j));
+      } else if (pos < n) {
+        *gbl(out + o++) = p + pos < end ? *gbl(p + pos) : 0u;
+        ++pos;
+      }
+    }
+    while (o < lim) *gbl(out + o++) = 0u;
+  }
+}
+
+// Range `si` with the calling 256-thread workgroup: its chunk descriptors
+// into chunks[first_chunk + k]; for a segment, header checks
+// (fsst.cu:79-87), the symbol table import, one code per thread
+// (detail/fsst.cuh:50-96), into `tab`, and for each chunk k its
+// compressed-length prefix bounds (fsst.cu:132-188) and flags into
+// info[first_chunk + k].  Bytes past the segment read as zero; no symbol
+// table parse reads past byte 2057.
+__device__ __forceinline__ void fsst_prep_segment(const FsstSeg sg, uint32_t si, FsstChunk* __restrict__ chunks,
+                                                  FsstChunkInfo* __restrict__ info, FsstSymTab* __restrict__ tab,
+                                                  FsstPrepSmem& sm)
+{
+  const uint32_t tid = threadIdx.x;
+  const uint8_t* seg = sg.bytes;
+  const uint8_t* end = seg + sg.bytes_size;
+  const bool gap     = seg == nullptr;
+  for (uint32_t k = tid; k * FSST_CHUNK_ROWS < sg.row_count; k += FSST_THREADS) {
+    const uint32_t r0 = k * FSST_CHUNK_ROWS;
+    const uint32_t n  = sg.row_count - r0 < FSST_CHUNK_ROWS ? sg.row_count - r0 : FSST_CHUNK_ROWS;
+    const FsstChunk fc{gap ? FSST_GAP : si, gap ? 0u : r0, n, sg.col_row + r0};
+    *gbl(reinterpret_cast<u32x4*>(chunks + sg.first_chunk + k)) = __builtin_bit_cast(u32x4, fc);
+  }
+  if (gap) return;  // uniform: nothing else of a gap is read
+  if (sg.codec == SDR_CODEC_DICTIONARY || sg.codec == SDR_CODEC_UNCOMPRESSED) {
+    plain_prep_segment(sg, info);
+    return;  // uniform per segment
+  }
+  if (sg.codec == SDR_CODEC_DICT_FSST) {  // bytes: the range's DictFsstSeg; pad: 1 if valid
+    for (uint32_t k = tid; k * FSST_CHUNK_ROWS < sg.row_count; k += FSST_THREADS) {
+      FsstChunkInfo ci{sg.bytes, 0u, 0u, 0u, FI_DFSST | (sg.pad ? FI_OK : 0u), 0u, 0u};
+      const gptr<u32x4> dst = gbl(reinterpret_cast<u32x4*>(info + sg.first_chunk + k));
+      const u32x8 cv        = __builtin_bit_cast(u32x8, ci);
+      dst[0]                = cv.lo;
+      dst[1]                = cv.hi;
+    }
+    return;  // uniform per segment
+  }
+  bool ok = false;
+  uint32_t dict_end = 0, width = 0, symoff = 0;
+  if (sg.bytes_size >= 16u) {
+    dict_end = su32(seg + 4);
+    width    = su32(seg + 8);
+    symoff   = su32(seg + 12);
+    ok       = dict_end <= sg.bytes_size && symoff < dict_end && width <= 32u;
+  }
+  import_symtab(seg, sg.bytes_size, symoff, ok, tab, sm);
 
   // chunk bounds: wave w sums chunks w, w + 4, ..., 16 fields per lane
   const uint32_t rows = sg.row_count;
@@ -1087,7 +1265,7 @@ __device__ __forceinline__ void fsst_chunk(const FsstChunk ch, const FsstChunkIn
 {
   const uint32_t tid      = threadIdx.x;
   const bool ok           = ch.seg != FSST_GAP && (ci.flags & FI_OK);
-  if (ok && (ci.flags & (FI_DICT | FI_UNC))) {  // uniform per chunk
+  if (ok && (ci.flags & (FI_DICT | FI_UNC | FI_DFSST))) {  // uniform per chunk
     plain_chunk(ci.bytes, ci.bytes_size, ci.dict_end, ci.flags, ci.c_lo, ci.c_hi, ch.row_begin, ch.rows, ch.col_row,
                 c, n_chunks, st, offsets, chars, chars_cap, total_rows, spin_limit, sm);
     return;

@@ -660,6 +660,10 @@ int sdr_batch_add_strings(sdr_batch* b, sdr_string_column* col)
   if (!b) return fail(SDR_E_INVALID_ARG, "batch is NULL");
   int rc = validate_string_column(col);
   if (rc) return rc;
+  if (has_dict_fsst(col))
+    return fail(SDR_E_UNSUPPORTED_CODEC,
+                "DICT_FSST strings need host round trips for their dictionaries; decode them with "
+                "sdr_decode_strings_column");
   col->chars_used = 0;
   col->null_count = 0;
   return batch_add_strings(b, col);

@@ -49,6 +49,7 @@ constexpr uint32_t FI_LFAST = 1u << 9;    // the chunk's length fields lie insid
 constexpr uint32_t FI_HUGE  = 1u << 10;   // a row of the chunk has >= 1 MiB of compressed bytes
 constexpr uint32_t FI_DICT  = 1u << 11;   // a DICTIONARY segment (bits 0-7: selection width)
 constexpr uint32_t FI_UNC   = 1u << 12;   // an UNCOMPRESSED string segment
+constexpr uint32_t FI_DFSST = 1u << 13;   // a DICT_FSST segment (bytes: its DictFsstSeg)
 
 struct FsstChunkInfo {
   const uint8_t* bytes;  // the segment
@@ -70,6 +71,41 @@ struct FsstSymTab {
   uint8_t len[256];
 };
 static_assert(sizeof(FsstSymTab) == 2304, "FsstSymTab layout");
+
+// A DICT_FSST segment (strings/dict_fsst.cu:17-38): the host fills the
+// region offsets from the 16-byte header (:64-72, :516-525), the dictionary
+// pre-pass the entry offsets, and `src` is the raw dict region (mode 0, or
+// a dictionary of at most one entry) or the entries the predecode pass
+// wrote.  Entry k of the segment is [e[k], e[k + 1]) of src.
+struct DictFsstSeg {
+  const uint8_t* bytes;  // the segment
+  const uint32_t* e;     // dict_count + 1 entry offsets into src
+  const uint8_t* src;
+  uint64_t src_size;     // readable bytes at src
+  uint32_t bytes_size;
+  uint32_t dict_count;
+  uint32_t off_dict, off_symtab, off_slens, off_didx;
+  uint32_t range;        // its row range: index of its symbol table
+  uint8_t mode, slw, diw, valid;
+  uint32_t e_byte;       // offset of its byte_off entries in the pre-pass array
+  uint32_t pad;
+};
+static_assert(sizeof(DictFsstSeg) == 72, "DictFsstSeg layout");
+
+// DICT_FSST passes (k_fsst.hip): the 16-byte headers of the DICT_FSST
+// ranges into hdr[4 * i] (ranges of other codecs are skipped); the
+// per-segment dictionary pre-pass (byte and decoded entry offsets, symbol
+// table, per-segment decoded total and inline-NULL flag into out[2 * j]);
+// the predecode of FSST-mode entries; and the inline-NULL marking of a
+// column's mask, counting the rows it clears into *d_count.
+hipError_t launch_dfsst_headers(const FsstSeg* d_segs, uint32_t n_ranges, uint32_t* d_hdr, hipStream_t s);
+hipError_t launch_dfsst_prep(const DictFsstSeg* d_dseg, uint32_t n_dseg, uint32_t* d_e_byte, uint32_t* d_e_dec,
+                             FsstSymTab* d_tabs, uint32_t* d_out, hipStream_t s);
+hipError_t launch_dfsst_predecode(const DictFsstSeg* d_dseg, uint32_t n_dseg, const uint32_t* d_e_byte,
+                                  const FsstSymTab* d_tabs, hipStream_t s);
+hipError_t launch_dfsst_nulls(const FsstSeg* d_segs, uint32_t n_ranges, const DictFsstSeg* d_dseg,
+                              const uint32_t* d_range_dseg, uint8_t* d_mask, uint32_t total_rows,
+                              unsigned long long* d_count, hipStream_t s);
 
 // Device state of one column decode: ticket counter, look-back status per
 // chunk, error flag and the decoded chars total.

@@ -48,6 +48,8 @@ struct RunTarget {
 int decode_runs(const std::vector<RunTarget>& targets, hipStream_t s, uint32_t mode);
 int validate_fixed_columns(const sdr_column* cols, uint32_t n_cols);
 int validate_string_column(const sdr_string_column* col);
+// The column has DICT_FSST segments (decoded only by the per-column passes).
+bool has_dict_fsst(const sdr_string_column* col);
 // SDR_LAUNCH_PERSISTENT: one persistent launch through a thread-local batch.
 int decode_table_persistent(sdr_column* cols, uint32_t n_cols, hipStream_t s);
 int decode_strings_persistent(sdr_string_column* col, hipStream_t s);

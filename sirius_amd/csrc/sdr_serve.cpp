@@ -542,6 +542,9 @@ int sdr_server_submit(sdr_server* sv, const sdr_request* r)
     s.n_data     = r->n_data;
     s.d_offsets  = reinterpret_cast<int32_t*>(placeholder);
     rc           = validate_string_column(&s);
+    if (!rc && sv->cfg.mode == SDR_LAUNCH_PERSISTENT && has_dict_fsst(&s))
+      rc = fail(SDR_E_UNSUPPORTED_CODEC,
+                "DICT_FSST requests need host round trips for their dictionaries: serve them in a drop-in mode");
     q->out_bytes = (((uint64_t{r->total_rows} + 1u) * 4u + 255u) & ~uint64_t{255}) + q->chars_cap;
   } else {
     auto& c = q->col;

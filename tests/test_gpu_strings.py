@@ -132,10 +132,11 @@ def test_capacity_overflow_reports_total():
     np.testing.assert_array_equal(g_offs.astype(np.uint64), offs)
 
 
-def test_unsupported_string_codec():
+@pytest.mark.parametrize("codec", [A.RLE, A.BITPACKING, A.ALP])
+def test_unsupported_string_codec(codec):
     chars, offs, enc = _zipf(100, 4)
     d, base = upload(enc.blob)
-    rc, col, _, _ = decode_strings_gpu([(A.DICT_FSST, enc.segments(base))], enc.n_rows, 10_000)
+    rc, col, _, _ = decode_strings_gpu([(codec, enc.segments(base))], enc.n_rows, 10_000)
     assert rc == A.SDR_E_UNSUPPORTED_CODEC
 
 
