@@ -15,8 +15,13 @@
  *     errors where the reference throws std::runtime_error;
  *   - malformed segment bytes never fault: the affected rows decode to zero
  *     (strings: to empty), as in the reference kernels;
- *   - all device work is enqueued on `stream`; entries are reentrant per
- *     stream and take no global lock.
+ *   - all device work is enqueued on `stream`; the entries may be called
+ *     from several host threads at once, on the same or different streams.
+ *     The pinned staging and descriptor slots they share are chosen under
+ *     short internal locks that make no blocking HIP call (waits, growth and
+ *     copies run on a slot the caller holds), so one caller's waits never
+ *     stall another's.  A batch, plan or server object is used from one
+ *     thread at a time.
  */
 #ifndef SDR_DECODE_H
 #define SDR_DECODE_H

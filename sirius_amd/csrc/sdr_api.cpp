@@ -66,15 +66,19 @@ void device_free(void* p, hipStream_t s)
 }
 
 namespace {
-// Pinned staging slots for descriptor uploads, kept for the process.
+// Pinned staging slots for descriptor uploads, kept for the process.  The
+// list's lock only guards choosing a slot (a non-blocking event query per
+// slot); allocation, the host copy and the enqueue run outside it on a slot
+// the caller holds.
 struct StageSlot {
   void* h          = nullptr;
   size_t cap       = 0;
   hipEvent_t done  = nullptr;  // recorded after the slot's last copy
   int dev          = -1;
+  bool busy        = false;    // held by a caller between choice and enqueue
 };
 std::mutex g_stage_mu;
-std::vector<StageSlot> g_stage;
+std::vector<StageSlot*> g_stage;  // never shrinks: held pointers stay valid
 constexpr size_t STAGE_SLOTS_MAX = 64;
 constexpr size_t STAGE_MIN_BYTES = 64u << 10;
 }  // namespace
@@ -84,33 +88,45 @@ int upload_async(void* d_dst, const void* h_src, size_t bytes, hipStream_t s)
   if (bytes == 0) return SDR_OK;
   int dev = 0;
   HIP_TRY(hipGetDevice(&dev));
-  std::lock_guard<std::mutex> lk(g_stage_mu);
   StageSlot* slot = nullptr;
-  for (auto& sl : g_stage)
-    if (sl.dev == dev && sl.cap >= bytes && hipEventQuery(sl.done) == hipSuccess) {
-      slot = &sl;
-      break;
+  bool fresh      = false;
+  {
+    std::lock_guard<std::mutex> lk(g_stage_mu);
+    for (StageSlot* sl : g_stage)
+      if (sl->dev == dev && !sl->busy && sl->cap >= bytes && hipEventQuery(sl->done) == hipSuccess) {
+        slot = sl;
+        break;
+      }
+    if (!slot && g_stage.size() < STAGE_SLOTS_MAX) {
+      slot      = new StageSlot;
+      slot->dev = dev;
+      g_stage.push_back(slot);
+      fresh = true;
     }
-  if (!slot && g_stage.size() < STAGE_SLOTS_MAX) {
-    StageSlot n;
-    n.dev = dev;
-    n.cap = STAGE_MIN_BYTES;
-    while (n.cap < bytes) n.cap *= 2;
-    HIP_TRY(hipHostMalloc(&n.h, n.cap, hipHostMallocDefault));
-    if (hipEventCreateWithFlags(&n.done, hipEventDisableTiming) != hipSuccess) {
-      (void)hipHostFree(n.h);
-      return fail(SDR_E_HIP, "staging event creation failed");
-    }
-    g_stage.push_back(n);
-    slot = &g_stage.back();
+    if (slot) slot->busy = true;
   }
   if (!slot) {  // every slot busy: a copy from the caller's memory (the runtime stages it)
     HIP_TRY(hipMemcpyAsync(d_dst, h_src, bytes, hipMemcpyHostToDevice, s));
     return SDR_OK;
   }
-  std::memcpy(slot->h, h_src, bytes);
-  HIP_TRY(hipMemcpyAsync(d_dst, slot->h, bytes, hipMemcpyHostToDevice, s));
-  HIP_TRY(hipEventRecord(slot->done, s));
+  hipError_t e = hipSuccess;
+  if (fresh) {
+    size_t cap = STAGE_MIN_BYTES;
+    while (cap < bytes) cap *= 2;
+    e = hipHostMalloc(&slot->h, cap, hipHostMallocDefault);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&slot->done, hipEventDisableTiming);
+    if (e == hipSuccess) slot->cap = cap;
+  }
+  if (e == hipSuccess) {
+    std::memcpy(slot->h, h_src, bytes);
+    e = hipMemcpyAsync(d_dst, slot->h, bytes, hipMemcpyHostToDevice, s);
+  }
+  if (e == hipSuccess) e = hipEventRecord(slot->done, s);
+  {
+    std::lock_guard<std::mutex> lk(g_stage_mu);
+    slot->busy = false;  // a slot whose set-up failed keeps cap 0 and is never chosen
+  }
+  if (e != hipSuccess) return fail(SDR_E_HIP, "descriptor upload: %s", hipGetErrorString(e));
   return SDR_OK;
 }
 
@@ -140,14 +156,17 @@ constexpr size_t DESC_MIN_BYTES     = 64u << 10;
 // One copy stream per device (default priority, not synchronising with the
 // null stream) carries descriptor uploads while the caller's stream is busy,
 // so a call's upload runs while that stream still works on earlier calls;
-// the caller's stream then waits only for that copy's event.  Guarded by
-// g_desc_mu.
+// the caller's stream then waits only for that copy's event.  Created once
+// per device.
 hipStream_t g_copy_stream[64] = {};
+std::once_flag g_copy_once[64];
 
 int copy_stream(int dev, hipStream_t* out)
 {
   if (dev < 0 || dev >= 64) return fail(SDR_E_HIP, "device %d out of range", dev);
-  if (!g_copy_stream[dev]) HIP_TRY(hipStreamCreateWithFlags(&g_copy_stream[dev], hipStreamNonBlocking));
+  hipError_t e = hipSuccess;
+  std::call_once(g_copy_once[dev], [&]() { e = hipStreamCreateWithFlags(&g_copy_stream[dev], hipStreamNonBlocking); });
+  if (!g_copy_stream[dev]) return fail(SDR_E_HIP, "copy stream: %s", hipGetErrorString(e));
   *out = g_copy_stream[dev];
   return SDR_OK;
 }
@@ -159,67 +178,85 @@ int desc_acquire(const void* h_src, size_t bytes, hipStream_t s, void** d_out, v
   *token = nullptr;
   int dev = 0;
   HIP_TRY(hipGetDevice(&dev));
-  std::lock_guard<std::mutex> lk(g_desc_mu);
-  DescSlot* pick   = nullptr;
-  DescSlot* oldest = nullptr;
-  size_t n_dev     = 0;
-  for (DescSlot* sl : g_desc) {
-    if (sl->dev != dev) continue;
-    ++n_dev;
-    if (sl->held) continue;
-    if (hipEventQuery(sl->done) == hipSuccess) {
-      if (!pick || (sl->cap >= bytes && pick->cap < bytes)) pick = sl;
-    } else if (!oldest || sl->seq < oldest->seq) {
-      oldest = sl;
+  DescSlot* pick = nullptr;
+  bool fresh = false, wait = false;
+  {
+    // choose under the lock (non-blocking event queries only); the waits,
+    // allocations and copies below run outside it on the held slot
+    std::lock_guard<std::mutex> lk(g_desc_mu);
+    DescSlot* oldest = nullptr;
+    size_t n_dev     = 0;
+    for (DescSlot* sl : g_desc) {
+      if (sl->dev != dev) continue;
+      ++n_dev;
+      if (sl->held) continue;
+      if (hipEventQuery(sl->done) == hipSuccess) {
+        if (!pick || (sl->cap >= bytes && pick->cap < bytes)) pick = sl;
+      } else if (!oldest || sl->seq < oldest->seq) {
+        oldest = sl;
+      }
     }
+    if (!pick && (n_dev < DESC_SLOTS_PER_DEV || !oldest)) {
+      pick      = new DescSlot;
+      pick->dev = dev;
+      g_desc.push_back(pick);
+      fresh = true;
+    } else if (!pick) {  // every slot in flight: wait for the oldest
+      pick = oldest;
+      wait = true;
+    }
+    pick->held = true;
   }
-  if (!pick && (n_dev < DESC_SLOTS_PER_DEV || !oldest)) {
-    auto* n = new DescSlot;
-    n->dev  = dev;
-    if (hipEventCreateWithFlags(&n->done, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&n->up, hipEventDisableTiming) != hipSuccess) {
-      if (n->done) (void)hipEventDestroy(n->done);
-      delete n;
+  auto unhold = [&](int rc) {
+    std::lock_guard<std::mutex> lk(g_desc_mu);
+    pick->held = false;
+    return rc;
+  };
+  hipError_t e = hipSuccess;
+  if (fresh) {
+    e = hipEventCreateWithFlags(&pick->done, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&pick->up, hipEventDisableTiming);
+    if (e != hipSuccess) {  // leave a slot whose events exist, or none
+      std::lock_guard<std::mutex> lk(g_desc_mu);
+      g_desc.erase(std::find(g_desc.begin(), g_desc.end(), pick));
+      if (pick->done) (void)hipEventDestroy(pick->done);
+      delete pick;
       return fail(SDR_E_HIP, "descriptor slot event creation failed");
     }
-    g_desc.push_back(n);
-    pick = n;
   }
-  if (!pick) {  // every slot in flight: wait for the oldest
-    HIP_TRY(hipEventSynchronize(oldest->done));
-    pick = oldest;
-  }
-  if (pick->cap < bytes) {  // its event has completed: nothing still reads the old buffers
-    if (pick->h) HIP_TRY(hipHostFree(pick->h));
-    if (pick->d) HIP_TRY(hipFree(pick->d));
+  if (wait) e = hipEventSynchronize(pick->done);
+  if (e == hipSuccess && pick->cap < bytes) {  // its event has completed: nothing still reads the old buffers
+    if (pick->h) e = hipHostFree(pick->h);
+    if (e == hipSuccess && pick->d) e = hipFree(pick->d);
     pick->h = pick->d = nullptr;
     pick->cap         = 0;
     size_t cap        = DESC_MIN_BYTES;
     while (cap < bytes) cap *= 2;
-    HIP_TRY(hipHostMalloc(&pick->h, cap, hipHostMallocDefault));
-    HIP_TRY(hipMalloc(&pick->d, cap));
-    pick->cap = cap;
+    if (e == hipSuccess) e = hipHostMalloc(&pick->h, cap, hipHostMallocDefault);
+    if (e == hipSuccess) e = hipMalloc(&pick->d, cap);
+    if (e == hipSuccess) pick->cap = cap;
   }
+  if (e != hipSuccess) return unhold(fail(SDR_E_HIP, "descriptor slot: %s", hipGetErrorString(e)));
   std::memcpy(pick->h, h_src, bytes);
   // The copy goes on the copy stream only while the caller's stream still
   // has work queued: on an idle stream the cross-stream wait would only add
   // latency.  Under stream capture it stays on the caller's stream, as a
   // node of the caller's graph.
   hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
-  HIP_TRY(hipStreamIsCapturing(s, &cap));
-  if (cap == hipStreamCaptureStatusNone && hipStreamQuery(s) == hipErrorNotReady) {
+  e = hipStreamIsCapturing(s, &cap);
+  if (e == hipSuccess && cap == hipStreamCaptureStatusNone && hipStreamQuery(s) == hipErrorNotReady) {
     hipStream_t cs = nullptr;
-    int rc = copy_stream(dev, &cs);
-    if (rc) return rc;
-    HIP_TRY(hipMemcpyAsync(pick->d, pick->h, bytes, hipMemcpyHostToDevice, cs));
-    HIP_TRY(hipEventRecord(pick->up, cs));
-    HIP_TRY(hipStreamWaitEvent(s, pick->up, 0));
-  } else {
-    HIP_TRY(hipMemcpyAsync(pick->d, pick->h, bytes, hipMemcpyHostToDevice, s));
+    const int rc   = copy_stream(dev, &cs);
+    if (rc) return unhold(rc);
+    e = hipMemcpyAsync(pick->d, pick->h, bytes, hipMemcpyHostToDevice, cs);
+    if (e == hipSuccess) e = hipEventRecord(pick->up, cs);
+    if (e == hipSuccess) e = hipStreamWaitEvent(s, pick->up, 0);
+  } else if (e == hipSuccess) {
+    e = hipMemcpyAsync(pick->d, pick->h, bytes, hipMemcpyHostToDevice, s);
   }
-  pick->held = true;
-  *d_out     = pick->d;
-  *token     = pick;
+  if (e != hipSuccess) return unhold(fail(SDR_E_HIP, "descriptor upload: %s", hipGetErrorString(e)));
+  *d_out = pick->d;
+  *token = pick;
   return SDR_OK;
 }
 
@@ -227,14 +264,14 @@ int desc_release(void* token, hipStream_t s)
 {
   if (!token) return SDR_OK;
   auto* sl = static_cast<DescSlot*>(token);
-  std::lock_guard<std::mutex> lk(g_desc_mu);
-  const hipError_t e = hipEventRecord(sl->done, s);
-  sl->seq  = ++g_desc_seq;
-  sl->held = false;
-  if (e != hipSuccess) {  // the slot cannot be tracked: wait for the stream instead
-    (void)hipStreamSynchronize(s);
-    return fail(SDR_E_HIP, "descriptor slot event: %s", hipGetErrorString(e));
+  const hipError_t e = hipEventRecord(sl->done, s);  // the slot is still held: no other caller touches it
+  if (e != hipSuccess) (void)hipStreamSynchronize(s);  // the slot cannot be tracked: wait for the stream instead
+  {
+    std::lock_guard<std::mutex> lk(g_desc_mu);
+    sl->seq  = ++g_desc_seq;
+    sl->held = false;
   }
+  if (e != hipSuccess) return fail(SDR_E_HIP, "descriptor slot event: %s", hipGetErrorString(e));
   return SDR_OK;
 }
 

@@ -11,8 +11,8 @@
 // a segment sits at its piece's position plus its offset in the piece.  The
 // decoders take segments at any byte alignment, so no gather follows.
 // Reads of up to 4 MiB are spread over up to 8 threads.  Pinned buffers are
-// reused per device, two at a time, so one call's reads overlap the
-// previous call's copy.
+// reused per device, up to eight of them, so one call's reads overlap the
+// previous call's copy and calls from several threads run side by side.
 #include "sdr_internal.hpp"
 
 #include <fcntl.h>
@@ -73,17 +73,54 @@ uint64_t plan(sdr_extent* ext, uint32_t n, uint64_t max_gap, std::vector<Piece>&
   return (end + PIECE_ALIGN - 1u) & ~(PIECE_ALIGN - 1u);
 }
 
+// Pinned buffers reused across calls, per device.  The pool's lock only
+// guards choosing a buffer (non-blocking event queries); the wait for its
+// last copy, a growth, the file reads and the enqueue run outside it on a
+// buffer the caller holds, so calls from several threads overlap.
 struct PinnedBuf {
   void* h         = nullptr;
   uint64_t cap    = 0;
   hipEvent_t done = nullptr;  // recorded after the copy that last read it
-};
-struct DevStage {
-  PinnedBuf buf[2];
-  uint32_t next = 0;
+  int dev         = -1;
+  bool busy       = false;    // held by a caller
 };
 std::mutex g_stage_mu;
-DevStage g_dev[64];
+std::vector<PinnedBuf*> g_bufs;  // never shrinks: held pointers stay valid
+constexpr size_t BUFS_PER_DEV = 8;
+
+// A buffer of this device for `total` bytes: an idle one whose last copy has
+// finished (the largest fitting, else any), else a new one while the device
+// has fewer than BUFS_PER_DEV, else the idle one with the oldest copy
+// (waited for by the caller).  Returns nullptr when all are held.
+PinnedBuf* take_buf(int dev, uint64_t total)
+{
+  std::lock_guard<std::mutex> lk(g_stage_mu);
+  PinnedBuf *done = nullptr, *idle = nullptr;
+  size_t n_dev = 0;
+  for (PinnedBuf* b : g_bufs) {
+    if (b->dev != dev) continue;
+    ++n_dev;
+    if (b->busy) continue;
+    idle = idle ? idle : b;
+    if (!b->done || hipEventQuery(b->done) == hipSuccess)
+      if (!done || (b->cap >= total && done->cap < total)) done = b;
+  }
+  PinnedBuf* b = done;
+  if (!b && n_dev < BUFS_PER_DEV) {
+    b      = new PinnedBuf;
+    b->dev = dev;
+    g_bufs.push_back(b);
+  }
+  if (!b) b = idle;
+  if (b) b->busy = true;
+  return b;
+}
+
+void give_buf(PinnedBuf* b)
+{
+  std::lock_guard<std::mutex> lk(g_stage_mu);
+  b->busy = false;
+}
 
 }  // namespace
 }  // namespace sdr
@@ -118,11 +155,13 @@ int sdr_stage_file(const char* path, sdr_extent* extents, uint32_t n, uint64_t m
   const int fd = open(path, O_RDONLY | O_CLOEXEC);
   if (fd < 0) return fail(SDR_E_INVALID_ARG, "cannot open %s: %s", path, std::strerror(errno));
 
-  std::lock_guard<std::mutex> lk(g_stage_mu);
-  DevStage& ds   = g_dev[dev];
-  PinnedBuf& pb  = ds.buf[ds.next];
-  ds.next       ^= 1u;
-  hipError_t e   = hipSuccess;
+  PinnedBuf* held = take_buf(dev, total);
+  if (!held) {
+    close(fd);
+    return fail(SDR_E_CAPACITY, "all %zu staging buffers of device %d are in use", BUFS_PER_DEV, dev);
+  }
+  PinnedBuf& pb = *held;
+  hipError_t e  = hipSuccess;
   if (pb.done) e = hipEventSynchronize(pb.done);  // the copy that last read this buffer is done
   if (e == hipSuccess && pb.cap < total) {
     if (pb.h) e = hipHostFree(pb.h);
@@ -134,6 +173,7 @@ int sdr_stage_file(const char* path, sdr_extent* extents, uint32_t n, uint64_t m
   if (e == hipSuccess && !pb.done) e = hipEventCreateWithFlags(&pb.done, hipEventDisableTiming);
   if (e != hipSuccess) {
     close(fd);
+    give_buf(held);
     return fail(SDR_E_HIP, "staging buffer: %s", hipGetErrorString(e));
   }
   // reads of at most READ_BYTES, spread over up to READ_THREADS threads
@@ -166,11 +206,16 @@ int sdr_stage_file(const char* path, sdr_extent* extents, uint32_t n, uint64_t m
   work();
   for (auto& th : pool) th.join();
   close(fd);
-  if (bad.load() != UINT64_MAX)
+  if (bad.load() != UINT64_MAX) {
+    give_buf(held);
     return fail(SDR_E_BOUNDS, "short read of %s at offset %llu", path, static_cast<unsigned long long>(bad.load()));
+  }
   hipStream_t s = static_cast<hipStream_t>(stream);
-  HIP_TRY(hipMemcpyAsync(d_dst, pb.h, total, hipMemcpyHostToDevice, s));
-  HIP_TRY(hipEventRecord(pb.done, s));
+  e             = hipMemcpyAsync(d_dst, pb.h, total, hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) e = hipEventRecord(pb.done, s);
+  if (e != hipSuccess) (void)hipStreamSynchronize(s);  // the buffer is not tracked: nothing may still read it
+  give_buf(held);
+  if (e != hipSuccess) return fail(SDR_E_HIP, "staging copy: %s", hipGetErrorString(e));
   return SDR_OK;
 }
 
