@@ -277,7 +277,11 @@ def apply_tuning(lib):
     if DEFAULT_TABLE.exists():
         L.check(lib.sdr_tune_load(str(DEFAULT_TABLE).encode()))
         log(f"[bench] tuning table {DEFAULT_TABLE.name} applied")
-    return {name: int(lib.sdr_tune_get(k)) for k, name in A.TUNE_NAMES.items()}
+    out = {}
+    for k, name in A.TUNE_NAMES.items():
+        vals = {w: int(lib.sdr_tune_get_width(k, w)) for w in A.TUNE_WIDTHS}
+        out[name] = vals[4] if len(set(vals.values())) == 1 else {f"@{w}": v for w, v in vals.items()}
+    return out
 
 
 def rounded(x, nd=4):

@@ -379,20 +379,36 @@ typedef enum sdr_tune_key {
                                        round: 512, 1024 or 2048 (2048)       */
   SDR_TUNE_ALP_VECTORS_PER_WAVE = 4, /* 1024-row ALP/ALPRD vectors one
                                        wavefront decodes in turn, 1..16 (1)  */
-  SDR_TUNE_FSST_SPIN_LIMIT     = 5  /* sleeps a VARCHAR chunk waits for its
+  SDR_TUNE_FSST_SPIN_LIMIT     = 5, /* sleeps a VARCHAR chunk waits for its
                                        predecessors' chars offsets before its
                                        decode reports SDR_E_HIP, 0..2^31
                                        (2^24, never reached); 0 reports at
                                        once without waiting (for tests)      */
+  /* work units per queue entry of a persistent batch, for every batch that
+   * does not set SDR_BATCH_UNITS_* itself: 1..2^20 */
+  SDR_TUNE_BATCH_UNITS_BITPACKING = 6, /* 2048-row groups (16)               */
+  SDR_TUNE_BATCH_UNITS_RLE      = 7,   /* RLE slices (2)                     */
+  SDR_TUNE_BATCH_UNITS_ALP      = 8,   /* 1024-row vectors (16)              */
+  SDR_TUNE_BATCH_UNITS_ALPRD    = 9,   /* 1024-row vectors (16)              */
+  SDR_TUNE_BATCH_UNITS_COPY     = 10   /* 16 KiB UNCOMPRESSED / CONSTANT
+                                          byte ranges (4)                    */
 } sdr_tune_key;
 
-/* Set a knob; SDR_E_INVALID_ARG for an unknown key or a value out of range. */
+/* Every knob holds one value per value width (1, 2, 4 and 8 bytes): the
+ * value used for a column of that type size.  sdr_tune_set sets all four;
+ * SDR_E_INVALID_ARG for an unknown key or a value out of range. */
 int sdr_tune_set(uint32_t key, uint64_t value);
-/* Current value of a knob, 0 for an unknown key. */
+/* Set one width's value (SDR_E_BAD_WIDTH for a width other than 1, 2, 4, 8). */
+int sdr_tune_set_width(uint32_t key, uint32_t type_size, uint64_t value);
+/* Current value of a knob for 4-byte values, 0 for an unknown key. */
 uint64_t sdr_tune_get(uint32_t key);
-/* Apply a table of "name value" lines (names: rle_slice_rows,
- * bp_groups_per_wave, bp_chunk_rows, alp_vectors_per_wave, fsst_spin_limit;
- * '#' starts a comment).  Stops at the first bad line with SDR_E_INVALID_ARG. */
+/* Current value for `type_size`-byte values, 0 for an unknown key or width. */
+uint64_t sdr_tune_get_width(uint32_t key, uint32_t type_size);
+/* Apply a table of "name value" lines (every width) and "name@ts value"
+ * lines (one width; names: rle_slice_rows, bp_groups_per_wave,
+ * bp_chunk_rows, alp_vectors_per_wave, fsst_spin_limit,
+ * batch_units_{bitpacking,rle,alp,alprd,copy}; '#' starts a comment).
+ * Stops at the first bad line with SDR_E_INVALID_ARG. */
 int sdr_tune_load(const char* path);
 
 /* ---- string decode ----------------------------------------------------- */

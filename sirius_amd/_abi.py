@@ -50,9 +50,17 @@ TUNE_BP_GROUPS_PER_WAVE = 2
 TUNE_BP_CHUNK_ROWS = 3
 TUNE_ALP_VECTORS_PER_WAVE = 4
 TUNE_FSST_SPIN_LIMIT = 5
+TUNE_BATCH_UNITS_BITPACKING = 6
+TUNE_BATCH_UNITS_RLE = 7
+TUNE_BATCH_UNITS_ALP = 8
+TUNE_BATCH_UNITS_ALPRD = 9
+TUNE_BATCH_UNITS_COPY = 10
 TUNE_NAMES = {TUNE_RLE_SLICE_ROWS: "rle_slice_rows", TUNE_BP_GROUPS_PER_WAVE: "bp_groups_per_wave",
               TUNE_BP_CHUNK_ROWS: "bp_chunk_rows", TUNE_ALP_VECTORS_PER_WAVE: "alp_vectors_per_wave",
-              TUNE_FSST_SPIN_LIMIT: "fsst_spin_limit"}
+              TUNE_FSST_SPIN_LIMIT: "fsst_spin_limit", TUNE_BATCH_UNITS_BITPACKING: "batch_units_bitpacking",
+              TUNE_BATCH_UNITS_RLE: "batch_units_rle", TUNE_BATCH_UNITS_ALP: "batch_units_alp",
+              TUNE_BATCH_UNITS_ALPRD: "batch_units_alprd", TUNE_BATCH_UNITS_COPY: "batch_units_copy"}
+TUNE_WIDTHS = (1, 2, 4, 8)
 
 
 class RowRange(C.Structure):

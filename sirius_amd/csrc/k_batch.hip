@@ -46,7 +46,7 @@ __global__ __launch_bounds__(BLOCK_THREADS) void k_bgroup_bp(const BatchItem* __
 {
   __shared__ uint32_t lds[WAVES_PER_BLOCK][(BATCH_BP_STAGE + 48) / 4];
   const uint32_t w     = wave_in_block();
-  const uint32_t chunk = q->bp_chunk_rows;
+  const uint32_t chunk = q->bp_chunk_rows[sizeof(T) == 1 ? 0 : sizeof(T) == 2 ? 1 : sizeof(T) == 4 ? 2 : 3];
   const uint32_t first = q->first[g], n = q->count[g];
   for (uint32_t e = blockIdx.x; e < n; e += gridDim.x) {
     const BatchItem it = items[first + e];

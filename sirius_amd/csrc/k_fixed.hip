@@ -143,8 +143,8 @@ hipError_t launch_bitpacking(const FixedSeg* d_segs, uint32_t seg_base, uint32_t
                              uint32_t ts, hipStream_t s)
 {
   if (n == 0 || max_rows == 0) return hipSuccess;
-  const uint32_t gpw   = tune(SDR_TUNE_BP_GROUPS_PER_WAVE);
-  const uint32_t chunk = tune(SDR_TUNE_BP_CHUNK_ROWS);
+  const uint32_t gpw   = tune(SDR_TUNE_BP_GROUPS_PER_WAVE, ts);
+  const uint32_t chunk = tune(SDR_TUNE_BP_CHUNK_ROWS, ts);
   const uint32_t units = (max_rows + BP_GROUP - 1) / BP_GROUP;
   const uint32_t gx    = (units + WAVES_PER_BLOCK * gpw - 1) / (WAVES_PER_BLOCK * gpw);
   return chunked(seg_base, n, [&](uint32_t b, uint32_t m) {
@@ -162,7 +162,7 @@ hipError_t launch_rle(const FixedSeg* d_segs, uint32_t seg_base, uint32_t n, uin
                       hipStream_t s)
 {
   if (n == 0 || max_rows == 0) return hipSuccess;
-  const uint32_t sr     = tune(SDR_TUNE_RLE_SLICE_ROWS);
+  const uint32_t sr     = tune(SDR_TUNE_RLE_SLICE_ROWS, ts);
   const uint32_t slices = (max_rows + sr - 1) / sr;
   // segments per launch so that the 1-D grid stays below 2^31 blocks
   const uint32_t per = static_cast<uint32_t>(std::min<uint64_t>(MAX_GRID_Y, (0x7FFFFFF0ull / slices)));
@@ -186,7 +186,7 @@ hipError_t launch_alp(const FixedSeg* d_segs, uint32_t seg_base, uint32_t n, uin
                       bool rd, hipStream_t s)
 {
   if (n == 0 || max_rows == 0) return hipSuccess;
-  const uint32_t vpw   = tune(SDR_TUNE_ALP_VECTORS_PER_WAVE);
+  const uint32_t vpw   = tune(SDR_TUNE_ALP_VECTORS_PER_WAVE, ts);
   const uint32_t units = (max_rows + ALP_VEC - 1) / ALP_VEC;
   const uint32_t gx    = (units + WAVES_PER_BLOCK * vpw - 1) / (WAVES_PER_BLOCK * vpw);
   return chunked(seg_base, n, [&](uint32_t b, uint32_t m) {

@@ -51,11 +51,11 @@ struct sdr_batch {
   size_t n_entries     = 0;                         // queue entries of the last upload
   uint32_t entry_count[sdr::BG_COUNT] = {};         // ... per group
   // launch configuration
-  // work units per queue entry, per BatchKind.  Each entry costs one claim
-  // (an atomic on the queue's one counter and two barriers); on the
-  // integer/date mix 16 groups and 2 RLE slices per entry took a step from
-  // 392 to 277 us against 4 and 1, and 16 ALP vectors 198 -> 152 us against 4.
+  // work units per queue entry, per BatchKind, when the batch sets them
+  // (SDR_BATCH_UNITS_*); otherwise the SDR_TUNE_BATCH_UNITS_* knob of the
+  // kind and value width
   uint32_t units[8]    = {0, 16, 2, 16, 16, 4, 4, 1};
+  bool units_set[8]    = {};
   uint32_t grid        = 0;                         // 0 = resident capacity
   bool use_graph       = true;
   hipGraphExec_t gexec = nullptr;
@@ -147,7 +147,7 @@ int batch_add_fixed(sdr_batch* b, sdr_column* cols, uint32_t n_cols)
     for (uint32_t r = 0; r < col.n_data; ++r) {
       const sdr_run& run = col.data[r];
       const BatchKind k  = kind_of(run.codec);
-      const uint32_t rle_slice = tune(SDR_TUNE_RLE_SLICE_ROWS);
+      const uint32_t rle_slice = tune(SDR_TUNE_RLE_SLICE_ROWS, col.type_size);
       for (uint32_t s = 0; s < run.n_segments; ++s) {
         const sdr_segment& g = run.segments[s];
         if (!g.row_count) continue;
@@ -231,6 +231,21 @@ int batch_group_caps(sdr_batch* b, uint32_t mask)
   return SDR_OK;
 }
 
+// Units per queue entry of kind k over values of ts bytes.
+uint32_t batch_units(const sdr_batch* b, uint32_t k, uint32_t ts)
+{
+  if (b->units_set[k]) return b->units[k];
+  switch (k) {
+    case BK_BITPACKING: return tune(SDR_TUNE_BATCH_UNITS_BITPACKING, ts);
+    case BK_RLE: return tune(SDR_TUNE_BATCH_UNITS_RLE, ts);
+    case BK_ALP: return tune(SDR_TUNE_BATCH_UNITS_ALP, ts);
+    case BK_ALPRD: return tune(SDR_TUNE_BATCH_UNITS_ALPRD, ts);
+    case BK_COPY:
+    case BK_CONSTANT: return tune(SDR_TUNE_BATCH_UNITS_COPY, ts);
+    default: return b->units[k];
+  }
+}
+
 // How the work items become queue entries.  A fixed-width group's items
 // are cut into entries of per[g] units: enough entries to give every
 // resident block of the group's kernel work (a multiple of the four
@@ -246,7 +261,7 @@ int batch_plan(sdr_batch* b, uint32_t per[BG_COUNT], uint32_t entries[BG_COUNT],
     const uint32_t g = b->item_group[i];
     *mask |= 1u << g;
     tot[g] += b->items[i].n_units;
-    maxu[g] = std::max<uint32_t>(1, b->units[b->items[i].kind]);
+    maxu[g] = std::max<uint32_t>(1, batch_units(b, b->items[i].kind, b->items[i].ts));
   }
   int rc = batch_group_caps(b, *mask);
   if (rc) return rc;
@@ -345,7 +360,7 @@ int batch_upload(sdr_batch* b, hipStream_t s)
     o += entries[g];
   }
   q.n_items       = static_cast<uint32_t>(n_entries);
-  q.bp_chunk_rows = tune(SDR_TUNE_BP_CHUNK_ROWS);
+  for (uint32_t w = 0; w < 4; ++w) q.bp_chunk_rows[w] = tune(SDR_TUNE_BP_CHUNK_ROWS, 1u << w);
   q.n_fsst_status = static_cast<uint32_t>(state_words);
   q.fsst_status   = reinterpret_cast<uint64_t*>(d + b->off_state);
   q.n_fsst_segs   = static_cast<uint32_t>(n_segs);
@@ -632,7 +647,8 @@ int sdr_batch_set_option(sdr_batch* b, uint32_t key, int64_t value)
   if (!b) return fail(SDR_E_INVALID_ARG, "batch is NULL");
   if (key >= SDR_BATCH_UNITS_BITPACKING && key <= SDR_BATCH_UNITS_CONSTANT) {
     if (value < 1 || value > 1 << 20) return fail(SDR_E_INVALID_ARG, "units per entry out of range");
-    b->units[key - SDR_BATCH_UNITS_BITPACKING + 1] = static_cast<uint32_t>(value);
+    b->units[key - SDR_BATCH_UNITS_BITPACKING + 1]     = static_cast<uint32_t>(value);
+    b->units_set[key - SDR_BATCH_UNITS_BITPACKING + 1] = true;
   } else if (key == SDR_BATCH_GRID) {
     if (value < 0 || value > 1 << 24) return fail(SDR_E_INVALID_ARG, "grid out of range");
     b->grid = static_cast<uint32_t>(value);

@@ -83,7 +83,7 @@ struct BatchQueue {
   uint32_t n_fsst_status;  // status words to clear in `fsst_status`
   uint32_t spin_limit;     // look-back wait bound of the FSST chunks
   uint32_t n_fsst_segs;    // row ranges of all FSST columns, for the pre-pass
-  uint32_t bp_chunk_rows;  // most rows a bitpacking wavefront stages at once
+  uint32_t bp_chunk_rows[4];  // most rows a bitpacking wavefront stages at once, per width 1/2/4/8
   uint64_t* fsst_status;   // contiguous FsstState storage of all FSST columns
   const FsstSeg* fsst_segs;      // their first_chunk index fsst_chunks and fsst_info
   FsstChunk* fsst_chunks;

@@ -23,7 +23,7 @@ DECLARED = [
     "sdr_plan_null_counts", "sdr_plan_destroy",
     "sdr_batch_create", "sdr_batch_set_option", "sdr_batch_add_table", "sdr_batch_add_strings",
     "sdr_batch_num_entries", "sdr_batch_launch", "sdr_batch_finish", "sdr_batch_clear", "sdr_batch_destroy",
-    "sdr_tune_set", "sdr_tune_get", "sdr_tune_load",
+    "sdr_tune_set", "sdr_tune_get", "sdr_tune_load", "sdr_tune_set_width", "sdr_tune_get_width",
     "sdr_server_create", "sdr_server_stage", "sdr_server_unstage", "sdr_server_submit", "sdr_server_progress",
     "sdr_server_poll", "sdr_server_release", "sdr_server_pending", "sdr_server_held_bytes", "sdr_server_now_ms",
     "sdr_server_destroy", "sdr_server_get_stats", "sdr_server_stage_wait",
@@ -124,6 +124,10 @@ def load() -> C.CDLL:
     lib.sdr_tune_get.restype = C.c_uint64
     lib.sdr_tune_load.argtypes = [C.c_char_p]
     lib.sdr_tune_load.restype = i32
+    lib.sdr_tune_set_width.argtypes = [u32, u32, C.c_uint64]
+    lib.sdr_tune_set_width.restype = i32
+    lib.sdr_tune_get_width.argtypes = [u32, u32]
+    lib.sdr_tune_get_width.restype = C.c_uint64
     lib.sdr_arrow_export.argtypes = [vp, vp, vp]
     lib.sdr_arrow_export.restype = i32
     lib.sdr_stage_plan.argtypes = [P(_abi.Extent), u32, C.c_uint64]

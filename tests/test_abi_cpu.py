@@ -118,6 +118,20 @@ def test_tuning_knobs_set_get_load(tmp_path):
         assert lib.sdr_tune_get(A.TUNE_RLE_SLICE_ROWS) == 4096  # lines before the bad one applied
         assert lib.sdr_tune_get(A.TUNE_BP_CHUNK_ROWS) == 2048
         assert lib.sdr_tune_load(str(tmp_path / "missing.txt").encode()) == A.SDR_E_NOT_FOUND
+        # per-width values: a plain line sets every width, "name@ts" one of them
+        t.write_text("rle_slice_rows 8192\nrle_slice_rows@8 24576\nbatch_units_rle@2 3\n")
+        assert lib.sdr_tune_load(str(t).encode()) == A.SDR_OK
+        assert [lib.sdr_tune_get_width(A.TUNE_RLE_SLICE_ROWS, w) for w in A.TUNE_WIDTHS] == [8192, 8192, 8192, 24576]
+        assert [lib.sdr_tune_get_width(A.TUNE_BATCH_UNITS_RLE, w) for w in A.TUNE_WIDTHS] == [2, 3, 2, 2]
+        assert lib.sdr_tune_get(A.TUNE_RLE_SLICE_ROWS) == 8192  # the 4-byte value
+        assert lib.sdr_tune_set_width(A.TUNE_BP_CHUNK_ROWS, 8, 1024) == A.SDR_OK
+        assert lib.sdr_tune_get_width(A.TUNE_BP_CHUNK_ROWS, 8) == 1024 and lib.sdr_tune_get(A.TUNE_BP_CHUNK_ROWS) == 2048
+        assert lib.sdr_tune_set_width(A.TUNE_BP_CHUNK_ROWS, 3, 1024) == A.SDR_E_BAD_WIDTH
+        assert lib.sdr_tune_set_width(A.TUNE_BP_CHUNK_ROWS, 8, 700) == A.SDR_E_INVALID_ARG
+        assert lib.sdr_tune_get_width(A.TUNE_BP_CHUNK_ROWS, 3) == 0
+        for bad in ("rle_slice_rows@3 8192\n", "rle_slice_rows@ 8192\n", "rle_slice_rows@8x 8192\n"):
+            t.write_text(bad)
+            assert lib.sdr_tune_load(str(t).encode()) == A.SDR_E_INVALID_ARG, bad
     finally:
         for k, v in defaults.items():
             assert lib.sdr_tune_set(k, v) == A.SDR_OK
