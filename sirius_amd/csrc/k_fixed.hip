@@ -7,19 +7,32 @@
 #include "sdr_tune.hpp"
 
 #include <algorithm>
+#include <cstring>
 
 namespace sdr {
 
+// Where a launch finds its segment descriptors: a device array, or the
+// kernel arguments themselves (at most SEGS_INLINE descriptors, 3 KiB of the
+// 4 KiB kernel-argument segment), which spares a small call the descriptor
+// upload.  blockIdx-indexed, so the load is uniform per workgroup.
+struct DevSegs {
+  const FixedSeg* p;
+  uint32_t base;
+  __device__ __forceinline__ FixedSeg at(uint32_t i) const { return p[base + i]; }
+};
+struct SegsInline {
+  FixedSeg seg[SEGS_INLINE];
+  __device__ __forceinline__ FixedSeg at(uint32_t i) const { return seg[i]; }
+};
+
 // Wave w of block x decodes groups [(4x + w) * gpw, (4x + w + 1) * gpw).
-template <typename T>
-__global__ __launch_bounds__(BLOCK_THREADS) void k_bitpacking(const FixedSeg* __restrict__ segs,
-                                                              uint32_t seg_base, uint32_t gpw,
-                                                              uint32_t chunk_rows)
+template <typename T, class Src>
+__global__ __launch_bounds__(BLOCK_THREADS) void k_bitpacking(Src src, uint32_t gpw, uint32_t chunk_rows)
 {
   // 4 KiB of packed bits per wave keeps 8 waves per SIMD resident
   constexpr uint32_t SB = 4096;
   __shared__ uint32_t lds[WAVES_PER_BLOCK][(SB + 48) / 4];
-  const FixedSeg sd  = segs[seg_base + blockIdx.y];
+  const FixedSeg sd  = src.at(blockIdx.y);
   const uint32_t w   = wave_in_block();
   const uint32_t ngr = (sd.row_count + BP_GROUP - 1) / BP_GROUP;
   const uint32_t k0  = (blockIdx.x * WAVES_PER_BLOCK + w) * gpw;
@@ -30,12 +43,11 @@ __global__ __launch_bounds__(BLOCK_THREADS) void k_bitpacking(const FixedSeg* __
   }
 }
 
-template <typename T, bool RD>
-__global__ __launch_bounds__(BLOCK_THREADS) void k_alp(const FixedSeg* __restrict__ segs, uint32_t seg_base,
-                                                       uint32_t vpw)
+template <typename T, bool RD, class Src>
+__global__ __launch_bounds__(BLOCK_THREADS) void k_alp(Src src, uint32_t vpw)
 {
   __shared__ uint32_t lds[WAVES_PER_BLOCK][RD ? ALPRD_LEFT_WORDS + ALP_CHUNK_WORDS : ALP_CHUNK_WORDS];
-  const FixedSeg sd  = segs[seg_base + blockIdx.y];
+  const FixedSeg sd  = src.at(blockIdx.y);
   const uint32_t w   = wave_in_block();
   const uint32_t nv  = (sd.row_count + ALP_VEC - 1) / ALP_VEC;
   const uint32_t v0  = (blockIdx.x * WAVES_PER_BLOCK + w) * vpw;
@@ -54,16 +66,15 @@ __global__ __launch_bounds__(BLOCK_THREADS) void k_alp(const FixedSeg* __restric
 // share one L2 and its counts are fetched from HBM about once.
 constexpr uint32_t N_XCD = 8;
 
-template <uint32_t TS>
-__global__ __launch_bounds__(RLE_THREADS) void k_rle(const FixedSeg* __restrict__ segs, uint32_t seg_base,
-                                                    uint32_t n_segs, uint32_t slices, uint32_t slice_rows)
+template <uint32_t TS, class Src>
+__global__ __launch_bounds__(RLE_THREADS) void k_rle(Src src, uint32_t n_segs, uint32_t slices, uint32_t slice_rows)
 {
   __shared__ RleSmem sm;
   const uint32_t per_xcd = gridDim.x / N_XCD;
   const uint32_t logical = (blockIdx.x % N_XCD) * per_xcd + blockIdx.x / N_XCD;
   const uint32_t sgi     = logical / slices;
   if (sgi >= n_segs) return;
-  const FixedSeg sd = segs[seg_base + sgi];
+  const FixedSeg sd = src.at(sgi);
   const uint32_t ra = (logical % slices) * slice_rows;
   if (ra >= sd.row_count) return;
   const uint32_t rb = sd.row_count - ra < slice_rows ? sd.row_count : ra + slice_rows;
@@ -78,10 +89,10 @@ __global__ __launch_bounds__(RLE_THREADS) void k_rle(const FixedSeg* __restrict_
 // ===========================================================================
 constexpr uint32_t COPY_UNIT = 65536;
 
-__global__ __launch_bounds__(BLOCK_THREADS) void k_copy(const FixedSeg* __restrict__ segs, uint32_t seg_base,
-                                                        uint32_t type_size)
+template <class Src>
+__global__ __launch_bounds__(BLOCK_THREADS) void k_copy(Src src, uint32_t type_size)
 {
-  const FixedSeg sd    = segs[seg_base + blockIdx.y];
+  const FixedSeg sd    = src.at(blockIdx.y);
   const uint64_t total = uint64_t{sd.row_count} * type_size;
   const uint64_t b0    = uint64_t{blockIdx.x} * COPY_UNIT;
   if (b0 >= total) return;
@@ -89,10 +100,10 @@ __global__ __launch_bounds__(BLOCK_THREADS) void k_copy(const FixedSeg* __restri
   copy_span<BLOCK_THREADS>(sd.bytes, sd.out, b0, b1, threadIdx.x);
 }
 
-__global__ __launch_bounds__(BLOCK_THREADS) void k_constant(const FixedSeg* __restrict__ segs, uint32_t seg_base,
-                                                            uint32_t type_size)
+template <class Src>
+__global__ __launch_bounds__(BLOCK_THREADS) void k_constant(Src src, uint32_t type_size)
 {
-  const FixedSeg sd    = segs[seg_base + blockIdx.y];
+  const FixedSeg sd    = src.at(blockIdx.y);
   const uint64_t total = uint64_t{sd.row_count} * type_size;
   const uint64_t b0    = uint64_t{blockIdx.x} * COPY_UNIT;
   if (b0 >= total) return;
@@ -126,98 +137,109 @@ __global__ void k_count_nulls(const uint8_t* mask, uint32_t rows, unsigned long 
 namespace {
 constexpr uint32_t MAX_GRID_Y = 65535;
 
-template <typename Launch>
-hipError_t chunked(uint32_t seg_base, uint32_t n, Launch f)
+// Call f(src, m) for every launch over the n descriptors at
+// segs[seg_base..]: device descriptors in pieces of at most max_m, or host
+// descriptors copied into the kernel arguments SEGS_INLINE at a time.
+template <typename F>
+hipError_t for_segs(const FixedSeg* segs, uint32_t seg_base, uint32_t n, bool host, uint32_t max_m, F f)
 {
-  for (uint32_t s = 0; s < n; s += MAX_GRID_Y) {
-    const uint32_t m = n - s < MAX_GRID_Y ? n - s : MAX_GRID_Y;
-    f(seg_base + s, m);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-  }
-  return hipSuccess;
-}
-}  // namespace
-
-hipError_t launch_bitpacking(const FixedSeg* d_segs, uint32_t seg_base, uint32_t n, uint32_t max_rows,
-                             uint32_t ts, hipStream_t s)
-{
-  if (n == 0 || max_rows == 0) return hipSuccess;
-  const uint32_t gpw   = tune(SDR_TUNE_BP_GROUPS_PER_WAVE, ts);
-  const uint32_t chunk = tune(SDR_TUNE_BP_CHUNK_ROWS, ts);
-  const uint32_t units = (max_rows + BP_GROUP - 1) / BP_GROUP;
-  const uint32_t gx    = (units + WAVES_PER_BLOCK * gpw - 1) / (WAVES_PER_BLOCK * gpw);
-  return chunked(seg_base, n, [&](uint32_t b, uint32_t m) {
-    dim3 g(gx, m);
-    switch (ts) {
-      case 1: hipLaunchKernelGGL(k_bitpacking<uint8_t>, g, dim3(BLOCK_THREADS), 0, s, d_segs, b, gpw, chunk); break;
-      case 2: hipLaunchKernelGGL(k_bitpacking<uint16_t>, g, dim3(BLOCK_THREADS), 0, s, d_segs, b, gpw, chunk); break;
-      case 4: hipLaunchKernelGGL(k_bitpacking<uint32_t>, g, dim3(BLOCK_THREADS), 0, s, d_segs, b, gpw, chunk); break;
-      default: hipLaunchKernelGGL(k_bitpacking<uint64_t>, g, dim3(BLOCK_THREADS), 0, s, d_segs, b, gpw, chunk); break;
-    }
-  });
-}
-
-hipError_t launch_rle(const FixedSeg* d_segs, uint32_t seg_base, uint32_t n, uint32_t max_rows, uint32_t ts,
-                      hipStream_t s)
-{
-  if (n == 0 || max_rows == 0) return hipSuccess;
-  const uint32_t sr     = tune(SDR_TUNE_RLE_SLICE_ROWS, ts);
-  const uint32_t slices = (max_rows + sr - 1) / sr;
-  // segments per launch so that the 1-D grid stays below 2^31 blocks
-  const uint32_t per = static_cast<uint32_t>(std::min<uint64_t>(MAX_GRID_Y, (0x7FFFFFF0ull / slices)));
-  for (uint32_t b = 0; b < n; b += per) {
-    const uint32_t m      = n - b < per ? n - b : per;
-    const uint64_t blocks = (uint64_t{m} * slices + N_XCD - 1) / N_XCD * N_XCD;
-    const dim3 grid(static_cast<uint32_t>(blocks)), block(RLE_THREADS);
-    switch (ts) {
-      case 1: hipLaunchKernelGGL(k_rle<1>, grid, block, 0, s, d_segs, seg_base + b, m, slices, sr); break;
-      case 2: hipLaunchKernelGGL(k_rle<2>, grid, block, 0, s, d_segs, seg_base + b, m, slices, sr); break;
-      case 4: hipLaunchKernelGGL(k_rle<4>, grid, block, 0, s, d_segs, seg_base + b, m, slices, sr); break;
-      default: hipLaunchKernelGGL(k_rle<8>, grid, block, 0, s, d_segs, seg_base + b, m, slices, sr); break;
+  const uint32_t lim = host ? std::min(max_m, SEGS_INLINE) : max_m;
+  for (uint32_t b = 0; b < n; b += lim) {
+    const uint32_t m = n - b < lim ? n - b : lim;
+    if (host) {
+      SegsInline si{};
+      std::memcpy(si.seg, segs + seg_base + b, size_t{m} * sizeof(FixedSeg));
+      f(si, m);
+    } else {
+      f(DevSegs{segs, seg_base + b}, m);
     }
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
   }
   return hipSuccess;
 }
+}  // namespace
 
-hipError_t launch_alp(const FixedSeg* d_segs, uint32_t seg_base, uint32_t n, uint32_t max_rows, uint32_t ts,
-                      bool rd, hipStream_t s)
+hipError_t launch_bitpacking(const FixedSeg* segs, uint32_t seg_base, uint32_t n, uint32_t max_rows, uint32_t ts,
+                             hipStream_t s, bool host)
+{
+  if (n == 0 || max_rows == 0) return hipSuccess;
+  const uint32_t gpw   = tune(SDR_TUNE_BP_GROUPS_PER_WAVE, ts);
+  const uint32_t chunk = tune(SDR_TUNE_BP_CHUNK_ROWS, ts);
+  const uint32_t units = (max_rows + BP_GROUP - 1) / BP_GROUP;
+  const uint32_t gx    = (units + WAVES_PER_BLOCK * gpw - 1) / (WAVES_PER_BLOCK * gpw);
+  return for_segs(segs, seg_base, n, host, MAX_GRID_Y, [&](auto src, uint32_t m) {
+    using S = decltype(src);
+    const dim3 g(gx, m);
+    switch (ts) {
+      case 1: hipLaunchKernelGGL((k_bitpacking<uint8_t, S>), g, dim3(BLOCK_THREADS), 0, s, src, gpw, chunk); break;
+      case 2: hipLaunchKernelGGL((k_bitpacking<uint16_t, S>), g, dim3(BLOCK_THREADS), 0, s, src, gpw, chunk); break;
+      case 4: hipLaunchKernelGGL((k_bitpacking<uint32_t, S>), g, dim3(BLOCK_THREADS), 0, s, src, gpw, chunk); break;
+      default: hipLaunchKernelGGL((k_bitpacking<uint64_t, S>), g, dim3(BLOCK_THREADS), 0, s, src, gpw, chunk); break;
+    }
+  });
+}
+
+hipError_t launch_rle(const FixedSeg* segs, uint32_t seg_base, uint32_t n, uint32_t max_rows, uint32_t ts,
+                      hipStream_t s, bool host)
+{
+  if (n == 0 || max_rows == 0) return hipSuccess;
+  const uint32_t sr     = tune(SDR_TUNE_RLE_SLICE_ROWS, ts);
+  const uint32_t slices = (max_rows + sr - 1) / sr;
+  // segments per launch so that the 1-D grid stays below 2^31 blocks
+  const uint32_t per = static_cast<uint32_t>(std::min<uint64_t>(MAX_GRID_Y, (0x7FFFFFF0ull / slices)));
+  return for_segs(segs, seg_base, n, host, per, [&](auto src, uint32_t m) {
+    using S = decltype(src);
+    const uint64_t blocks = (uint64_t{m} * slices + N_XCD - 1) / N_XCD * N_XCD;
+    const dim3 grid(static_cast<uint32_t>(blocks)), block(RLE_THREADS);
+    switch (ts) {
+      case 1: hipLaunchKernelGGL((k_rle<1, S>), grid, block, 0, s, src, m, slices, sr); break;
+      case 2: hipLaunchKernelGGL((k_rle<2, S>), grid, block, 0, s, src, m, slices, sr); break;
+      case 4: hipLaunchKernelGGL((k_rle<4, S>), grid, block, 0, s, src, m, slices, sr); break;
+      default: hipLaunchKernelGGL((k_rle<8, S>), grid, block, 0, s, src, m, slices, sr); break;
+    }
+  });
+}
+
+hipError_t launch_alp(const FixedSeg* segs, uint32_t seg_base, uint32_t n, uint32_t max_rows, uint32_t ts, bool rd,
+                      hipStream_t s, bool host)
 {
   if (n == 0 || max_rows == 0) return hipSuccess;
   const uint32_t vpw   = tune(SDR_TUNE_ALP_VECTORS_PER_WAVE, ts);
   const uint32_t units = (max_rows + ALP_VEC - 1) / ALP_VEC;
   const uint32_t gx    = (units + WAVES_PER_BLOCK * vpw - 1) / (WAVES_PER_BLOCK * vpw);
-  return chunked(seg_base, n, [&](uint32_t b, uint32_t m) {
-    dim3 g(gx, m);
+  return for_segs(segs, seg_base, n, host, MAX_GRID_Y, [&](auto src, uint32_t m) {
+    using S = decltype(src);
+    const dim3 g(gx, m);
     if (ts == 8) {
-      if (rd) hipLaunchKernelGGL((k_alp<double, true>), g, dim3(BLOCK_THREADS), 0, s, d_segs, b, vpw);
-      else hipLaunchKernelGGL((k_alp<double, false>), g, dim3(BLOCK_THREADS), 0, s, d_segs, b, vpw);
+      if (rd) hipLaunchKernelGGL((k_alp<double, true, S>), g, dim3(BLOCK_THREADS), 0, s, src, vpw);
+      else hipLaunchKernelGGL((k_alp<double, false, S>), g, dim3(BLOCK_THREADS), 0, s, src, vpw);
     } else {
-      if (rd) hipLaunchKernelGGL((k_alp<float, true>), g, dim3(BLOCK_THREADS), 0, s, d_segs, b, vpw);
-      else hipLaunchKernelGGL((k_alp<float, false>), g, dim3(BLOCK_THREADS), 0, s, d_segs, b, vpw);
+      if (rd) hipLaunchKernelGGL((k_alp<float, true, S>), g, dim3(BLOCK_THREADS), 0, s, src, vpw);
+      else hipLaunchKernelGGL((k_alp<float, false, S>), g, dim3(BLOCK_THREADS), 0, s, src, vpw);
     }
   });
 }
 
-hipError_t launch_uncompressed(const FixedSeg* d_segs, uint32_t seg_base, uint32_t n, uint32_t max_rows,
-                               uint32_t ts, hipStream_t s)
+hipError_t launch_uncompressed(const FixedSeg* segs, uint32_t seg_base, uint32_t n, uint32_t max_rows, uint32_t ts,
+                               hipStream_t s, bool host)
 {
   if (n == 0 || max_rows == 0) return hipSuccess;
   const uint64_t gx = (uint64_t{max_rows} * ts + COPY_UNIT - 1) / COPY_UNIT;
-  return chunked(seg_base, n, [&](uint32_t b, uint32_t m) {
-    hipLaunchKernelGGL(k_copy, dim3(static_cast<uint32_t>(gx), m), dim3(BLOCK_THREADS), 0, s, d_segs, b, ts);
+  return for_segs(segs, seg_base, n, host, MAX_GRID_Y, [&](auto src, uint32_t m) {
+    hipLaunchKernelGGL((k_copy<decltype(src)>), dim3(static_cast<uint32_t>(gx), m), dim3(BLOCK_THREADS), 0, s, src,
+                       ts);
   });
 }
 
-hipError_t launch_constant(const FixedSeg* d_segs, uint32_t seg_base, uint32_t n, uint32_t max_rows,
-                           uint32_t ts, hipStream_t s)
+hipError_t launch_constant(const FixedSeg* segs, uint32_t seg_base, uint32_t n, uint32_t max_rows, uint32_t ts,
+                           hipStream_t s, bool host)
 {
   if (n == 0 || max_rows == 0) return hipSuccess;
   const uint64_t gx = (uint64_t{max_rows} * ts + COPY_UNIT - 1) / COPY_UNIT;
-  return chunked(seg_base, n, [&](uint32_t b, uint32_t m) {
-    hipLaunchKernelGGL(k_constant, dim3(static_cast<uint32_t>(gx), m), dim3(BLOCK_THREADS), 0, s, d_segs, b, ts);
+  return for_segs(segs, seg_base, n, host, MAX_GRID_Y, [&](auto src, uint32_t m) {
+    hipLaunchKernelGGL((k_constant<decltype(src)>), dim3(static_cast<uint32_t>(gx), m), dim3(BLOCK_THREADS), 0, s,
+                       src, ts);
   });
 }
 

@@ -390,15 +390,15 @@ int validate_fixed_columns(const sdr_column* cols, uint32_t n_cols)
 namespace {
 
 hipError_t launch_batch(const Batch& b, const FixedSeg* d, uint32_t base, uint32_t n, uint32_t max_rows,
-                        hipStream_t s)
+                        hipStream_t s, bool host)
 {
   switch (b.codec) {
-    case SDR_CODEC_BITPACKING: return launch_bitpacking(d, base, n, max_rows, b.ts, s);
-    case SDR_CODEC_RLE: return launch_rle(d, base, n, max_rows, b.ts, s);
-    case SDR_CODEC_ALP: return launch_alp(d, base, n, max_rows, b.ts, false, s);
-    case SDR_CODEC_ALPRD: return launch_alp(d, base, n, max_rows, b.ts, true, s);
-    case SDR_CODEC_UNCOMPRESSED: return launch_uncompressed(d, base, n, max_rows, b.ts, s);
-    case SDR_CODEC_CONSTANT: return launch_constant(d, base, n, max_rows, b.ts, s);
+    case SDR_CODEC_BITPACKING: return launch_bitpacking(d, base, n, max_rows, b.ts, s, host);
+    case SDR_CODEC_RLE: return launch_rle(d, base, n, max_rows, b.ts, s, host);
+    case SDR_CODEC_ALP: return launch_alp(d, base, n, max_rows, b.ts, false, s, host);
+    case SDR_CODEC_ALPRD: return launch_alp(d, base, n, max_rows, b.ts, true, s, host);
+    case SDR_CODEC_UNCOMPRESSED: return launch_uncompressed(d, base, n, max_rows, b.ts, s, host);
+    case SDR_CODEC_CONSTANT: return launch_constant(d, base, n, max_rows, b.ts, s, host);
     default: return hipErrorInvalidValue;
   }
 }
@@ -437,24 +437,39 @@ int build_batches(const std::vector<RunTarget>& targets, std::vector<Batch>& bat
   return SDR_OK;
 }
 
+// `d`: the descriptors on the device, or null to carry them in the kernel
+// arguments from `all`.
 hipError_t launch_batches(const std::vector<Batch>& batches, const std::vector<FixedSeg>& all, const FixedSeg* d,
                           uint32_t mode, uint32_t codec_filter, hipStream_t s, int* n_launches)
 {
   hipError_t e = hipSuccess;
   int n = 0;
+  const bool host      = d == nullptr;
+  const FixedSeg* segs = host ? all.data() : d;
   for (size_t i = 0; e == hipSuccess && i < batches.size(); ++i) {
     const Batch& b = batches[i];
     if (!b.count || (codec_filter && b.codec != codec_filter)) continue;
     if (mode == SDR_LAUNCH_PER_SEGMENT) {
       for (uint32_t k = 0; k < b.count && e == hipSuccess; ++k, ++n)
-        e = launch_batch(b, d, b.first + k, 1, all[b.first + k].row_count, s);
+        e = launch_batch(b, segs, b.first + k, 1, all[b.first + k].row_count, s, host);
     } else {
-      e = launch_batch(b, d, b.first, b.count, b.max_rows, s);
+      e = launch_batch(b, segs, b.first, b.count, b.max_rows, s, host);
       ++n;
     }
   }
   if (n_launches) *n_launches = n;
   return e;
+}
+
+// A call's descriptors travel in the kernel arguments when each launch holds
+// one segment (the plain mode) or every (codec, width) group fits two
+// launches of SEGS_INLINE: no upload copy, event or cross-stream wait.
+bool inline_descriptors(const std::vector<Batch>& batches, uint32_t mode)
+{
+  if (mode == SDR_LAUNCH_PER_SEGMENT) return true;
+  for (const Batch& b : batches)
+    if (b.count > 2u * SEGS_INLINE) return false;
+  return true;
 }
 
 int decode_runs(const std::vector<RunTarget>& targets, hipStream_t s, uint32_t mode)
@@ -463,6 +478,11 @@ int decode_runs(const std::vector<RunTarget>& targets, hipStream_t s, uint32_t m
   std::vector<FixedSeg> all;
   build_batches(targets, batches, all);
   if (all.empty()) return SDR_OK;
+  if (inline_descriptors(batches, mode)) {
+    const hipError_t e = launch_batches(batches, all, nullptr, mode, 0, s, nullptr);
+    if (e != hipSuccess) return fail(SDR_E_HIP, "kernel launch failed: %s", hipGetErrorString(e));
+    return SDR_OK;
+  }
   void* d     = nullptr;
   void* token = nullptr;
   int rc = desc_acquire(all.data(), all.size() * sizeof(FixedSeg), s, &d, &token);

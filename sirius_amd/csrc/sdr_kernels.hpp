@@ -19,18 +19,23 @@ static_assert(sizeof(FixedSeg) == 24, "FixedSeg layout");
 // Rows of one RLE slice (one 256-thread block decodes a slice).
 constexpr uint32_t RLE_SLICE_ROWS = 16384;
 
-// Launch the decode of segments [seg_base, seg_base + n) of `d_segs` (device
-// array).  `max_rows` is the largest row_count among them (sizes the grid).
-hipError_t launch_bitpacking(const FixedSeg* d_segs, uint32_t seg_base, uint32_t n, uint32_t max_rows,
-                             uint32_t type_size, hipStream_t s);
-hipError_t launch_rle(const FixedSeg* d_segs, uint32_t seg_base, uint32_t n, uint32_t max_rows,
-                      uint32_t type_size, hipStream_t s);
-hipError_t launch_alp(const FixedSeg* d_segs, uint32_t seg_base, uint32_t n, uint32_t max_rows,
-                      uint32_t type_size, bool rd, hipStream_t s);
-hipError_t launch_uncompressed(const FixedSeg* d_segs, uint32_t seg_base, uint32_t n, uint32_t max_rows,
-                               uint32_t type_size, hipStream_t s);
-hipError_t launch_constant(const FixedSeg* d_segs, uint32_t seg_base, uint32_t n, uint32_t max_rows,
-                           uint32_t type_size, hipStream_t s);
+// Descriptors a launch can carry in its kernel arguments.
+constexpr uint32_t SEGS_INLINE = 128;
+
+// Launch the decode of segments [seg_base, seg_base + n) of `segs`: a device
+// array, or with `host` a host array whose descriptors travel in the kernel
+// arguments (SEGS_INLINE per launch, no upload).  `max_rows` is the largest
+// row_count among them (sizes the grid).
+hipError_t launch_bitpacking(const FixedSeg* segs, uint32_t seg_base, uint32_t n, uint32_t max_rows,
+                             uint32_t type_size, hipStream_t s, bool host = false);
+hipError_t launch_rle(const FixedSeg* segs, uint32_t seg_base, uint32_t n, uint32_t max_rows, uint32_t type_size,
+                      hipStream_t s, bool host = false);
+hipError_t launch_alp(const FixedSeg* segs, uint32_t seg_base, uint32_t n, uint32_t max_rows, uint32_t type_size,
+                      bool rd, hipStream_t s, bool host = false);
+hipError_t launch_uncompressed(const FixedSeg* segs, uint32_t seg_base, uint32_t n, uint32_t max_rows,
+                               uint32_t type_size, hipStream_t s, bool host = false);
+hipError_t launch_constant(const FixedSeg* segs, uint32_t seg_base, uint32_t n, uint32_t max_rows,
+                           uint32_t type_size, hipStream_t s, bool host = false);
 
 // Row ids: out[r.out_offset + i] = r.first_row + i for each range r (i < r.count).
 struct RowRange {

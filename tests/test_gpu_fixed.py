@@ -311,3 +311,24 @@ def test_alp_wide_codes_staged_in_chunks(dtype, bits):
     for mode in (A.LAUNCH_BATCHED, A.LAUNCH_PERSISTENT):
         got = decode_gpu(enc, mode)
         np.testing.assert_array_equal(got, O.decode_encoded(enc))
+
+
+@pytest.mark.parametrize("n_segs", [1, 100, 128, 129, 256, 257, 300])
+@pytest.mark.parametrize("codec", ["bp", "rle", "alp", "unc"])
+def test_descriptors_in_kernel_arguments_and_on_the_device(codec, n_segs):
+    """A group of up to 256 segments travels in the kernel arguments (128
+    per launch), a larger one through the uploaded descriptor array; both
+    decode the same column bit-exact, batched and one launch per segment."""
+    rows = n_segs * 2048
+    if codec == "bp":
+        enc = E.encode_bitpacking(E.gen_bp_int32(0xD0 + n_segs, rows), rg_rows=2048)
+    elif codec == "rle":
+        enc = E.encode_rle(E.gen_runs_int64(0xD1 + n_segs, rows, 16.0, 1 << 40), rg_rows=2048)
+    elif codec == "alp":
+        enc = E.encode_alp(E.gen_alp_doubles(0xD2 + n_segs, rows), rg_rows=2048)
+    else:
+        enc = E.encode_uncompressed(E.gen_u64(0xD3 + n_segs, rows).view(np.int64), rg_rows=2048)
+    assert len(enc.segs) == n_segs
+    want = O.decode_encoded(enc, threads=8)
+    for mode in (A.LAUNCH_BATCHED, A.LAUNCH_PER_SEGMENT):
+        np.testing.assert_array_equal(decode_gpu(enc, mode), want, err_msg=f"mode {mode}")
