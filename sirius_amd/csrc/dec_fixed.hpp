@@ -74,7 +74,7 @@ constexpr uint32_t BP_GROUP = 2048;
 // `lds` holds STAGE_B bytes of packed bits (plus the head and guard words);
 // groups wider than that are staged in power-of-two row chunks.
 template <typename T, uint32_t STAGE_B = STAGE_BYTES>
-__device__ void bp_group(const uint8_t* seg, uint32_t bytes, uint32_t k, uint32_t grc, T* out,
+__device__ __forceinline__ void bp_group(const uint8_t* seg, uint32_t bytes, uint32_t k, uint32_t grc, T* out,
                          uint32_t* lds, uint32_t max_chunk = BP_GROUP)
 {
   constexpr uint32_t TS  = sizeof(T);
@@ -254,7 +254,7 @@ __device__ __forceinline__ void patch_exceptions(uint32_t ec, uint32_t fill, con
 }
 
 template <typename T>
-__device__ void alp_vector(const uint8_t* seg, uint32_t bytes, uint32_t v, uint32_t fill, T* out,
+__device__ __forceinline__ void alp_vector(const uint8_t* seg, uint32_t bytes, uint32_t v, uint32_t fill, T* out,
                            uint32_t* lds)
 {
   constexpr uint32_t TS  = sizeof(T);
@@ -351,7 +351,7 @@ __device__ void alp_vector(const uint8_t* seg, uint32_t bytes, uint32_t v, uint3
 }
 
 template <typename T>
-__device__ void alprd_vector(const uint8_t* seg, uint32_t bytes, uint32_t v, uint32_t fill, T* out,
+__device__ __forceinline__ void alprd_vector(const uint8_t* seg, uint32_t bytes, uint32_t v, uint32_t fill, T* out,
                              uint32_t* lds)
 {
   constexpr uint32_t TS    = sizeof(T);

@@ -313,7 +313,7 @@ __device__ __forceinline__ void rle_expand(const uint8_t* values, uint32_t t0, u
 // (1, 2, 4 or 8) into out[0, rows).  Called by all 256 threads of a block
 // (the control flow is block-uniform).
 template <uint32_t TS>
-__device__ void rle_slice_t(const uint8_t* seg, uint32_t bytes, uint32_t rows, uint32_t ra, uint32_t rb, uint8_t* out,
+__device__ __forceinline__ void rle_slice_t(const uint8_t* seg, uint32_t bytes, uint32_t rows, uint32_t ra, uint32_t rb, uint8_t* out,
                             RleSmem& sm)
 {
   bool malformed        = true;

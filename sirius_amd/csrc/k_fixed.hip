@@ -15,14 +15,32 @@ namespace sdr {
 // kernel arguments themselves (at most SEGS_INLINE descriptors, 3 KiB of the
 // 4 KiB kernel-argument segment), which spares a small call the descriptor
 // upload.  blockIdx-indexed, so the load is uniform per workgroup.
+// A descriptor read at a workgroup-uniform index, its fields made uniform
+// (scalar registers), so the decoders' address arithmetic on them stays
+// scalar as with a restrict kernel-argument pointer.
+__device__ __forceinline__ FixedSeg uniform_seg(const FixedSeg& v)
+{
+  const uint64_t b = reinterpret_cast<uint64_t>(v.bytes), o = reinterpret_cast<uint64_t>(v.out);
+  FixedSeg r;
+  r.bytes = reinterpret_cast<const uint8_t*>(
+    (uint64_t{static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(b >> 32)))} << 32) |
+    static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(b))));
+  r.out = reinterpret_cast<uint8_t*>(
+    (uint64_t{static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(o >> 32)))} << 32) |
+    static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(o))));
+  r.bytes_size = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(v.bytes_size)));
+  r.row_count  = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(v.row_count)));
+  return r;
+}
+
 struct DevSegs {
   const FixedSeg* p;
   uint32_t base;
-  __device__ __forceinline__ FixedSeg at(uint32_t i) const { return p[base + i]; }
+  __device__ __forceinline__ FixedSeg at(uint32_t i) const { return uniform_seg(gload(p + base + i)); }
 };
 struct SegsInline {
   FixedSeg seg[SEGS_INLINE];
-  __device__ __forceinline__ FixedSeg at(uint32_t i) const { return seg[i]; }
+  __device__ __forceinline__ FixedSeg at(uint32_t i) const { return uniform_seg(seg[i]); }
 };
 
 // Wave w of block x decodes groups [(4x + w) * gpw, (4x + w + 1) * gpw).
