@@ -73,7 +73,15 @@ typedef enum sdr_codec {
  * `bytes_size` is an upper bound on the segment's bytes (it may include block
  * slack); decoders bound every read by it.  `max_string_length` is the
  * segment-statistics bound on decoded chars per row (strings only; 0 means
- * unknown). */
+ * unknown).
+ *
+ * The reference's string descriptor also carries `seg_row_start`, rows
+ * skipped at the head of a segment that a caller slices into chunks.  Its
+ * one production caller sets it to 0 (duckdb_native_decoder.cpp:914), and
+ * only the DICT_FSST and UNCOMPRESSED kernels read it.  This ABI has no such
+ * field: the decoders split segments into chunks themselves, and a caller
+ * that wants rows [k, n) of a segment decodes the whole segment and takes
+ * the rows from the output (offsets rebased by the chars before row k). */
 typedef struct sdr_segment {
   const uint8_t* d_bytes;
   uint32_t bytes_size;

@@ -1,6 +1,7 @@
 """GPU parity of the fixed-width path (BITPACKING, RLE, ALP, ALPRD,
 UNCOMPRESSED, CONSTANT) through the C ABI, bit-exact against the oracle."""
 import ctypes as C
+import functools
 
 import numpy as np
 import pytest
@@ -294,6 +295,24 @@ def test_config2_columns_full_size():
         enc = f(v)
         got = decode_gpu(enc)
         assert np.array_equal(got, v.view(np.uint8))
+
+
+@pytest.mark.parametrize("mode", [A.LAUNCH_BATCHED, A.LAUNCH_PERSISTENT, A.LAUNCH_PER_SEGMENT],
+                         ids=["batched", "persistent", "per_segment"])
+def test_config3_alp_full_size(mode):
+    """32 Mi ALP doubles and 8 Mi ALP-RD doubles, the benchmark's seeds:
+    every bit pattern equals the generated value's (-0.0, NaN payloads and
+    exceptions included)."""
+    for v, enc in _config3_columns():
+        got = decode_gpu(enc, mode)
+        assert np.array_equal(got.view(np.uint64), v.view(np.uint64))
+
+
+@functools.lru_cache(maxsize=1)
+def _config3_columns():
+    a = E.gen_alp_doubles(0xC0DE0003, 32 << 20)
+    r = E.gen_unit_doubles(0xC0DE0013, 8 << 20)
+    return ((a, E.encode_alp(a)), (r, E.encode_alprd(r)))
 
 
 @pytest.mark.parametrize("dtype,bits", [(np.float64, b) for b in (31, 32, 45, 48, 52)] +
