@@ -195,6 +195,17 @@ struct FsstSmemT {
   uint32_t wave[FSST_WAVES];
   uint32_t win;
   uint64_t base;
+
+  // The second walk's output window: out[] and cst[], which are adjacent.
+  // The first walk is the last reader of the staged compressed bytes; the
+  // second reads them from global memory (L2), so the window spans both.
+  static constexpr uint32_t WINDOW = (OUT_CAP > FSST_PIECE ? OUT_CAP : FSST_PIECE) + FSST_CSTAGE;
+  __device__ __forceinline__ uint8_t* window()
+  {
+    static_assert(offsetof(FsstSmemT, cst) == offsetof(FsstSmemT, out) + sizeof(out), "out[] and cst[] adjacent");
+    static_assert(WINDOW + 15u + 35u <= sizeof(out) + sizeof(cst), "a window's clear stays inside out[] and cst[]");
+    return reinterpret_cast<uint8_t*>(this) + offsetof(FsstSmemT, out);
+  }
 };
 
 // Walk the n compressed bytes of one row with DuckDB's serial semantics:
@@ -456,7 +467,7 @@ __device__ __forceinline__ void emit_byte_slow(SM& sm, uint32_t y, uint32_t b, b
   if (valid) {
     const uint32_t sh = 8u * (o & 3u);
     const uint64_t w  = v << sh;
-    uint32_t* d       = reinterpret_cast<uint32_t*>(&sm.out[o & ~3u]);
+    uint32_t* d       = reinterpret_cast<uint32_t*>(&sm.window()[o & ~3u]);
     __hip_atomic_fetch_or(d, static_cast<uint32_t>(w), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     __hip_atomic_fetch_or(d + 1, static_cast<uint32_t>(w >> 32), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     __hip_atomic_fetch_or(d + 2, static_cast<uint32_t>((uint64_t{static_cast<uint32_t>(v >> 32)} << sh) >> 32),
@@ -480,10 +491,20 @@ constexpr uint32_t EMIT_AHEAD = 4;  // row starts a lane prefetches per 16-byte 
 // flag (empty rows start where the next row starts; the last match wins).
 // A lane whose group holds more row starts than that takes the group byte
 // by byte.
-template <uint32_t OUT_CAP, class SM>
+//
+// The window is SM::WINDOW bytes of LDS (the output buffer and the
+// compressed stage), and the compressed bytes come from global memory at
+// `stage`, the address of stage position 0 (the same 16-byte blocks the
+// stage holds, read again from L2).  A lane loads its first group while the
+// window is cleared, and each next group while it emits the current one.
+template <class SM>
 __device__ __forceinline__ bool fsst_emit_staged(SM& sm, uint32_t rows, uint32_t c_h, uint32_t c_hi, uint32_t T,
-                                                 uint64_t base, uint8_t* chars, uint64_t chars_cap)
+                                                 uint64_t base, uint8_t* chars, uint64_t chars_cap,
+                                                 const uint8_t* stage)
 {
+  constexpr uint32_t OUT_CAP = SM::WINDOW;
+  uint8_t* const wb         = sm.window();
+  const gptr<const u32x4> cg = gbl(reinterpret_cast<const u32x4*>(stage));
   const uint32_t tid  = threadIdx.x;
   const uint32_t lane = lane_id();
   const uint32_t top  = c_h + c_hi;
@@ -526,13 +547,6 @@ __device__ __forceinline__ bool fsst_emit_staged(SM& sm, uint32_t rows, uint32_t
       const uintptr_t pa  = g_lo - pad;
       const uint32_t nout = pad + (whi - wlo);
       const uint32_t wofs = wlo - pad;  // chunk output position of out[0]
-      // Cleared with 4-byte stores.  Measured on gfx950: clearing with
-      // 16-byte stores (ds_write_b128) and then ORing from other waves
-      // after a barrier lost about 0.2% of the ORed bytes once five or more
-      // workgroups shared a CU; 4-byte stores never did.
-      for (uint32_t i = tid * 4u; i < nout + 32u; i += FSST_THREADS * 4u)
-        *reinterpret_cast<uint32_t*>(&sm.out[i]) = 0u;
-      __syncthreads();
       const uint32_t ya   = top - sm.comp[r_hi + 1];
       const uint32_t yb   = top - sm.comp[r_lo];
       const uint32_t g0   = ya & ~15u;
@@ -540,6 +554,15 @@ __device__ __forceinline__ bool fsst_emit_staged(SM& sm, uint32_t rows, uint32_t
       const uint32_t s0   = g0 + tid * span;
       const uint32_t ys   = tid ? s0 : ya;
       const uint32_t ye   = s0 + span < yb ? s0 + span : yb;
+      u32x4 qn            = u32x4{0u, 0u, 0u, 0u};
+      if (ys < ye) qn = cg[ys >> 4];
+      // Cleared with 4-byte stores.  Measured on gfx950: clearing with
+      // 16-byte stores (ds_write_b128) and then ORing from other waves
+      // after a barrier lost about 0.2% of the ORed bytes once five or more
+      // workgroups shared a CU; 4-byte stores never did.
+      for (uint32_t i = tid * 4u; i < nout + 32u; i += FSST_THREADS * 4u)
+        *reinterpret_cast<uint32_t*>(&wb[i]) = 0u;
+      __syncthreads();
       // bytes outside [ys, ye) OR into this word, which is never flushed
       const uint32_t trash = (nout + 19u) & ~3u;
       if (ys < ye) {
@@ -556,7 +579,8 @@ __device__ __forceinline__ bool fsst_emit_staged(SM& sm, uint32_t rows, uint32_t
         uint32_t rend = top - sm.comp[r];
 #pragma unroll 1
         for (uint32_t g = ys & ~15u; g < ye; g += 16) {
-          const u32x4 q = *reinterpret_cast<const u32x4*>(&sm.cst[g]);
+          const u32x4 q = qn;
+          if (g + 16u < ye) qn = cg[(g >> 4) + 1u];
           // the next EMIT_AHEAD rows below r: ends e[k] and output starts
           // d[k]; row r - k - 1 starts at e[k - 1] (e[-1] = rend)
           uint32_t e[EMIT_AHEAD], d[EMIT_AHEAD];
@@ -604,7 +628,7 @@ __device__ __forceinline__ bool fsst_emit_staged(SM& sm, uint32_t rows, uint32_t
               const uint32_t sh = 8u * (o & 3u);
               const uint64_t w  = v << sh;
               const uint32_t w2 = static_cast<uint32_t>((uint64_t{static_cast<uint32_t>(v >> 32)} << sh) >> 32);
-              uint32_t* dw      = reinterpret_cast<uint32_t*>(&sm.out[valid ? (o & ~3u) : trash]);
+              uint32_t* dw      = reinterpret_cast<uint32_t*>(&wb[valid ? (o & ~3u) : trash]);
               __hip_atomic_fetch_or(dw, static_cast<uint32_t>(w), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
               __hip_atomic_fetch_or(dw + 1, static_cast<uint32_t>(w >> 32), __ATOMIC_RELAXED,
                                     __HIP_MEMORY_SCOPE_WORKGROUP);
@@ -629,11 +653,11 @@ __device__ __forceinline__ bool fsst_emit_staged(SM& sm, uint32_t rows, uint32_t
       __syncthreads();
       for (uintptr_t blk = pa + uintptr_t{tid} * 16u; blk < g_hi; blk += uintptr_t{FSST_THREADS} * 16u) {
         if (blk >= g_lo && blk + 16u <= g_hi) {
-          const u32x4 v = *reinterpret_cast<const u32x4*>(&sm.out[blk - pa]);
+          const u32x4 v = *reinterpret_cast<const u32x4*>(&wb[blk - pa]);
           __builtin_nontemporal_store(v, gbl(reinterpret_cast<u32x4*>(blk)));
         } else {
           for (uintptr_t x = blk; x < blk + 16u; ++x)
-            if (x >= g_lo && x < g_hi) *gbl(reinterpret_cast<uint8_t*>(x)) = sm.out[x - pa];
+            if (x >= g_lo && x < g_hi) *gbl(reinterpret_cast<uint8_t*>(x)) = wb[x - pa];
         }
       }
     }
@@ -1342,7 +1366,7 @@ __device__ __forceinline__ void fsst_chunk(const FsstChunk ch, const FsstChunkIn
     const uint64_t base = fsst_lookback(c, n_chunks, T, st, spin_limit, sm);
     fsst_offsets(ch, c, n_chunks, rows, T, base, offsets, total_rows, sm);
     if (T == 0 || base >= chars_cap) return;
-    if (fsst_emit_staged<OUT_CAP>(sm, rows, c_h, c_hi, T, base, chars, chars_cap)) return;
+    if (fsst_emit_staged(sm, rows, c_h, c_hi, T, base, chars, chars_cap, seg + dict_end - c_hi - c_h)) return;
     fsst_emit_rows(sm, seg, dict_end, rows, T, base, chars, chars_cap);
   } else {
     const uint32_t T    = fsst_lengths_rows(sm, ok, seg, dict_end, rows);
