@@ -213,10 +213,13 @@ int sdr_plan_destroy(sdr_plan* plan);
 /* ---- persistent batches ------------------------------------------------- */
 
 /* A batch collects the work of many columns (fixed-width and VARCHAR, any
- * codec mix) into one device-resident work queue that a single persistent
- * kernel drains; the queue is uploaded from pinned memory with one copy and
- * the launch is replayed from a hipGraph.  It has no counterpart in the
- * reference, which launches one kernel per codec run
+ * codec mix) into one launch: the fixed-width segments of each codec and
+ * value width decode in one per-codec grid over descriptors in the batch's
+ * arena (or, with SDR_BATCH_FIXED_KERNELS = 1, in persistent group kernels
+ * that drain a work queue), and the VARCHAR chunks in a persistent kernel
+ * that claims them from the queue.  The arena is uploaded from pinned memory
+ * with one copy and the launch is replayed from a hipGraph.  It has no
+ * counterpart in the reference, which launches one kernel per codec run
  * (src/cuda/scan/gpu_native_decode.cu:356-386, gpu_decode_strings.cu:257-366);
  * every output is identical to sdr_decode_table / sdr_decode_strings_column.
  * Columns, segment bytes and outputs must stay valid until sdr_batch_finish.
@@ -227,6 +230,12 @@ typedef struct sdr_batch sdr_batch;
 typedef enum sdr_batch_option {
   SDR_BATCH_USE_GRAPH          = 1,  /* 1 (default): replay a captured hipGraph     */
   SDR_BATCH_GRID               = 2,  /* blocks; 0 (default): resident capacity     */
+  SDR_BATCH_FIXED_KERNELS      = 3,  /* fixed-width segments: 0 (default) the      */
+                                     /*   per-codec kernels, one grid per codec    */
+                                     /*   and value width over descriptors in the  */
+                                     /*   batch; 1 persistent group kernels that   */
+                                     /*   drain queue entries (SDR_BATCH_UNITS_*,  */
+                                     /*   SDR_BATCH_GRID apply)                    */
   SDR_BATCH_UNITS_BITPACKING   = 16, /* work units per queue entry, per codec:     */
   SDR_BATCH_UNITS_RLE          = 17, /*   2048-row groups (default 16), RLE slices */
   SDR_BATCH_UNITS_ALP          = 18, /*   of rle_slice_rows (2), 1024-row vectors  */

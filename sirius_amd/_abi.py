@@ -42,6 +42,7 @@ LAUNCH_PERSISTENT = 2
 
 BATCH_USE_GRAPH = 1
 BATCH_GRID = 2
+BATCH_FIXED_KERNELS = 3  # 0: per-codec grids (default), 1: persistent group kernels
 BATCH_UNITS = {BITPACKING: 16, RLE: 17, ALP: 18, ALPRD: 19, UNCOMPRESSED: 20, CONSTANT: 21}
 
 # sdr_tune_key

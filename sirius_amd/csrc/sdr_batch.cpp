@@ -6,6 +6,7 @@
 // branches.
 #include "sdr_batch.hpp"
 #include "sdr_internal.hpp"
+#include "sdr_kernels.hpp"
 #include "sdr_tune.hpp"
 
 #include <algorithm>
@@ -58,6 +59,14 @@ struct sdr_batch {
   bool units_set[8]    = {};
   uint32_t grid        = 0;                         // 0 = resident capacity
   bool use_graph       = true;
+  // fixed-width segments: per-codec grids over descriptors in the arena
+  // (default), or persistent group kernels draining queue entries
+  bool fixed_grids     = true;
+  size_t off_fsegs     = 0;
+  std::vector<sdr::FixedLaunch> fixed;              // the per-codec grids of the last upload
+  std::vector<sdr::FixedLaunch> graph_fixed;        // ... that the recorded graph launches
+  size_t graph_fsegs   = 0;                         // ... and where their descriptors were
+  hipStream_t capture  = nullptr;                   // records the graph of a launch with grids
   hipGraphExec_t gexec = nullptr;
   hipGraph_t graph     = nullptr;
   uint8_t* graph_arena = nullptr;
@@ -81,6 +90,7 @@ sdr_batch::~sdr_batch()
   }
   if (gexec) (void)hipGraphExecDestroy(gexec);
   if (graph) (void)hipGraphDestroy(graph);
+  if (capture) (void)hipStreamDestroy(capture);
   if (d_arena) (void)hipFree(d_arena);
   if (h_stage) (void)hipHostFree(h_stage);
   if (h_readback) (void)hipHostFree(h_readback);
@@ -263,6 +273,15 @@ int batch_plan(sdr_batch* b, uint32_t per[BG_COUNT], uint32_t entries[BG_COUNT],
     tot[g] += b->items[i].n_units;
     maxu[g] = std::max<uint32_t>(1, batch_units(b, b->items[i].kind, b->items[i].ts));
   }
+  if (b->fixed_grids) {  // fixed-width groups become per-codec grids: one work entry per segment
+    for (uint32_t g = 0; g < BG_COUNT; ++g) {
+      entries[g] = 0;
+      per[g]     = 1;
+    }
+    for (size_t i = 0; i < b->items.size(); ++i) ++entries[b->item_group[i]];
+    *mask &= 1u << BG_FSST;
+    return batch_group_caps(b, *mask);
+  }
   int rc = batch_group_caps(b, *mask);
   if (rc) return rc;
   for (uint32_t g = 0; g < BG_COUNT; ++g) {
@@ -290,10 +309,37 @@ int batch_upload(sdr_batch* b, hipStream_t s)
   if (prc) return prc;
   size_t n_entries = 0;
   for (uint32_t g = 0; g < BG_COUNT; ++g) {
+    if (!(mask >> g & 1u)) entries[g] = 0;  // no queue entries: per-codec grids (or nothing)
     n_entries += entries[g];
     b->entry_count[g] = entries[g];
   }
   b->n_entries = n_entries;
+  // the per-codec grids' descriptors, by codec kind and value width, in the
+  // order the items were added
+  std::vector<FixedSeg> fsegs;
+  b->fixed.clear();
+  if (b->fixed_grids) {
+    // value widths 1, 2, 4, 8 and 16 (UNCOMPRESSED and CONSTANT)
+    uint32_t count[8][5] = {};
+    auto wi = [](uint32_t ts) { return ts == 1 ? 0u : ts == 2 ? 1u : ts == 4 ? 2u : ts == 8 ? 3u : 4u; };
+    for (size_t i = 0; i < b->items.size(); ++i)
+      if (b->item_group[i] != BG_FSST) ++count[b->items[i].kind][wi(b->items[i].ts)];
+    uint32_t at_k[8][5] = {}, base = 0;
+    for (uint32_t k = 0; k < 8; ++k)
+      for (uint32_t w = 0; w < 5; ++w) {
+        at_k[k][w] = base;
+        if (count[k][w]) b->fixed.push_back({k, 1u << w, base, count[k][w], 0u});
+        base += count[k][w];
+      }
+    fsegs.resize(base);
+    for (size_t i = 0; i < b->items.size(); ++i) {
+      if (b->item_group[i] == BG_FSST) continue;
+      const BatchItem& it = b->items[i];
+      fsegs[at_k[it.kind][wi(it.ts)]++] = FixedSeg{it.seg, it.out, it.bytes_size, it.row_count};
+    }
+    for (auto& fl : b->fixed)
+      for (uint32_t j = 0; j < fl.n; ++j) fl.max_rows = std::max(fl.max_rows, fsegs[fl.base + j].row_count);
+  }
   auto al = [](size_t v) { return (v + 255u) & ~size_t{255}; };
   size_t n_chunks = 0, n_segs = 0;
   for (auto& c : b->cols) {
@@ -306,6 +352,8 @@ int batch_upload(sdr_batch* b, hipStream_t s)
   size_t off                  = al(sizeof(BatchQueue));
   b->off_items                = off;
   off                         = al(off + n_entries * sizeof(BatchItem));
+  b->off_fsegs                = off;
+  off                         = al(off + fsegs.size() * sizeof(FixedSeg));
   b->off_cols                 = off;
   off                         = al(off + b->cols.size() * sizeof(FsstColumn));
   b->off_segs                 = off;
@@ -397,9 +445,11 @@ int batch_upload(sdr_batch* b, hipStream_t s)
     st += 2 + c.n_chunks;
   }
   if (!dcols.empty()) std::memcpy(h + b->off_cols, dcols.data(), dcols.size() * sizeof(FsstColumn));
+  if (!fsegs.empty()) std::memcpy(h + b->off_fsegs, fsegs.data(), fsegs.size() * sizeof(FixedSeg));
   auto* hi = reinterpret_cast<BatchItem*>(h + b->off_items);
   for (size_t i = 0; i < b->items.size(); ++i) {
     const uint32_t g  = b->item_group[i];
+    if (!(mask >> g & 1u)) continue;  // a per-codec grid's segment
     const BatchItem& it = b->items[i];
     if (g == BG_FSST) {
       BatchItem& dst = hi[at[g]++];
@@ -419,6 +469,41 @@ int batch_upload(sdr_batch* b, hipStream_t s)
   HIP_TRY(hipEventRecord(b->upload_done, s));
   b->upload_pending = true;
   return SDR_OK;
+}
+
+// The per-codec grids of the batch's fixed-width segments (descriptors in
+// the arena), one launch per codec kind and value width.
+hipError_t launch_fixed_grids(const sdr_batch* b, hipStream_t s)
+{
+  const auto* segs = reinterpret_cast<const FixedSeg*>(b->d_arena + b->off_fsegs);
+  for (const auto& fl : b->fixed) {
+    hipError_t e = hipSuccess;
+    switch (fl.kind) {
+      case BK_BITPACKING: e = launch_bitpacking(segs, fl.base, fl.n, fl.max_rows, fl.ts, s); break;
+      case BK_RLE: e = launch_rle(segs, fl.base, fl.n, fl.max_rows, fl.ts, s); break;
+      case BK_ALP: e = launch_alp(segs, fl.base, fl.n, fl.max_rows, fl.ts, false, s); break;
+      case BK_ALPRD: e = launch_alp(segs, fl.base, fl.n, fl.max_rows, fl.ts, true, s); break;
+      case BK_COPY: e = launch_uncompressed(segs, fl.base, fl.n, fl.max_rows, fl.ts, s); break;
+      default: e = launch_constant(segs, fl.base, fl.n, fl.max_rows, fl.ts, s); break;
+    }
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
+// Every kernel of a launch in stream order: the reset (when the batch holds
+// string columns or queue entries), the FSST pre-pass, the group kernels,
+// then the per-codec grids.
+hipError_t enqueue_launch(const sdr_batch* b, BatchQueue* q, const BatchItem* items, uint32_t mask,
+                          const uint32_t* grid, const uint32_t* cnt, hipStream_t s)
+{
+  hipError_t e = hipSuccess;
+  if (mask || !b->cols.empty()) e = launch_batch_reset(q, s);
+  if (e == hipSuccess && (mask & (1u << BG_FSST))) e = launch_batch_fsst_prep(q, s);
+  for (uint32_t g = 0; e == hipSuccess && g < BG_COUNT; ++g)
+    if (mask >> g & 1u) e = launch_batch_group(items, q, g, std::min(grid[g], cnt[g]), s);
+  if (e == hipSuccess) e = launch_fixed_grids(b, s);
+  return e;
 }
 
 int batch_launch(sdr_batch* b, hipStream_t s)
@@ -443,12 +528,38 @@ int batch_launch(sdr_batch* b, hipStream_t s)
     grid[g] = std::max<uint32_t>(1, b->grid ? std::min(b->grid, b->group_cap[g]) : b->group_cap[g]);
   }
   const bool strings = mask & (1u << BG_FSST);
-  if (b->use_graph && mask) {
+  if (b->use_graph && !b->fixed.empty()) {
+    // with per-codec grids the graph is recorded from the stream order (on
+    // the batch's own stream), and recorded again when the grids change
+    if (!b->gexec || b->graph_arena != b->d_arena || b->graph_mask != mask || b->graph_gridopt != b->grid ||
+        !(b->graph_fixed == b->fixed) || b->graph_fsegs != b->off_fsegs) {
+      if (b->gexec) HIP_TRY(hipGraphExecDestroy(b->gexec));
+      if (b->graph) HIP_TRY(hipGraphDestroy(b->graph));
+      b->gexec = nullptr;
+      b->graph = nullptr;
+      if (!b->capture) HIP_TRY(hipStreamCreateWithFlags(&b->capture, hipStreamNonBlocking));
+      HIP_TRY(hipStreamBeginCapture(b->capture, hipStreamCaptureModeThreadLocal));
+      const hipError_t e = enqueue_launch(b, q, items, mask, grid, cnt, b->capture);
+      hipGraph_t gr      = nullptr;
+      const hipError_t ee = hipStreamEndCapture(b->capture, &gr);
+      HIP_TRY(e);
+      HIP_TRY(ee);
+      b->graph = gr;
+      HIP_TRY(hipGraphInstantiate(&b->gexec, gr, nullptr, nullptr, 0));
+      b->graph_arena   = b->d_arena;
+      b->graph_mask    = mask;
+      b->graph_gridopt = b->grid;
+      b->graph_fixed   = b->fixed;
+      b->graph_fsegs   = b->off_fsegs;
+    }
+    HIP_TRY(hipGraphLaunch(b->gexec, s));
+  } else if (b->use_graph && mask) {
     // nodes: reset -> [FSST pre-pass] -> one persistent kernel per group, the
     // group kernels parallel branches; node arguments are the arena
     // addresses, entries change through the upload above, so one
     // instantiation replays every batch with the same groups
-    if (!b->gexec || b->graph_arena != b->d_arena || b->graph_mask != mask || b->graph_gridopt != b->grid) {
+    if (!b->gexec || b->graph_arena != b->d_arena || b->graph_mask != mask || b->graph_gridopt != b->grid ||
+        !b->graph_fixed.empty()) {
       if (b->gexec) HIP_TRY(hipGraphExecDestroy(b->gexec));
       if (b->graph) HIP_TRY(hipGraphDestroy(b->graph));
       b->gexec = nullptr;
@@ -492,13 +603,11 @@ int batch_launch(sdr_batch* b, hipStream_t s)
       b->graph_arena   = b->d_arena;
       b->graph_mask    = mask;
       b->graph_gridopt = b->grid;
+      b->graph_fixed.clear();
     }
     HIP_TRY(hipGraphLaunch(b->gexec, s));
   } else {  // also a batch with no entries: its columns' look-back state is cleared
-    HIP_TRY(launch_batch_reset(q, s));
-    if (strings) HIP_TRY(launch_batch_fsst_prep(q, s));
-    for (uint32_t g = 0; g < BG_COUNT; ++g)
-      if (mask >> g & 1u) HIP_TRY(launch_batch_group(items, q, g, std::min(grid[g], cnt[g]), s));
+    HIP_TRY(enqueue_launch(b, q, items, mask, grid, cnt, s));
   }
   // a column of no rows has no chunk to write its closing offset
   for (const auto& c : b->cols)
@@ -654,6 +763,9 @@ int sdr_batch_set_option(sdr_batch* b, uint32_t key, int64_t value)
     b->grid = static_cast<uint32_t>(value);
   } else if (key == SDR_BATCH_USE_GRAPH) {
     b->use_graph = value != 0;
+  } else if (key == SDR_BATCH_FIXED_KERNELS) {
+    if (value < 0 || value > 1) return fail(SDR_E_INVALID_ARG, "fixed-width kernel choice out of range");
+    b->fixed_grids = value == 0;
   } else {
     return fail(SDR_E_INVALID_ARG, "unknown batch option %u", key);
   }

@@ -100,6 +100,17 @@ constexpr uint32_t BATCH_WAVES         = 4;    // wavefronts of a fixed-width gr
 constexpr uint32_t BATCH_XCDS          = 8;    // XCDs workgroups are dispatched to round-robin;
                                                // fixed-width group grids are a multiple
 
+// One per-codec grid of a batch's fixed-width segments: the segments of
+// codec kind `kind` and value width `ts`, descriptors [base, base + n) of the
+// batch's FixedSeg array, the largest of them max_rows rows.
+struct FixedLaunch {
+  uint32_t kind, ts, base, n, max_rows;
+  bool operator==(const FixedLaunch& o) const
+  {
+    return kind == o.kind && ts == o.ts && base == o.base && n == o.n && max_rows == o.max_rows;
+  }
+};
+
 hipError_t launch_batch_reset(BatchQueue* q, hipStream_t s);
 // FSST segment pre-pass over the queue's segments.
 hipError_t launch_batch_fsst_prep(BatchQueue* q, hipStream_t s);

@@ -1,7 +1,9 @@
 """The persistent mixed-codec path (sdr_batch, SDR_LAUNCH_PERSISTENT):
 identical outputs to the oracle for mixed fixed-width and FSST columns,
-with and without hipGraph replay, for several entry granularities, and on
-repeated replays of one instantiated graph with new queues."""
+with and without hipGraph replay, with the fixed-width segments in per-codec
+grids (the default) and in persistent group kernels at several entry
+granularities, and on repeated replays of one instantiated graph with new
+queues."""
 import ctypes as C
 
 import numpy as np
@@ -84,14 +86,16 @@ def work():
 
 
 @pytest.mark.parametrize("graph", [1, 0], ids=["graph", "nograph"])
-@pytest.mark.parametrize("units", [1, 4, 16])
-def test_batch_mixed_codecs(work, graph, units):
+@pytest.mark.parametrize("fixed,units", [(0, None), (1, 1), (1, 4), (1, 16)],
+                         ids=["grids", "groups-units1", "groups-units4", "groups-units16"])
+def test_batch_mixed_codecs(work, graph, fixed, units):
     lib = L.load()
     b = C.c_void_p()
     L.check(lib.sdr_batch_create(C.byref(b)))
     try:
         L.check(lib.sdr_batch_set_option(b, A.BATCH_USE_GRAPH, graph))
-        for codec, key in A.BATCH_UNITS.items():
+        L.check(lib.sdr_batch_set_option(b, A.BATCH_FIXED_KERNELS, fixed))
+        for codec, key in A.BATCH_UNITS.items() if units else ():
             L.check(lib.sdr_batch_set_option(b, key, units))
         work.wipe()
         L.check(lib.sdr_batch_add_table(b, work.arr, len(work.cols)))
@@ -104,15 +108,18 @@ def test_batch_mixed_codecs(work, graph, units):
         lib.sdr_batch_destroy(b)
 
 
-def test_graph_replays_with_new_queues(work):
-    """One graph instantiation, three replays with different queue contents,
-    eager work and synchronisation between replays; every replay checked."""
+@pytest.mark.parametrize("fixed", [0, 1], ids=["grids", "groups"])
+def test_graph_replays_with_new_queues(work, fixed):
+    """Graph replays with different queue contents (a new recording where
+    the launch shapes change), eager work and synchronisation between
+    replays; every replay checked."""
     lib = L.load()
     other = Workload(100_000, 200)
     b = C.c_void_p()
     L.check(lib.sdr_batch_create(C.byref(b)))
     try:
-        for w in (work, other, work):
+        L.check(lib.sdr_batch_set_option(b, A.BATCH_FIXED_KERNELS, fixed))
+        for w in (work, other, work, work):
             w.wipe()
             torch.cuda.synchronize()
             L.check(lib.sdr_batch_clear(b))
@@ -133,6 +140,16 @@ def test_persistent_launch_mode_entries(work):
     L.check(lib.sdr_decode_strings_column_mode(C.byref(work.scol), sh(), A.LAUNCH_PERSISTENT))
     torch.cuda.synchronize()
     work.check()
+
+
+def test_fixed_kernel_option_is_checked():
+    lib = L.load()
+    b = C.c_void_p()
+    L.check(lib.sdr_batch_create(C.byref(b)))
+    try:
+        assert lib.sdr_batch_set_option(b, A.BATCH_FIXED_KERNELS, 2) == A.SDR_E_INVALID_ARG
+    finally:
+        lib.sdr_batch_destroy(b)
 
 
 def test_batch_validity_null_counts():
