@@ -5,8 +5,9 @@ decoders specialise on (DuckDB-format segments, self-encoded), uploads them
 once, and for every tuning knob and width times the kernels the knob shapes
 under each candidate value: the per-codec kernels (HIP events around
 repeated launches of a prepared plan) for the launch-shape knobs, and the
-persistent batch (replayed launches of one batch) for the units-per-entry
-knobs.  The fastest value of each (knob, width) is kept before the next one
+persistent batch with its fixed-width segments in the group kernels
+(replayed launches of one batch, SDR_BATCH_FIXED_KERNELS = 1) for the
+units-per-entry knobs, which shape only those kernels.  The fastest value of each (knob, width) is kept before the next one
 is tried, and the result is written as a table that ``sdr_tune_load``
 applies: ``name@width value`` lines, timings as comments.  Knobs change
 launch shapes only, never results.
@@ -124,6 +125,7 @@ def main(argv=None) -> int:
         """ms per persistent batch launch over the columns `idx`."""
         b = C.c_void_p()
         L.check(lib.sdr_batch_create(C.byref(b)))
+        L.check(lib.sdr_batch_set_option(b, A.BATCH_FIXED_KERNELS, 1))
         arrs = [(A.Column * 1)(cols[ci]) for ci in idx]
 
         def launch():

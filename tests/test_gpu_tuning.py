@@ -2,7 +2,9 @@
 choose for every knob (set at every value width), and the shipped table
 sirius_amd/tuning/gfx950.txt, decode bit-exact.  Under each, columns of
 every codec and value width the decoders specialise on go through the
-per-codec kernels (sdr_decode_table) and through one persistent batch."""
+per-codec kernels (sdr_decode_table) and through one persistent batch, with
+its fixed-width segments in per-codec grids and in the group kernels that
+the units-per-entry knobs shape."""
 import ctypes as C
 
 import numpy as np
@@ -60,20 +62,22 @@ def test_decode_under_tuned_shape(cols, knob, value):
             L.check(lib.sdr_decode_table(arr, 1, sh))
             got = d_out.cpu().numpy()
             assert np.array_equal(got, want), f"{name}: batched path differs"
-        # one persistent batch over every column
-        made = [_column(enc, blob, dev) for _, enc, blob, _ in cols]
-        b = C.c_void_p()
-        L.check(lib.sdr_batch_create(C.byref(b)))
-        try:
-            arrs = [(A.Column * 1)(col) for col, _, _ in made]
-            for arr in arrs:
-                L.check(lib.sdr_batch_add_table(b, arr, 1))
-            L.check(lib.sdr_batch_launch(b, sh))
-            L.check(lib.sdr_batch_finish(b, sh))
-        finally:
-            lib.sdr_batch_destroy(b)
-        for (name, _, _, want), (_, d_out, _) in zip(cols, made):
-            assert np.array_equal(d_out.cpu().numpy(), want), f"{name}: persistent path differs"
+        # one persistent batch over every column, in both fixed-width forms
+        for fixed in (0, 1):
+            made = [_column(enc, blob, dev) for _, enc, blob, _ in cols]
+            b = C.c_void_p()
+            L.check(lib.sdr_batch_create(C.byref(b)))
+            try:
+                L.check(lib.sdr_batch_set_option(b, A.BATCH_FIXED_KERNELS, fixed))
+                arrs = [(A.Column * 1)(col) for col, _, _ in made]
+                for arr in arrs:
+                    L.check(lib.sdr_batch_add_table(b, arr, 1))
+                L.check(lib.sdr_batch_launch(b, sh))
+                L.check(lib.sdr_batch_finish(b, sh))
+            finally:
+                lib.sdr_batch_destroy(b)
+            for (name, _, _, want), (_, d_out, _) in zip(cols, made):
+                assert np.array_equal(d_out.cpu().numpy(), want), f"{name}: persistent path (fixed={fixed}) differs"
     finally:
         for (k, w), v in saved.items():
             L.check(lib.sdr_tune_set_width(k, w, v))
