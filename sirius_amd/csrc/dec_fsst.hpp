@@ -168,8 +168,13 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* s_wave
 constexpr uint32_t FSST_SYMSTAGE = 17u + 8u * FSST_NSYM;  // the largest symbol table (2057 bytes)
 constexpr uint32_t FSST_CSTAGE   = 16384;                  // chunk compressed bytes staged in LDS
 constexpr uint32_t FSST_NGRID    = (FSST_CSTAGE + 16u) / 16u + 2u;
-constexpr uint32_t FSST_OUT_COLUMN = 8192;   // output window of the per-column kernel (4 blocks per CU)
-constexpr uint32_t FSST_OUT_BATCH  = 8192;   // ... of the persistent batch kernel
+constexpr uint32_t FSST_OUT_COLUMN = 16384;  // output window of the per-column kernel (5 blocks per CU)
+constexpr uint32_t FSST_OUT_BATCH  = 16384;  // ... of the persistent batch kernel
+// 16-byte groups of a thread's first-walk span of a chunk's compressed bytes
+// (at most FSST_CSTAGE of them after a head of up to 15 bytes), loaded into
+// registers at the chunk's start
+constexpr uint32_t SPAN_GROUPS = (FSST_CSTAGE + 15u + 16u * FSST_THREADS - 1u) / (16u * FSST_THREADS);
+static_assert(SPAN_GROUPS == 5, "first-walk span registers");
 
 // Walk state at a grid point: decoded prefix (bits 0-19), escape flag (bit
 // 20), row (bits 21-31).  A staged chunk decodes to at most 8 * FSST_CSTAGE
@@ -190,21 +195,19 @@ struct FsstSmemT {
   uint32_t part[FSST_THREADS];         // decoded bytes before each thread's span
   // output window; before that, pieces of the compressed-length stream
   __attribute__((aligned(16))) uint8_t out[(OUT_CAP > FSST_PIECE ? OUT_CAP : FSST_PIECE) + 48];
-  // the symbol table, then the chunk's compressed bytes
-  __attribute__((aligned(16))) uint8_t cst[FSST_CSTAGE + 48];
   uint32_t wave[FSST_WAVES];
   uint32_t win;
   uint64_t base;
 
-  // The second walk's output window: out[] and cst[], which are adjacent.
-  // The first walk is the last reader of the staged compressed bytes; the
-  // second reads them from global memory (L2), so the window spans both.
-  static constexpr uint32_t WINDOW = (OUT_CAP > FSST_PIECE ? OUT_CAP : FSST_PIECE) + FSST_CSTAGE;
+  // The second walk's output window.  The chunk's compressed bytes are not
+  // staged in LDS: the first walk holds each thread's span in registers and
+  // the second reads the same 16-byte blocks again from L2.
+  static constexpr uint32_t WINDOW = OUT_CAP > FSST_PIECE ? OUT_CAP : FSST_PIECE;
   __device__ __forceinline__ uint8_t* window()
   {
-    static_assert(offsetof(FsstSmemT, cst) == offsetof(FsstSmemT, out) + sizeof(out), "out[] and cst[] adjacent");
-    static_assert(WINDOW + 15u + 35u <= sizeof(out) + sizeof(cst), "a window's clear stays inside out[] and cst[]");
-    return reinterpret_cast<uint8_t*>(this) + offsetof(FsstSmemT, out);
+    // a window of nout <= WINDOW + 15 bytes (16-byte head) is cleared in 4-byte words below nout + 32
+    static_assert((WINDOW + 15u + 31u) / 4u * 4u + 4u <= sizeof(out), "a window's clear stays inside out[]");
+    return out;
   }
 };
 
@@ -333,7 +336,7 @@ __device__ __attribute__((noinline)) void fsst_emit_rows(SM& sm, const uint8_t* 
 
 // ---- span path (staged chunks) ----
 //
-// Stage coordinates: the chunk's compressed bytes sit at cst[c_h, c_h + C);
+// Stage coordinates: the chunk's compressed bytes sit at stage positions [c_h, c_h + C);
 // row r occupies [top - comp[r + 1], top - comp[r]) with top = c_h + c_hi
 // (rows are stored back to front, so row indices fall as the position
 // rises).  A walk's row variable r is the row holding the byte it is about
@@ -355,9 +358,34 @@ __device__ __forceinline__ uint32_t group_byte(const u32x4& q, uint32_t j)
 // First walk: decoded lengths.  Records the walk state at each grid point
 // and, per row, the decoded prefix at its start; returns the chunk's decoded
 // total and leaves dec[] as the chunk-relative exclusive decoded offsets.
+// Thread t's first-walk span: the 16-byte groups [t * span, t * span + span)
+// of stage positions below y_hi, stage position 0 being the 16-byte-aligned
+// address `stage` (the blocks an LDS stage of the chunk would hold).
+__device__ __forceinline__ void fsst_span_load(u32x4 (&qv)[SPAN_GROUPS], const uint8_t* stage, uint32_t y_hi)
+{
+  const uint32_t span = span_bytes(0, y_hi);
+  const uint32_t s0   = threadIdx.x * span;
+  const gptr<const u32x4> cg = gbl(reinterpret_cast<const u32x4*>(stage));
+#pragma unroll
+  for (uint32_t k = 0; k < SPAN_GROUPS; ++k)
+    if (16u * k < span && s0 + 16u * k < y_hi) qv[k] = cg[(s0 >> 4) + k];
+}
+
+// The last word of thread t's span (the four bytes before thread t + 1's
+// first byte), for the escape parity of the thread after it.
+__device__ __forceinline__ uint32_t fsst_span_tail(const u32x4 (&qv)[SPAN_GROUPS], uint32_t y_hi)
+{
+  const uint32_t last = span_bytes(0, y_hi) / 16u - 1u;
+  uint32_t w          = 0;
+#pragma unroll
+  for (uint32_t k = 0; k < SPAN_GROUPS; ++k) w = k == last ? qv[k][3] : w;
+  return w;
+}
+
 template <class SM>
 __device__ __forceinline__ uint32_t fsst_lengths_staged(SM& sm, uint32_t rows, uint32_t c_h, uint32_t c_hi,
-                                                        uint32_t C)
+                                                        uint32_t C, const u32x4 (&qv)[SPAN_GROUPS],
+                                                        const uint8_t* stage)
 {
   const uint32_t tid  = threadIdx.x;
   const uint32_t y_hi = c_h + C;
@@ -378,12 +406,26 @@ __device__ __forceinline__ uint32_t fsst_lengths_staged(SM& sm, uint32_t rows, u
     uint32_t rnext = r ? top - sm.comp[r - 1] : rend;  // end of row r - 1
     bool esc       = false;
     if (r < rows) {  // ya inside row r: an odd run of 255s before it escapes it
+      // the four bytes before ya come from the previous thread's span (sm.part,
+      // written before the barrier that ends the compressed-prefix scan),
+      // earlier ones from L2
       const uint32_t rs = top - sm.comp[r + 1];
-      for (uint32_t y = ya; y > rs && sm.cst[y - 1] == FSST_ESC; --y) esc = !esc;
+      const uint32_t pw = tid ? sm.part[tid - 1] : 0u;
+      for (uint32_t y = ya; y > rs; --y) {
+        const uint32_t d = ya - y;
+        const uint32_t b = d < 4u ? (pw >> (24u - 8u * d)) & 0xFFu : ld_u8(stage + y - 1u);
+        if (b != FSST_ESC) break;
+        esc = !esc;
+      }
     }
+    u32x4 qr[SPAN_GROUPS];
+#pragma unroll
+    for (uint32_t k = 0; k < SPAN_GROUPS; ++k) qr[k] = qv[k];
 #pragma unroll 1
     for (uint32_t g = ya & ~15u; g < yb; g += 16) {
-      const u32x4 q = *reinterpret_cast<const u32x4*>(&sm.cst[g]);
+      const u32x4 q = qr[0];
+#pragma unroll
+      for (uint32_t k = 0; k + 1 < SPAN_GROUPS; ++k) qr[k] = qr[k + 1];
       uint32_t L[16];
 #pragma unroll
       for (uint32_t j = 0; j < 16; ++j) L[j] = sm.len[group_byte(q, j)];
@@ -1319,10 +1361,18 @@ __device__ __forceinline__ void fsst_chunk(const FsstChunk ch, const FsstChunkIn
     bit0 = h * 8u + static_cast<uint32_t>((uint64_t{ch.row_begin} * width) & 7u);
   }
   // Rows are stored back to front ending at dict_end: row r is
-  // [dict_end - comp[r+1], dict_end - comp[r]).
-  uint32_t c_h = 0;
-  if (staged)
-    c_h = block_issue<FSST_THREADS>(reinterpret_cast<uint32_t*>(sm.cst), seg + dict_end - c_hi, c_hi - c_lo);
+  // [dict_end - comp[r+1], dict_end - comp[r]).  Stage position 0 is the
+  // 16-byte-aligned address at or below the chunk's first compressed byte;
+  // each thread loads its first-walk span into registers.
+  uint32_t c_h          = 0;
+  const uint8_t* stage  = nullptr;
+  u32x4 qv[SPAN_GROUPS] = {};
+  if (staged) {
+    const uintptr_t a = reinterpret_cast<uintptr_t>(seg + dict_end - c_hi);
+    c_h               = static_cast<uint32_t>(a & 15u);
+    stage             = reinterpret_cast<const uint8_t*>(a - c_h);
+    fsst_span_load(qv, stage, c_h + (c_hi - c_lo));
+  }
   __builtin_amdgcn_s_waitcnt(WAITCNT_VMEM_DONE);
   __syncthreads();
 
@@ -1352,13 +1402,14 @@ __device__ __forceinline__ void fsst_chunk(const FsstChunk ch, const FsstChunkIn
       if (r < rows) sm.comp[r + 1] = run;
     }
   }
+  if (staged) sm.part[tid] = fsst_span_tail(qv, c_h + (c_hi - c_lo));
   __syncthreads();
 
   if (staged) {
     const uint32_t C = c_hi - c_lo;
     uint32_t T       = 0;
     if (C) {
-      T = fsst_lengths_staged(sm, rows, c_h, c_hi, C);
+      T = fsst_lengths_staged(sm, rows, c_h, c_hi, C, qv, stage);
     } else {
       for (uint32_t i = tid; i <= rows; i += FSST_THREADS) sm.dec[i] = 0;
       __syncthreads();
@@ -1366,7 +1417,7 @@ __device__ __forceinline__ void fsst_chunk(const FsstChunk ch, const FsstChunkIn
     const uint64_t base = fsst_lookback(c, n_chunks, T, st, spin_limit, sm);
     fsst_offsets(ch, c, n_chunks, rows, T, base, offsets, total_rows, sm);
     if (T == 0 || base >= chars_cap) return;
-    if (fsst_emit_staged(sm, rows, c_h, c_hi, T, base, chars, chars_cap, seg + dict_end - c_hi - c_h)) return;
+    if (fsst_emit_staged(sm, rows, c_h, c_hi, T, base, chars, chars_cap, stage)) return;
     fsst_emit_rows(sm, seg, dict_end, rows, T, base, chars, chars_cap);
   } else {
     const uint32_t T    = fsst_lengths_rows(sm, ok, seg, dict_end, rows);
