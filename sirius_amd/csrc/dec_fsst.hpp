@@ -534,14 +534,19 @@ constexpr uint32_t EMIT_AHEAD = 4;  // row starts a lane prefetches per 16-byte 
 // A lane whose group holds more row starts than that takes the group byte
 // by byte.
 //
-// The window is SM::WINDOW bytes of LDS (the output buffer and the
-// compressed stage), and the compressed bytes come from global memory at
-// `stage`, the address of stage position 0 (the same 16-byte blocks the
-// stage holds, read again from L2).  A lane loads its first group while the
-// window is cleared, and each next group while it emits the current one.
-template <class SM>
+// The window is SM::WINDOW bytes of LDS; byte i of a window is chunk output
+// position wlo + i, whatever the chunk's place in the column, so a window can
+// be built before the chunk's base is known.  `get_base()` returns the base;
+// every thread calls it after each window's ORs, and it synchronises the
+// workgroup (after the first window's ORs the look-back completes there).
+// The stores shift the window to the destination's 16-byte alignment.  The
+// compressed bytes come from global memory at `stage`, the address of stage
+// position 0 (16-byte blocks read again from L2).  A lane loads its first
+// group while the window is cleared, and each next group while it emits the
+// current one.
+template <class SM, class GetBase>
 __device__ __forceinline__ bool fsst_emit_staged(SM& sm, uint32_t rows, uint32_t c_h, uint32_t c_hi, uint32_t T,
-                                                 uint64_t base, uint8_t* chars, uint64_t chars_cap,
+                                                 GetBase&& get_base, uint8_t* chars, uint64_t chars_cap,
                                                  const uint8_t* stage)
 {
   constexpr uint32_t OUT_CAP = SM::WINDOW;
@@ -580,15 +585,11 @@ __device__ __forceinline__ bool fsst_emit_staged(SM& sm, uint32_t rows, uint32_t
     __syncthreads();
     const uint32_t r_lo = sm.win;
     if (r_lo == ~0u) return false;
-    const uint32_t wlo   = sm.dec[r_lo];
-    const uint32_t whi   = sm.dec[r_hi + 1];
-    const uintptr_t g_lo = cb + base + wlo;
-    const uintptr_t g_hi = cb + (base + whi < chars_cap ? base + whi : chars_cap);
-    if (g_lo < g_hi) {
-      const uint32_t pad  = static_cast<uint32_t>(g_lo & 15u);
-      const uintptr_t pa  = g_lo - pad;
-      const uint32_t nout = pad + (whi - wlo);
-      const uint32_t wofs = wlo - pad;  // chunk output position of out[0]
+    const uint32_t wlo  = sm.dec[r_lo];
+    const uint32_t whi  = sm.dec[r_hi + 1];
+    {
+      const uint32_t nout = whi - wlo;
+      const uint32_t wofs = wlo;  // chunk output position of window byte 0
       const uint32_t ya   = top - sm.comp[r_hi + 1];
       const uint32_t yb   = top - sm.comp[r_lo];
       const uint32_t g0   = ya & ~15u;
@@ -692,14 +693,23 @@ __device__ __forceinline__ bool fsst_emit_staged(SM& sm, uint32_t rows, uint32_t
           }
         }
       }
-      __syncthreads();
-      for (uintptr_t blk = pa + uintptr_t{tid} * 16u; blk < g_hi; blk += uintptr_t{FSST_THREADS} * 16u) {
+      const uint64_t base  = get_base();  // synchronises the workgroup
+      const uintptr_t g_lo = cb + base + wlo;
+      const uintptr_t g_hi = cb + (base + whi < chars_cap ? base + whi : chars_cap);
+      const uint32_t* ww   = reinterpret_cast<const uint32_t*>(wb);
+      for (uintptr_t blk = (g_lo & ~uintptr_t{15}) + uintptr_t{tid} * 16u; blk < g_hi;
+           blk += uintptr_t{FSST_THREADS} * 16u) {
         if (blk >= g_lo && blk + 16u <= g_hi) {
-          const u32x4 v = *reinterpret_cast<const u32x4*>(&wb[blk - pa]);
+          // window bytes [i, i + 16), i = blk - g_lo, from five aligned words
+          const uint32_t i  = static_cast<uint32_t>(blk - g_lo);
+          const uint32_t w  = i >> 2, sh = i & 3u;
+          const uint32_t x0 = ww[w], x1 = ww[w + 1], x2 = ww[w + 2], x3 = ww[w + 3], x4 = ww[w + 4];
+          const u32x4 v     = u32x4{__builtin_amdgcn_alignbyte(x1, x0, sh), __builtin_amdgcn_alignbyte(x2, x1, sh),
+                                __builtin_amdgcn_alignbyte(x3, x2, sh), __builtin_amdgcn_alignbyte(x4, x3, sh)};
           __builtin_nontemporal_store(v, gbl(reinterpret_cast<u32x4*>(blk)));
         } else {
           for (uintptr_t x = blk; x < blk + 16u; ++x)
-            if (x >= g_lo && x < g_hi) *gbl(reinterpret_cast<uint8_t*>(x)) = wb[x - pa];
+            if (x >= g_lo && x < g_hi) *gbl(reinterpret_cast<uint8_t*>(x)) = wb[x - g_lo];
         }
       }
     }
@@ -710,29 +720,40 @@ __device__ __forceinline__ bool fsst_emit_staged(SM& sm, uint32_t rows, uint32_t
 }
 
 // Decoupled look-back for the column-wide chars offset of chunk c, by wave
-// 0: publish this chunk's aggregate T, then sum predecessors 64 at a time
-// back to the nearest published inclusive prefix.  A chunk waits at most
-// `spin_limit` sleeps for an unpublished predecessor; past that (or at once
-// when spin_limit is 0, which only tests set) it sets the column's error
-// flag and ends, so the host reports the failure and no wave hangs.  A status word carries
-// its flag and value together and is the only datum exchanged between
-// chunks, so device-scope relaxed atomics (coherent across the XCDs' L2s)
-// suffice; release/acquire orderings would add an L2 write-back /
-// invalidate per access.  Returns the exclusive prefix to every thread.
+// 0: publish this chunk's aggregate T (fsst_lookback_publish), then sum
+// predecessors 64 at a time back to the nearest published inclusive prefix
+// (fsst_lookback_wait).  Between the two the workgroup may do other work.  A
+// chunk waits at most `spin_limit` sleeps for an unpublished predecessor;
+// past that (or at once when spin_limit is 0, which only tests set) it sets
+// the column's error flag and ends, so the host reports the failure and no
+// wave hangs.  A status word carries its flag and value together and is the
+// only datum exchanged between chunks, so device-scope relaxed atomics
+// (coherent across the XCDs' L2s) suffice; release/acquire orderings would
+// add an L2 write-back / invalidate per access.
+__device__ __forceinline__ void fsst_lookback_publish(uint32_t c, uint64_t T, FsstState* st, uint32_t spin_limit)
+{
+  if (threadIdx.x == 0) {
+    const gptr<uint64_t> status = gbl(st->status);
+    if (c == 0) __hip_atomic_store(&status[0], ST_INC | T, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    else if (spin_limit != 0) __hip_atomic_store(&status[c], ST_AGG | T, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
+// The wait half: synchronises the workgroup and returns the exclusive prefix
+// to every thread.
 template <class SM>
-__device__ __forceinline__ uint64_t fsst_lookback(uint32_t c, uint32_t n_chunks, uint64_t T, FsstState* st,
-                                                  uint32_t spin_limit, SM& sm)
+__device__ __forceinline__ uint64_t fsst_lookback_wait(uint32_t c, uint32_t n_chunks, uint64_t T, FsstState* st,
+                                                       uint32_t spin_limit, SM& sm)
 {
   if (threadIdx.x < WAVE) {
     const uint32_t lane = threadIdx.x;
     const gptr<uint64_t> status = gbl(st->status);
     uint64_t excl64     = 0;
     if (c == 0) {
-      if (lane == 0) __hip_atomic_store(&status[0], ST_INC | T, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      // published inclusive already
     } else if (spin_limit == 0) {  // no waiting allowed: report the look-back as failed
       if (lane == 0) __hip_atomic_fetch_or(gbl(&st->error), 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     } else {
-      if (lane == 0) __hip_atomic_store(&status[c], ST_AGG | T, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       uint32_t p = c - 1, spins = 0;
       for (;;) {
         const bool valid = lane <= p;
@@ -764,6 +785,15 @@ __device__ __forceinline__ uint64_t fsst_lookback(uint32_t c, uint32_t n_chunks,
   }
   __syncthreads();
   return sm.base;
+}
+
+// Both halves with nothing between them.
+template <class SM>
+__device__ __forceinline__ uint64_t fsst_lookback(uint32_t c, uint32_t n_chunks, uint64_t T, FsstState* st,
+                                                  uint32_t spin_limit, SM& sm)
+{
+  fsst_lookback_publish(c, T, st, spin_limit);
+  return fsst_lookback_wait(c, n_chunks, T, st, spin_limit, sm);
 }
 
 // Column offsets of the chunk's rows (and the column's closing offset).
@@ -1414,10 +1444,24 @@ __device__ __forceinline__ void fsst_chunk(const FsstChunk ch, const FsstChunkIn
       for (uint32_t i = tid; i <= rows; i += FSST_THREADS) sm.dec[i] = 0;
       __syncthreads();
     }
-    const uint64_t base = fsst_lookback(c, n_chunks, T, st, spin_limit, sm);
-    fsst_offsets(ch, c, n_chunks, rows, T, base, offsets, total_rows, sm);
-    if (T == 0 || base >= chars_cap) return;
-    if (fsst_emit_staged(sm, rows, c_h, c_hi, T, base, chars, chars_cap, stage)) return;
+    // The aggregate is published now; the look-back's wait, and the offsets,
+    // come once the first window's chars are in LDS.
+    fsst_lookback_publish(c, T, st, spin_limit);
+    uint64_t base = 0;
+    bool based    = false;
+    auto finish   = [&]() -> uint64_t {
+      if (!based) {
+        base = fsst_lookback_wait(c, n_chunks, T, st, spin_limit, sm);
+        fsst_offsets(ch, c, n_chunks, rows, T, base, offsets, total_rows, sm);
+        based = true;
+      } else {
+        __syncthreads();
+      }
+      return base;
+    };
+    const bool emitted = T != 0 && fsst_emit_staged(sm, rows, c_h, c_hi, T, finish, chars, chars_cap, stage);
+    if (!based) finish();
+    if (T == 0 || emitted || base >= chars_cap) return;
     fsst_emit_rows(sm, seg, dict_end, rows, T, base, chars, chars_cap);
   } else {
     const uint32_t T    = fsst_lengths_rows(sm, ok, seg, dict_end, rows);
