@@ -532,7 +532,7 @@ def headline_line(args, s: dict, x: dict | None) -> dict:
                    "mix": "40% BITPACKING INT32, 20% RLE INT64, 25% ALP DOUBLE, 15% FSST VARCHAR",
                    "arrivals": f"Poisson at {sr['rate_req_per_s']} req/s (0.8x measured capacity)",
                    "budget_bytes": 200 * 10 ** 9, "resident_gb": sr["resident_gb"],
-                   "path": "sdr_server, persistent group-kernel launches on 2 slot streams (continuous batching)",
+                   "path": "sdr_server, persistent batch launches on 2 slot streams (continuous batching)",
                    "max_batch_requests": sr.get("max_batch_requests")},
         "job_achieved_gbs": round(main["achieved_gbs"], 2),
         "job_frac_hbm": sr["frac_hbm"],
@@ -545,7 +545,7 @@ def headline_line(args, s: dict, x: dict | None) -> dict:
         "staged_from_file": x.get("staged_from_file"),
         "legs": {"extra": "ok" if x and "rc" not in x else (f"exit status {x['rc']}" if x else "skipped")},
         "roofline": {
-            "bound": "hbm", "kernel": "persistent batch launch (group kernels), served stream",
+            "bound": "hbm", "kernel": "persistent batch launches (per-codec grids and the FSST group kernel), served stream",
             "achieved": round(alg / (st["gpu_union_ms"] * 1e-3) / 1e9, 1) if st["gpu_union_ms"] > 0 else None,
             "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(alg / (st["gpu_union_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4) if st["gpu_union_ms"] > 0 else None,
