@@ -113,17 +113,18 @@ __global__ __launch_bounds__(BLOCK_THREADS) void k_bgroup_copy(const BatchItem* 
   }
 }
 
-__global__ __launch_bounds__(FSST_THREADS) void k_bgroup_fsst(const BatchItem* __restrict__ items,
+__global__ __launch_bounds__(FSST_THREADS) __attribute__((amdgpu_waves_per_eu(5))) void k_bgroup_fsst(const BatchItem* __restrict__ items,
                                                               BatchQueue* __restrict__ q, uint32_t g)
 {
   __shared__ FsstSmemT<FSST_OUT_BATCH> sm;
   __shared__ uint32_t s_idx;
   const uint32_t spin = q->spin_limit;
   for (uint32_t idx; (idx = claim(q, g, s_idx)) != UINT32_MAX;) {
-    const BatchItem it   = items[idx];
-    const FsstColumn col = gload(it.column);
-    const FsstChunk ch   = gload(col.chunks + it.chunk);
-    fsst_chunk<FSST_OUT_BATCH>(ch, gload(col.info + it.chunk), col.tabs + (ch.seg == FSST_GAP ? 0u : ch.seg),
+    // every lane reads the same descriptors: kept in SGPRs
+    const BatchItem it   = uniform(gload(items + idx));
+    const FsstColumn col = uniform(gload(it.column));
+    const FsstChunk ch   = uniform(gload(col.chunks + it.chunk));
+    fsst_chunk<FSST_OUT_BATCH>(ch, uniform(gload(col.info + it.chunk)), col.tabs + (ch.seg == FSST_GAP ? 0u : ch.seg),
                                it.chunk, col.n_chunks, col.state, col.offsets, col.chars, col.chars_cap,
                                col.total_rows, spin, sm);
   }

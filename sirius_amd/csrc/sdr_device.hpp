@@ -291,6 +291,22 @@ __device__ __forceinline__ uint32_t block_issue(uint32_t* lds, const uint8_t* p,
   return head;
 }
 
+// v with every 32-bit word taken from the first active lane: a descriptor
+// every lane loaded from the same address, made scalar (SGPRs instead of
+// VGPRs).
+template <typename T>
+__device__ __forceinline__ T uniform(const T& v)
+{
+  static_assert(sizeof(T) % 4 == 0, "whole words");
+  struct alignas(alignof(T)) Words {
+    uint32_t w[sizeof(T) / 4];
+  } x = __builtin_bit_cast(Words, v);
+#pragma unroll
+  for (uint32_t i = 0; i < sizeof(T) / 4; ++i)
+    x.w[i] = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(x.w[i])));
+  return __builtin_bit_cast(T, x);
+}
+
 // block_issue, then this wave's wait; the caller's barrier makes the whole
 // stage visible to the block.
 template <uint32_t NT>
