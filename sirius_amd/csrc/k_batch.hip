@@ -119,13 +119,16 @@ __global__ __launch_bounds__(FSST_THREADS) __attribute__((amdgpu_waves_per_eu(5)
   __shared__ FsstSmemT<FSST_OUT_BATCH> sm;
   __shared__ uint32_t s_idx;
   const uint32_t spin = q->spin_limit;
+  (void)items;  // the FSST group's entries are q->fsst_entries
+  const FsstEntry* entries = q->fsst_entries;
+  const FsstColumn* cols   = q->fsst_cols;
   for (uint32_t idx; (idx = claim(q, g, s_idx)) != UINT32_MAX;) {
     // every lane reads the same descriptors: kept in SGPRs
-    const BatchItem it   = uniform(gload(items + idx));
-    const FsstColumn col = uniform(gload(it.column));
-    const FsstChunk ch   = uniform(gload(col.chunks + it.chunk));
-    fsst_chunk<FSST_OUT_BATCH>(ch, uniform(gload(col.info + it.chunk)), col.tabs + (ch.seg == FSST_GAP ? 0u : ch.seg),
-                               it.chunk, col.n_chunks, col.state, col.offsets, col.chars, col.chars_cap,
+    const FsstEntry e    = uniform(gload(entries + idx));
+    const FsstColumn col = uniform(gload(cols + e.col));
+    const FsstChunk ch   = uniform(gload(col.chunks + e.chunk));
+    fsst_chunk<FSST_OUT_BATCH>(ch, uniform(gload(col.info + e.chunk)), col.tabs + (ch.seg == FSST_GAP ? 0u : ch.seg),
+                               e.chunk, col.n_chunks, col.state, col.offsets, col.chars, col.chars_cap,
                                col.total_rows, spin, sm);
   }
 }

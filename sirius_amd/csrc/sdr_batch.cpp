@@ -26,7 +26,6 @@ struct sdr_batch {
     uint8_t* chars;
     uint64_t cap;
     uint32_t total_rows;
-    uint32_t item_first;  // first queue entry of this column
     int status = SDR_OK;  // this column's outcome, set by batch_collect
   };
   std::vector<Col> cols;
@@ -208,16 +207,7 @@ int batch_add_strings(sdr_batch* b, sdr_string_column* col)
     cursor = g->row_offset + g->row_count;
   }
   if (cursor < n) add_range(nullptr, 0, cursor, n - cursor, 0);
-  bc.item_first = static_cast<uint32_t>(b->items.size());
-  const uint32_t ci = static_cast<uint32_t>(b->cols.size());
-  for (uint32_t k = 0; k < bc.n_chunks; ++k) {
-    BatchItem it{};
-    it.kind  = BK_FSST;
-    it.chunk = k;
-    it.unit0 = ci;  // resolved to a device column pointer at upload
-    b->items.push_back(it);
-    b->item_group.push_back(static_cast<uint8_t>(BG_FSST));
-  }
+  // the column's chunks become FSST queue entries at upload
   if (col->has_nulls && col->d_null_mask) b->masks.push_back({nullptr, col});
   b->cols.push_back(std::move(bc));
   return SDR_OK;
@@ -273,12 +263,16 @@ int batch_plan(sdr_batch* b, uint32_t per[BG_COUNT], uint32_t entries[BG_COUNT],
     tot[g] += b->items[i].n_units;
     maxu[g] = std::max<uint32_t>(1, batch_units(b, b->items[i].kind, b->items[i].ts));
   }
+  uint64_t fsst_chunks = 0;  // one FSST entry per chunk of every string column
+  for (const auto& c : b->cols) fsst_chunks += c.n_chunks;
+  if (fsst_chunks) *mask |= 1u << BG_FSST;
   if (b->fixed_grids) {  // fixed-width groups become per-codec grids: one work entry per segment
     for (uint32_t g = 0; g < BG_COUNT; ++g) {
       entries[g] = 0;
       per[g]     = 1;
     }
     for (size_t i = 0; i < b->items.size(); ++i) ++entries[b->item_group[i]];
+    entries[BG_FSST] = static_cast<uint32_t>(fsst_chunks);
     *mask &= 1u << BG_FSST;
     return batch_group_caps(b, *mask);
   }
@@ -296,8 +290,9 @@ int batch_plan(sdr_batch* b, uint32_t per[BG_COUNT], uint32_t entries[BG_COUNT],
   }
   for (size_t i = 0; i < b->items.size(); ++i) {
     const uint32_t g = b->item_group[i];
-    entries[g] += g == BG_FSST ? 1u : (b->items[i].n_units + per[g] - 1) / per[g];
+    entries[g] += (b->items[i].n_units + per[g] - 1) / per[g];
   }
+  entries[BG_FSST] = static_cast<uint32_t>(fsst_chunks);
   return SDR_OK;
 }
 
@@ -307,10 +302,10 @@ int batch_upload(sdr_batch* b, hipStream_t s)
   uint32_t per[BG_COUNT], entries[BG_COUNT], mask = 0;
   int prc = batch_plan(b, per, entries, &mask);
   if (prc) return prc;
-  size_t n_entries = 0;
+  size_t n_entries = 0;  // fixed-width queue entries (BatchItem); the FSST ones are FsstEntry
   for (uint32_t g = 0; g < BG_COUNT; ++g) {
     if (!(mask >> g & 1u)) entries[g] = 0;  // no queue entries: per-codec grids (or nothing)
-    n_entries += entries[g];
+    if (g != BG_FSST) n_entries += entries[g];
     b->entry_count[g] = entries[g];
   }
   b->n_entries = n_entries;
@@ -352,6 +347,8 @@ int batch_upload(sdr_batch* b, hipStream_t s)
   size_t off                  = al(sizeof(BatchQueue));
   b->off_items                = off;
   off                         = al(off + n_entries * sizeof(BatchItem));
+  const size_t off_fentries   = off;
+  off                         = al(off + size_t{entries[BG_FSST]} * sizeof(FsstEntry));
   b->off_fsegs                = off;
   off                         = al(off + fsegs.size() * sizeof(FixedSeg));
   b->off_cols                 = off;
@@ -399,14 +396,17 @@ int batch_upload(sdr_batch* b, hipStream_t s)
   uint8_t* h = b->h_stage;
   uint8_t* d = b->d_arena;
   BatchQueue q{};
-  // entries laid out group by group, in the order added within a group
+  // fixed-width entries laid out group by group, in the order added within a
+  // group; the FSST group's entries are an array of their own
   uint32_t at[BG_COUNT] = {};
   for (uint32_t g = 0, o = 0; g < BG_COUNT; ++g) {
     q.count[g] = entries[g];
-    q.first[g] = o;
-    at[g]      = o;
-    o += entries[g];
+    q.first[g] = g == BG_FSST ? 0u : o;
+    at[g]      = q.first[g];
+    if (g != BG_FSST) o += entries[g];
   }
+  q.fsst_entries = reinterpret_cast<const FsstEntry*>(d + off_fentries);
+  q.fsst_cols    = reinterpret_cast<const FsstColumn*>(d + b->off_cols);
   q.n_items       = static_cast<uint32_t>(n_entries);
   for (uint32_t w = 0; w < 4; ++w) q.bp_chunk_rows[w] = tune(SDR_TUNE_BP_CHUNK_ROWS, 1u << w);
   q.n_fsst_status = static_cast<uint32_t>(state_words);
@@ -446,17 +446,17 @@ int batch_upload(sdr_batch* b, hipStream_t s)
   }
   if (!dcols.empty()) std::memcpy(h + b->off_cols, dcols.data(), dcols.size() * sizeof(FsstColumn));
   if (!fsegs.empty()) std::memcpy(h + b->off_fsegs, fsegs.data(), fsegs.size() * sizeof(FixedSeg));
+  if (mask & (1u << BG_FSST)) {  // every chunk of every string column, column by column
+    auto* fe = reinterpret_cast<FsstEntry*>(h + off_fentries);
+    size_t j = 0;
+    for (uint32_t ci = 0; ci < b->cols.size(); ++ci)
+      for (uint32_t k = 0; k < b->cols[ci].n_chunks; ++k) fe[j++] = FsstEntry{ci, k};
+  }
   auto* hi = reinterpret_cast<BatchItem*>(h + b->off_items);
   for (size_t i = 0; i < b->items.size(); ++i) {
     const uint32_t g  = b->item_group[i];
     if (!(mask >> g & 1u)) continue;  // a per-codec grid's segment
     const BatchItem& it = b->items[i];
-    if (g == BG_FSST) {
-      BatchItem& dst = hi[at[g]++];
-      dst            = it;
-      dst.column     = reinterpret_cast<const FsstColumn*>(d + b->off_cols) + it.unit0;
-      continue;
-    }
     for (uint32_t u = 0; u < it.n_units; u += per[g]) {
       BatchItem& dst = hi[at[g]++];
       dst            = it;

@@ -35,25 +35,28 @@ struct FsstColumn {
   uint32_t n_chunks;
 };
 
-// One queue entry (64 bytes).  Fixed-width entries cover units
-// [unit0, unit0 + n_units) of one segment, spread over the block's four
-// wavefronts; an FSST entry is one chunk of one column, decoded by the
-// whole block.
+// One fixed-width queue entry (40 bytes): units [unit0, unit0 + n_units) of
+// one segment, spread over the block's four wavefronts (the host also keeps
+// one per segment to plan the per-codec grids).
 struct BatchItem {
   const uint8_t* seg;
-  uint8_t* out;  // output of the segment's row 0 (fixed-width)
+  uint8_t* out;  // output of the segment's row 0
   uint32_t bytes_size;
   uint32_t row_count;
   uint32_t unit0;
   uint32_t n_units;
   uint8_t kind;
   uint8_t ts;
-  uint16_t reserved;          // RLE: slice rows / 64
-  uint32_t chunk;             // FSST: chunk index in the column chain
-  const FsstColumn* column;   // FSST
-  uint64_t pad[2];
+  uint16_t reserved;  // RLE: slice rows / 64
 };
-static_assert(sizeof(BatchItem) == 64, "BatchItem layout");
+static_assert(sizeof(BatchItem) == 40, "BatchItem layout");
+
+// One FSST queue entry: chunk `chunk` of the batch's string column `col`,
+// decoded by a whole block.
+struct FsstEntry {
+  uint32_t col;
+  uint32_t chunk;
+};
 
 // Queue groups: entries of one decoder and value width, drained by one
 // specialised persistent kernel (k_batch.hip).  Entries are laid out group
@@ -89,6 +92,8 @@ struct BatchQueue {
   FsstChunk* fsst_chunks;
   FsstChunkInfo* fsst_info;
   FsstSymTab* fsst_tabs;
+  const FsstEntry* fsst_entries;  // the FSST group's entries (first[BG_FSST] is 0)
+  const FsstColumn* fsst_cols;    // the string columns they index
 };
 
 // Fixed grids of the reset and FSST pre-pass kernels (both loop), so one
