@@ -542,6 +542,7 @@ int batch_launch(sdr_batch* b, hipStream_t s)
       const hipError_t e = enqueue_launch(b, q, items, mask, grid, cnt, b->capture);
       hipGraph_t gr      = nullptr;
       const hipError_t ee = hipStreamEndCapture(b->capture, &gr);
+      if ((e != hipSuccess || ee != hipSuccess) && gr) (void)hipGraphDestroy(gr);
       HIP_TRY(e);
       HIP_TRY(ee);
       b->graph = gr;
