@@ -219,7 +219,7 @@ def kernel_times(lib, res, sh, stream, steps, warmup):
 
 
 def persistent_times(lib, res, sh, stream, steps, warmup):
-    """The same columns through one persistent, graph-replayed launch."""
+    """The same columns through one persistent batch launch (sdr_batch, default options)."""
     from sirius_amd import lib as L
 
     b = C.c_void_p()

@@ -218,7 +218,11 @@ int sdr_plan_destroy(sdr_plan* plan);
  * arena (or, with SDR_BATCH_FIXED_KERNELS = 1, in persistent group kernels
  * that drain a work queue), and the VARCHAR chunks in a persistent kernel
  * that claims them from the queue.  The arena is uploaded from pinned memory
- * with one copy and the launch is replayed from a hipGraph.  It has no
+ * with one copy ahead of the kernels.  A batch of fixed-width segments alone
+ * (no string column, validity mask or queue entry) passes its descriptors as
+ * sdr_decode_table does instead: up to 256 per per-codec grid in the kernel
+ * arguments, more through a reused descriptor buffer; it is never replayed
+ * from a graph.  It has no
  * counterpart in the reference, which launches one kernel per codec run
  * (src/cuda/scan/gpu_native_decode.cu:356-386, gpu_decode_strings.cu:257-366);
  * every output is identical to sdr_decode_table / sdr_decode_strings_column.
@@ -228,7 +232,9 @@ int sdr_plan_destroy(sdr_plan* plan);
 typedef struct sdr_batch sdr_batch;
 
 typedef enum sdr_batch_option {
-  SDR_BATCH_USE_GRAPH          = 1,  /* 1 (default): replay a captured hipGraph     */
+  SDR_BATCH_USE_GRAPH          = 1,  /* 1: replay a recorded hipGraph; 0 (default) */
+                                     /*   enqueue the kernels directly (faster on  */
+                                     /*   gfx950: no graph-launch overhead)        */
   SDR_BATCH_GRID               = 2,  /* blocks; 0 (default): resident capacity     */
   SDR_BATCH_FIXED_KERNELS      = 3,  /* fixed-width segments: 0 (default) the      */
                                      /*   per-codec kernels, one grid per codec    */

@@ -57,7 +57,7 @@ struct sdr_batch {
   uint32_t units[8]    = {0, 16, 2, 16, 16, 4, 4, 1};
   bool units_set[8]    = {};
   uint32_t grid        = 0;                         // 0 = resident capacity
-  bool use_graph       = true;
+  bool use_graph       = false;
   // fixed-width segments: per-codec grids over descriptors in the arena
   // (default), or persistent group kernels draining queue entries
   bool fixed_grids     = true;
@@ -65,6 +65,11 @@ struct sdr_batch {
   std::vector<sdr::FixedLaunch> fixed;              // the per-codec grids of the last upload
   std::vector<sdr::FixedLaunch> graph_fixed;        // ... that the recorded graph launches
   size_t graph_fsegs   = 0;                         // ... and where their descriptors were
+  // where the per-codec grids of the last upload read their descriptors: the
+  // arena, or (a batch of fixed-width segments alone, no graph) the kernel
+  // arguments or a descriptor slot, as the batched table call passes them
+  enum FixedSrc { FS_ARENA, FS_INLINE, FS_SLOT } fixed_src = FS_ARENA;
+  std::vector<sdr::FixedSeg> h_fsegs;               // ... the descriptors of the last upload, by grid
   hipStream_t capture  = nullptr;                   // records the graph of a launch with grids
   hipGraphExec_t gexec = nullptr;
   hipGraph_t graph     = nullptr;
@@ -311,7 +316,8 @@ int batch_upload(sdr_batch* b, hipStream_t s)
   b->n_entries = n_entries;
   // the per-codec grids' descriptors, by codec kind and value width, in the
   // order the items were added
-  std::vector<FixedSeg> fsegs;
+  std::vector<FixedSeg>& fsegs = b->h_fsegs;  // kept between launches: no allocation once warm
+  fsegs.clear();
   b->fixed.clear();
   if (b->fixed_grids) {
     // value widths 1, 2, 4, 8 and 16 (UNCOMPRESSED and CONSTANT)
@@ -334,6 +340,16 @@ int batch_upload(sdr_batch* b, hipStream_t s)
     }
     for (auto& fl : b->fixed)
       for (uint32_t j = 0; j < fl.n; ++j) fl.max_rows = std::max(fl.max_rows, fsegs[fl.base + j].row_count);
+  }
+  b->fixed_src = sdr_batch::FS_ARENA;
+  if (b->fixed_grids && mask == 0 && b->cols.empty() && b->masks.empty() && !b->fixed.empty()) {
+    // nothing on the device reads the arena: up to two launches of
+    // SEGS_INLINE descriptors per grid travel in the kernel arguments,
+    // larger grids read a descriptor slot uploaded at launch
+    const bool fits = std::all_of(b->fixed.begin(), b->fixed.end(),
+                                  [](const FixedLaunch& fl) { return fl.n <= 2u * SEGS_INLINE; });
+    b->fixed_src = fits ? sdr_batch::FS_INLINE : sdr_batch::FS_SLOT;
+    return SDR_OK;
   }
   auto al = [](size_t v) { return (v + 255u) & ~size_t{255}; };
   size_t n_chunks = 0, n_segs = 0;
@@ -471,20 +487,24 @@ int batch_upload(sdr_batch* b, hipStream_t s)
   return SDR_OK;
 }
 
-// The per-codec grids of the batch's fixed-width segments (descriptors in
-// the arena), one launch per codec kind and value width.
-hipError_t launch_fixed_grids(const sdr_batch* b, hipStream_t s)
+// The per-codec grids of the batch's fixed-width segments, one launch per
+// codec kind and value width: descriptors in the arena, in the kernel
+// arguments (FS_INLINE) or at `slot` (FS_SLOT).
+hipError_t launch_fixed_grids(const sdr_batch* b, hipStream_t s, const FixedSeg* slot = nullptr)
 {
-  const auto* segs = reinterpret_cast<const FixedSeg*>(b->d_arena + b->off_fsegs);
+  const bool host  = b->fixed_src == sdr_batch::FS_INLINE;
+  const auto* segs = host                                 ? b->h_fsegs.data()
+                     : b->fixed_src == sdr_batch::FS_SLOT ? slot
+                                                          : reinterpret_cast<const FixedSeg*>(b->d_arena + b->off_fsegs);
   for (const auto& fl : b->fixed) {
     hipError_t e = hipSuccess;
     switch (fl.kind) {
-      case BK_BITPACKING: e = launch_bitpacking(segs, fl.base, fl.n, fl.max_rows, fl.ts, s); break;
-      case BK_RLE: e = launch_rle(segs, fl.base, fl.n, fl.max_rows, fl.ts, s); break;
-      case BK_ALP: e = launch_alp(segs, fl.base, fl.n, fl.max_rows, fl.ts, false, s); break;
-      case BK_ALPRD: e = launch_alp(segs, fl.base, fl.n, fl.max_rows, fl.ts, true, s); break;
-      case BK_COPY: e = launch_uncompressed(segs, fl.base, fl.n, fl.max_rows, fl.ts, s); break;
-      default: e = launch_constant(segs, fl.base, fl.n, fl.max_rows, fl.ts, s); break;
+      case BK_BITPACKING: e = launch_bitpacking(segs, fl.base, fl.n, fl.max_rows, fl.ts, s, host); break;
+      case BK_RLE: e = launch_rle(segs, fl.base, fl.n, fl.max_rows, fl.ts, s, host); break;
+      case BK_ALP: e = launch_alp(segs, fl.base, fl.n, fl.max_rows, fl.ts, false, s, host); break;
+      case BK_ALPRD: e = launch_alp(segs, fl.base, fl.n, fl.max_rows, fl.ts, true, s, host); break;
+      case BK_COPY: e = launch_uncompressed(segs, fl.base, fl.n, fl.max_rows, fl.ts, s, host); break;
+      default: e = launch_constant(segs, fl.base, fl.n, fl.max_rows, fl.ts, s, host); break;
     }
     if (e != hipSuccess) return e;
   }
@@ -513,6 +533,23 @@ int batch_launch(sdr_batch* b, hipStream_t s)
   if (b->launch_done && b->last_stream && b->last_stream != s) HIP_TRY(hipStreamWaitEvent(s, b->launch_done, 0));
   rc = batch_upload(b, s);
   if (rc) return rc;
+  if (b->fixed_src != sdr_batch::FS_ARENA) {  // the launches the batched table call makes
+    void* d     = nullptr;
+    void* token = nullptr;
+    if (b->fixed_src == sdr_batch::FS_SLOT) {
+      rc = desc_acquire(b->h_fsegs.data(), b->h_fsegs.size() * sizeof(FixedSeg), s, &d, &token);
+      if (rc) return rc;
+    }
+    const hipError_t e = launch_fixed_grids(b, s, static_cast<const FixedSeg*>(d));
+    rc                 = desc_release(token, s);
+    HIP_TRY(e);
+    if (rc) return rc;
+    if (!b->launch_done) HIP_TRY(hipEventCreateWithFlags(&b->launch_done, hipEventDisableTiming));
+    HIP_TRY(hipEventRecord(b->launch_done, s));
+    b->last_stream = s;
+    b->launched    = true;
+    return SDR_OK;
+  }
   auto* q     = reinterpret_cast<BatchQueue*>(b->d_arena);
   auto* items = reinterpret_cast<const BatchItem*>(b->d_arena + b->off_items);
   // each group kernel's grid: its resident capacity (or the SDR_BATCH_GRID

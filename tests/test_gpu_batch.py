@@ -108,16 +108,19 @@ def test_batch_mixed_codecs(work, graph, fixed, units):
         lib.sdr_batch_destroy(b)
 
 
+@pytest.mark.parametrize("graph", [1, 0], ids=["graph", "nograph"])
 @pytest.mark.parametrize("fixed", [0, 1], ids=["grids", "groups"])
-def test_graph_replays_with_new_queues(work, fixed):
+def test_graph_replays_with_new_queues(work, fixed, graph):
     """Graph replays with different queue contents (a new recording where
     the launch shapes change), eager work and synchronisation between
-    replays; every replay checked."""
+    replays; every replay checked.  Without the graph the same sequence of
+    direct launches."""
     lib = L.load()
     other = Workload(100_000, 200)
     b = C.c_void_p()
     L.check(lib.sdr_batch_create(C.byref(b)))
     try:
+        L.check(lib.sdr_batch_set_option(b, A.BATCH_USE_GRAPH, graph))
         L.check(lib.sdr_batch_set_option(b, A.BATCH_FIXED_KERNELS, fixed))
         for w in (work, other, work, work):
             w.wipe()
@@ -169,3 +172,60 @@ def test_batch_validity_null_counts():
     assert arr[0].null_count == int((~valid).sum())
     np.testing.assert_array_equal(out.cpu().numpy(), O.decode_encoded(enc))
     np.testing.assert_array_equal(mask.cpu().numpy(), mb)
+
+
+@pytest.mark.parametrize("graph", [1, 0], ids=["graph", "nograph"])
+def test_fixed_only_descriptor_forms_alternate(graph):
+    """A batch of fixed-width segments alone passes its descriptors like the
+    batched table call: at most 256 per per-codec grid in the kernel
+    arguments, more through a descriptor slot (neither uploads the arena or
+    replays a graph); a batch that adds a string column goes through the
+    arena.  Launches of one batch object alternate between the three forms,
+    each checked."""
+    lib = L.load()
+    rows = 2048
+
+    def table(n_segs, seed):
+        n = n_segs * rows
+        encs = [E.encode_bitpacking(E.gen_bp_int32(seed, n), rg_rows=rows),
+                E.encode_rle(E.gen_runs_int64(seed + 1, n), rg_rows=rows)]
+        cols, outs, keep = [], [], []
+        for enc in encs:
+            blob = torch.from_numpy(enc.blob).to(_dev())
+            out = torch.full((enc.n_rows * enc.type_size,), 0xCC, dtype=torch.uint8, device=_dev())
+            segs = enc.segments(blob.data_ptr())
+            col, k = fixed_column([(enc.codec, segs)], enc.type_size, enc.n_rows, out.data_ptr())
+            keep += [blob, k]
+            cols.append(col)
+            outs.append(out)
+        return encs, outs, (A.Column * len(cols))(*cols), keep, len(encs[0].segments(0))
+
+    small = table(256, 31)
+    big = table(257, 41)
+    assert small[4] <= 256 < big[4]
+    strings = Workload(20_000, 51)
+    b = C.c_void_p()
+    L.check(lib.sdr_batch_create(C.byref(b)))
+    try:
+        L.check(lib.sdr_batch_set_option(b, A.BATCH_USE_GRAPH, graph))
+        for form in ("inline", "slot", "arena", "inline", "slot", "slot", "arena", "inline"):
+            encs, outs, arr, _, _ = big if form == "slot" else small
+            for o in outs:
+                o.fill_(0xCC)
+            strings.wipe()
+            torch.cuda.synchronize()
+            L.check(lib.sdr_batch_clear(b))
+            L.check(lib.sdr_batch_add_table(b, arr, len(encs)))
+            if form == "arena":
+                L.check(lib.sdr_batch_add_strings(b, C.byref(strings.scol)))
+            L.check(lib.sdr_batch_launch(b, sh()))
+            L.check(lib.sdr_batch_finish(b, sh()))
+            for enc, out in zip(encs, outs):
+                np.testing.assert_array_equal(out.cpu().numpy(), O.decode_encoded(enc),
+                                              err_msg=f"{form} {A.CODEC_NAMES[enc.codec]}")
+            if form == "arena":
+                assert strings.scol.chars_used == len(strings.str_chars)
+                np.testing.assert_array_equal(strings.d_chars.cpu().numpy()[: len(strings.str_chars)],
+                                              strings.str_chars)
+    finally:
+        lib.sdr_batch_destroy(b)
