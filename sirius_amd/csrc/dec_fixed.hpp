@@ -400,10 +400,15 @@ __device__ __forceinline__ void alprd_vector(const uint8_t* seg, uint32_t bytes,
     }
     return;
   }
-  // dictionary: unused slots read as zero
-  R dict[8];
+  // dictionary: eight u16 entries packed two to a word (unused slots read
+  // as zero), so it takes four SGPRs and the kernel keeps 8 waves per SIMD
+  uint32_t dict[4];
 #pragma unroll
-  for (uint32_t d = 0; d < 8; ++d) dict[d] = d < dsz ? static_cast<R>(su16(seg + 7u + 2u * d)) : R(0);
+  for (uint32_t d = 0; d < 4; ++d) {
+    const uint32_t lo = 2u * d < dsz ? su16(seg + 7u + 4u * d) : 0u;
+    const uint32_t hi = 2u * d + 1u < dsz ? su16(seg + 9u + 4u * d) : 0u;
+    dict[d]           = lo | hi << 16;
+  }
 
   const uint8_t* lp = seg + data_off + 2u;
   const uint8_t* rp = lp + lpb;
@@ -439,10 +444,9 @@ __device__ __forceinline__ void alprd_vector(const uint8_t* seg, uint32_t bytes,
           if (full_right) {
             bits = right;
           } else {
-            const uint32_t li = lds_bits32(lds_l, lhead + row * lbw, lbw);
-            R dv              = dict[0];
-#pragma unroll
-            for (uint32_t d = 1; d < 8; ++d) dv = li == d ? dict[d] : dv;
+            const uint32_t li = lds_bits32(lds_l, lhead + row * lbw, lbw);  // < 8: lbw <= 3
+            const uint32_t pw = (li & 4u) ? ((li & 2u) ? dict[3] : dict[2]) : ((li & 2u) ? dict[1] : dict[0]);
+            const R dv        = static_cast<R>((li & 1u) ? pw >> 16 : pw & 0xFFFFu);
             bits = static_cast<R>(dv << rbw) | right;
           }
         }
