@@ -1345,9 +1345,11 @@ __device__ __forceinline__ void fsst_prep_segment(const FsstSeg sg, uint32_t si,
 // ---- chunk decode ----
 
 // Decode chunk `c` (its position in the column's chunk order) with the
-// calling 256-thread workgroup.  Every thread must call it.  `ci` and `tab`
-// are the segment pre-pass's output for the chunk and its segment (unused
-// for a chunk of uncovered rows).
+// calling 256-thread workgroup.  Every thread must call it, after a
+// workgroup barrier that follows any earlier use of `sm` (the callers' claim
+// of the chunk ends with one), so the first loads issue without waiting.
+// `ci` and `tab` are the segment pre-pass's output for the chunk and its
+// segment (unused for a chunk of uncovered rows).
 //
 // One memory round trip stages, with LDS-direct loads, the symbol table,
 // the chunk's compressed-length fields and its compressed bytes; a second
@@ -1379,7 +1381,6 @@ __device__ __forceinline__ void fsst_chunk(const FsstChunk ch, const FsstChunkIn
   const bool staged = ok && rows && !(ci.flags & FI_HUGE) && c_hi >= c_lo && c_hi <= dict_end &&
                       c_hi - c_lo <= FSST_CSTAGE;
 
-  __syncthreads();  // the previous user of sm is done
   if (ok) block_issue<FSST_THREADS>(reinterpret_cast<uint32_t*>(sm.sym), reinterpret_cast<const uint8_t*>(tab),
                                     sizeof(FsstSymTab));
   uint32_t bit0 = 0;

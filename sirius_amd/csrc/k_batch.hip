@@ -30,14 +30,17 @@ static_assert(BATCH_PREP_THREADS == FSST_THREADS, "pre-pass workgroup size");
 static_assert(BATCH_WAVES == WAVES_PER_BLOCK, "fixed-width group block");
 
 // Claim the next FSST entry: every thread gets its index in the queue, or
-// UINT32_MAX once the group is drained.
-__device__ __forceinline__ uint32_t claim(BatchQueue* q, uint32_t g, uint32_t& s_idx)
+// UINT32_MAX once the group is drained.  The group's `first` and `count`
+// are read once by the caller: reloading them after the barriers would put
+// two more dependent memory round trips in front of every chunk.
+__device__ __forceinline__ uint32_t claim(BatchQueue* q, uint32_t g, uint32_t first, uint32_t count,
+                                          uint32_t& s_idx)
 {
   if (threadIdx.x == 0) s_idx = atomicAdd(&q->claimed[g].v, 1u);
   __syncthreads();
   const uint32_t i = s_idx;
   __syncthreads();  // every thread has read it before the next claim
-  return i < q->count[g] ? q->first[g] + i : UINT32_MAX;
+  return i < count ? first + i : UINT32_MAX;
 }
 
 template <typename T>
@@ -120,16 +123,26 @@ __global__ __launch_bounds__(FSST_THREADS) __attribute__((amdgpu_waves_per_eu(5)
   __shared__ uint32_t s_idx;
   const uint32_t spin = q->spin_limit;
   (void)items;  // the FSST group's entries are q->fsst_entries
-  const FsstEntry* entries = q->fsst_entries;
-  const FsstColumn* cols   = q->fsst_cols;
-  for (uint32_t idx; (idx = claim(q, g, s_idx)) != UINT32_MAX;) {
+  const FsstEntry* entries    = q->fsst_entries;
+  const FsstColumn* cols      = q->fsst_cols;
+  // the entries list every column's chunks column by column, as the batch's
+  // chunk and chunk-info arrays do: entry idx describes chunk idx of those
+  // arrays, so the chunk loads need not wait for the column's
+  const FsstChunk* chunks     = q->fsst_chunks;
+  const FsstChunkInfo* infos  = q->fsst_info;
+  const FsstSymTab* tabs      = q->fsst_tabs;
+  const uint32_t first = q->first[g], count = q->count[g];
+  for (uint32_t idx; (idx = claim(q, g, first, count, s_idx)) != UINT32_MAX;) {
     // every lane reads the same descriptors: kept in SGPRs
-    const FsstEntry e    = uniform(gload(entries + idx));
-    const FsstColumn col = uniform(gload(cols + e.col));
-    const FsstChunk ch   = uniform(gload(col.chunks + e.chunk));
-    fsst_chunk<FSST_OUT_BATCH>(ch, uniform(gload(col.info + e.chunk)), col.tabs + (ch.seg == FSST_GAP ? 0u : ch.seg),
-                               e.chunk, col.n_chunks, col.state, col.offsets, col.chars, col.chars_cap,
-                               col.total_rows, spin, sm);
+    const FsstEntry e      = uniform(gload(entries + idx));
+    const FsstChunk ch     = uniform(gload(chunks + idx));
+    const FsstChunkInfo ci = uniform(gload(infos + idx));
+    // the column comes through the scalar cache (read-only for the whole
+    // launch), so its wait falls at the chunk's first barrier, behind the
+    // loads of the symbol table and the compressed bytes
+    const FsstColumn col = sload(cols + e.col);
+    fsst_chunk<FSST_OUT_BATCH>(ch, ci, tabs + (ch.seg == FSST_GAP ? 0u : ch.seg), e.chunk, col.n_chunks, col.state,
+                               col.offsets, col.chars, col.chars_cap, col.total_rows, spin, sm);
   }
 }
 

@@ -123,6 +123,23 @@ __device__ __forceinline__ uint64_t uniform_addr(const uint8_t* p)
 
 __device__ __forceinline__ uint32_t su_word(uint64_t aligned4) { return *(const_word*)(aligned4); }
 
+// A whole wave-uniform descriptor read the same way, for descriptors that no
+// kernel of the launch writes.  A scalar load's wait is placed at the first
+// use of the value (or the next wait on the LDS/scalar counter), so loads
+// issued after it overlap it.
+template <typename T>
+__device__ __forceinline__ T sload(const T* p)
+{
+  static_assert(sizeof(T) % 4 == 0 && alignof(T) >= 4, "whole aligned words");
+  struct alignas(alignof(T)) Words {
+    uint32_t w[sizeof(T) / 4];
+  } x;
+  const uint64_t a = uniform_addr(reinterpret_cast<const uint8_t*>(p));
+#pragma unroll
+  for (uint32_t i = 0; i < sizeof(T) / 4; ++i) x.w[i] = su_word(a + 4u * i);
+  return __builtin_bit_cast(T, x);
+}
+
 __device__ __forceinline__ uint32_t su8(const uint8_t* p)
 {
   const uint64_t a = uniform_addr(p);
@@ -284,7 +301,8 @@ __device__ __forceinline__ uint32_t block_issue(uint32_t* lds, const uint8_t* p,
   const uintptr_t a16 = a & ~uintptr_t{15};
   const uint32_t head = static_cast<uint32_t>(a - a16);
   const uint32_t nblk = nbytes ? (head + nbytes + 15u) >> 4 : 0u;
-  const uint32_t wb   = (threadIdx.x / WAVE) * WAVE;  // first block of this wave per round
+  // first block of this wave per round (wave-uniform: its LDS address is an SGPR for M0)
+  const uint32_t wb   = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x / WAVE))) * WAVE;
   const uint8_t* src  = reinterpret_cast<const uint8_t*>(a16) + 16u * threadIdx.x;
   for (uint32_t k = 0; k * NT < nblk; ++k)
     if (k * NT + threadIdx.x < nblk) lds_direct_16(src + 16u * NT * k, lds + 4u * (NT * k + wb));
