@@ -437,9 +437,9 @@ __device__ __forceinline__ uint32_t fsst_lengths_staged(SM& sm, uint32_t rows, u
         if (y == rend && r) {  // rows starting at y
           do {
             --r;
-            sm.dec[r] = acc;
-            rend      = rnext;
-            rnext     = r ? top - sm.comp[r - 1] : rend;
+            sm.dec[r + 1] = acc;  // the prefix at row r's start
+            rend          = rnext;
+            rnext         = r ? top - sm.comp[r - 1] : rend;
           } while (y == rend && r);
           esc = false;
         }
@@ -454,9 +454,9 @@ __device__ __forceinline__ uint32_t fsst_lengths_staged(SM& sm, uint32_t rows, u
     if (yb == y_hi)
       while (rend == y_hi && r) {
         --r;
-        sm.dec[r] = acc;
-        rend      = rnext;
-        rnext     = r ? top - sm.comp[r - 1] : rend;
+        sm.dec[r + 1] = acc;
+        rend          = rnext;
+        rnext         = r ? top - sm.comp[r - 1] : rend;
       }
   }
   uint32_t T         = 0;
@@ -466,21 +466,15 @@ __device__ __forceinline__ uint32_t fsst_lengths_staged(SM& sm, uint32_t rows, u
     for (uint32_t g = (ya + 15u) & ~15u; g < yb; g += 16) sm.gst[g >> 4] += pre;
   __syncthreads();
   // dec[r + 1] = T - (decoded prefix at row r's start); the prefix was
-  // recorded by the thread whose span holds that start
-  uint32_t dv[ROWS_PER_THR];
+  // recorded at dec[r + 1], span-relative, by the thread whose span holds
+  // that start, so each slot is read and rewritten by one thread here
 #pragma unroll
   for (uint32_t k = 0; k < ROWS_PER_THR; ++k) {
     const uint32_t r = k * FSST_THREADS + tid;
     if (r < rows) {
       const uint32_t p = top - sm.comp[r + 1];
-      dv[k]            = T - (sm.dec[r] + sm.part[(p < y_hi ? p : y_hi - 1u) / span]);
+      sm.dec[r + 1]    = T - (sm.dec[r + 1] + sm.part[(p < y_hi ? p : y_hi - 1u) / span]);
     }
-  }
-  __syncthreads();
-#pragma unroll
-  for (uint32_t k = 0; k < ROWS_PER_THR; ++k) {
-    const uint32_t r = k * FSST_THREADS + tid;
-    if (r < rows) sm.dec[r + 1] = dv[k];
   }
   if (tid == 0) sm.dec[0] = 0;
   __syncthreads();
@@ -557,18 +551,20 @@ __device__ __forceinline__ bool fsst_emit_staged(SM& sm, uint32_t rows, uint32_t
   const uint32_t top  = c_h + c_hi;
   const uintptr_t cb  = reinterpret_cast<uintptr_t>(chars);
   uint32_t r_hi       = rows - 1;  // highest row of the next window
-  for (;;) {
+  // Wave 0 finds the lowest row r_lo of the window whose highest row is hi
+  // (dec[hi + 1] - dec[r_lo] <= OUT_CAP) and leaves it in sm.win; the next
+  // window's search runs before the barrier that ends a window.
+  auto find_window = [&](uint32_t hi) {
     if (tid < WAVE) {
-      // lowest r_lo with dec[r_hi + 1] - dec[r_lo] <= OUT_CAP, by 64-way
-      // search rounds over (a, b] (fails at a, holds at b)
-      const uint32_t d_top = sm.dec[r_hi + 1];
+      // 64-way search rounds over (a, b] (fails at a, holds at b)
+      const uint32_t d_top = sm.dec[hi + 1];
       uint32_t r_lo;
       if (d_top <= OUT_CAP) {
         r_lo = 0;
-      } else if (d_top - sm.dec[r_hi] > OUT_CAP) {
+      } else if (d_top - sm.dec[hi] > OUT_CAP) {
         r_lo = ~0u;
       } else {
-        uint32_t a = 0, b = r_hi;
+        uint32_t a = 0, b = hi;
         while (b - a > 1u) {
           const uint32_t step = (b - a + WAVE - 1u) / WAVE;
           const uint32_t m    = a + (lane + 1u) * step < b ? a + (lane + 1u) * step : b;
@@ -582,8 +578,11 @@ __device__ __forceinline__ bool fsst_emit_staged(SM& sm, uint32_t rows, uint32_t
       }
       if (tid == 0) sm.win = r_lo;
     }
-    __syncthreads();
-    const uint32_t r_lo = sm.win;
+  };
+  find_window(r_hi);
+  __syncthreads();
+  for (;;) {
+    const uint32_t r_lo = sm.win;  // every thread reads it before the get_base() barrier
     if (r_lo == ~0u) return false;
     const uint32_t wlo  = sm.dec[r_lo];
     const uint32_t whi  = sm.dec[r_hi + 1];
@@ -713,6 +712,7 @@ __device__ __forceinline__ bool fsst_emit_staged(SM& sm, uint32_t rows, uint32_t
         }
       }
     }
+    if (r_lo != 0) find_window(r_lo - 1u);  // sm.win was read before get_base()'s barrier
     __syncthreads();
     if (r_lo == 0) return true;
     r_hi = r_lo - 1;
