@@ -196,7 +196,6 @@ struct FsstSmemT {
   // output window; before that, pieces of the compressed-length stream
   __attribute__((aligned(16))) uint8_t out[(OUT_CAP > FSST_PIECE ? OUT_CAP : FSST_PIECE) + 48];
   uint32_t wave[FSST_WAVES];
-  uint32_t win;
   uint64_t base;
 
   // The second walk's output window.  The chunk's compressed bytes are not
@@ -551,38 +550,29 @@ __device__ __forceinline__ bool fsst_emit_staged(SM& sm, uint32_t rows, uint32_t
   const uint32_t top  = c_h + c_hi;
   const uintptr_t cb  = reinterpret_cast<uintptr_t>(chars);
   uint32_t r_hi       = rows - 1;  // highest row of the next window
-  // Wave 0 finds the lowest row r_lo of the window whose highest row is hi
-  // (dec[hi + 1] - dec[r_lo] <= OUT_CAP) and leaves it in sm.win; the next
-  // window's search runs before the barrier that ends a window.
-  auto find_window = [&](uint32_t hi) {
-    if (tid < WAVE) {
-      // 64-way search rounds over (a, b] (fails at a, holds at b)
-      const uint32_t d_top = sm.dec[hi + 1];
-      uint32_t r_lo;
-      if (d_top <= OUT_CAP) {
-        r_lo = 0;
-      } else if (d_top - sm.dec[hi] > OUT_CAP) {
-        r_lo = ~0u;
-      } else {
-        uint32_t a = 0, b = hi;
-        while (b - a > 1u) {
-          const uint32_t step = (b - a + WAVE - 1u) / WAVE;
-          const uint32_t m    = a + (lane + 1u) * step < b ? a + (lane + 1u) * step : b;
-          const uint64_t hold = ballot(d_top - sm.dec[m] <= OUT_CAP);
-          const uint32_t f    = static_cast<uint32_t>(__builtin_ctzll(hold));  // lane 63 tests b, which holds
-          const uint32_t nb   = a + (f + 1u) * step < b ? a + (f + 1u) * step : b;
-          a                   = a + f * step;
-          b                   = nb;
-        }
-        r_lo = b;
-      }
-      if (tid == 0) sm.win = r_lo;
+  // The lowest row of the window whose highest row is hi: the lowest r_lo
+  // with dec[hi + 1] - dec[r_lo] <= OUT_CAP (~0u: row hi alone is larger).
+  // Every wave searches for itself (the same answer from the same dec[]),
+  // so no barrier hands it out.
+  auto find_window = [&](uint32_t hi) -> uint32_t {
+    const uint32_t d_top = sm.dec[hi + 1];
+    if (d_top <= OUT_CAP) return 0u;
+    if (d_top - sm.dec[hi] > OUT_CAP) return ~0u;
+    // 64-way search rounds over (a, b] (fails at a, holds at b)
+    uint32_t a = 0, b = hi;
+    while (b - a > 1u) {
+      const uint32_t step = (b - a + WAVE - 1u) / WAVE;
+      const uint32_t m    = a + (lane + 1u) * step < b ? a + (lane + 1u) * step : b;
+      const uint64_t hold = ballot(d_top - sm.dec[m] <= OUT_CAP);
+      const uint32_t f    = static_cast<uint32_t>(__builtin_ctzll(hold));  // lane 63 tests b, which holds
+      const uint32_t nb   = a + (f + 1u) * step < b ? a + (f + 1u) * step : b;
+      a                   = a + f * step;
+      b                   = nb;
     }
+    return b;
   };
-  find_window(r_hi);
-  __syncthreads();
+  uint32_t r_lo = find_window(r_hi);
   for (;;) {
-    const uint32_t r_lo = sm.win;  // every thread reads it before the get_base() barrier
     if (r_lo == ~0u) return false;
     const uint32_t wlo  = sm.dec[r_lo];
     const uint32_t whi  = sm.dec[r_hi + 1];
@@ -712,10 +702,10 @@ __device__ __forceinline__ bool fsst_emit_staged(SM& sm, uint32_t rows, uint32_t
         }
       }
     }
-    if (r_lo != 0) find_window(r_lo - 1u);  // sm.win was read before get_base()'s barrier
     __syncthreads();
     if (r_lo == 0) return true;
     r_hi = r_lo - 1;
+    r_lo = find_window(r_hi);
   }
 }
 

@@ -31,15 +31,18 @@ static_assert(BATCH_WAVES == WAVES_PER_BLOCK, "fixed-width group block");
 
 // Claim the next FSST entry: every thread gets its index in the queue, or
 // UINT32_MAX once the group is drained.  The group's `first` and `count`
-// are read once by the caller: reloading them after the barriers would put
-// two more dependent memory round trips in front of every chunk.
+// are read once by the caller: reloading them after the barrier would put
+// two more dependent memory round trips in front of every chunk.  Claims
+// alternate between two LDS slots: slot k is written again two claims
+// later, after every thread has passed the next claim's barrier and so has
+// read it, so one barrier per claim suffices.
 __device__ __forceinline__ uint32_t claim(BatchQueue* q, uint32_t g, uint32_t first, uint32_t count,
-                                          uint32_t& s_idx)
+                                          uint32_t (&s_idx)[2], uint32_t& k)
 {
-  if (threadIdx.x == 0) s_idx = atomicAdd(&q->claimed[g].v, 1u);
+  k ^= 1u;
+  if (threadIdx.x == 0) s_idx[k] = atomicAdd(&q->claimed[g].v, 1u);
   __syncthreads();
-  const uint32_t i = s_idx;
-  __syncthreads();  // every thread has read it before the next claim
+  const uint32_t i = s_idx[k];
   return i < count ? first + i : UINT32_MAX;
 }
 
@@ -120,7 +123,8 @@ __global__ __launch_bounds__(FSST_THREADS) __attribute__((amdgpu_waves_per_eu(5)
                                                               BatchQueue* __restrict__ q, uint32_t g)
 {
   __shared__ FsstSmemT<FSST_OUT_BATCH> sm;
-  __shared__ uint32_t s_idx;
+  __shared__ uint32_t s_idx[2];
+  uint32_t slot = 0;
   const uint32_t spin = q->spin_limit;
   (void)items;  // the FSST group's entries are q->fsst_entries
   const FsstEntry* entries    = q->fsst_entries;
@@ -132,7 +136,7 @@ __global__ __launch_bounds__(FSST_THREADS) __attribute__((amdgpu_waves_per_eu(5)
   const FsstChunkInfo* infos  = q->fsst_info;
   const FsstSymTab* tabs      = q->fsst_tabs;
   const uint32_t first = q->first[g], count = q->count[g];
-  for (uint32_t idx; (idx = claim(q, g, first, count, s_idx)) != UINT32_MAX;) {
+  for (uint32_t idx; (idx = claim(q, g, first, count, s_idx, slot)) != UINT32_MAX;) {
     // every lane reads the same descriptors: kept in SGPRs
     const FsstEntry e      = uniform(gload(entries + idx));
     const FsstChunk ch     = uniform(gload(chunks + idx));
