@@ -175,6 +175,11 @@ constexpr uint32_t FSST_OUT_BATCH  = 16384;  // ... of the persistent batch kern
 // registers at the chunk's start
 constexpr uint32_t SPAN_GROUPS = (FSST_CSTAGE + 15u + 16u * FSST_THREADS - 1u) / (16u * FSST_THREADS);
 static_assert(SPAN_GROUPS == 5, "first-walk span registers");
+// s_waitcnt vmcnt(SPAN_GROUPS), expcnt and lgkmcnt left at their maxima
+// (gfx9 encoding: vmcnt bits 3:0, expcnt 6:4, lgkmcnt 11:8)
+constexpr int WAITCNT_VMEM_SPANS = 0x0F70 | static_cast<int>(SPAN_GROUPS);
+// What a chunk without compressed bytes loads as its span (never used).
+static __device__ __attribute__((aligned(16))) const uint32_t fsst_zero_group[4] = {0u, 0u, 0u, 0u};
 
 // Walk state at a grid point: decoded prefix (bits 0-19), escape flag (bit
 // 20), row (bits 21-31).  A staged chunk decodes to at most 8 * FSST_CSTAGE
@@ -360,14 +365,20 @@ __device__ __forceinline__ uint32_t group_byte(const u32x4& q, uint32_t j)
 // Thread t's first-walk span: the 16-byte groups [t * span, t * span + span)
 // of stage positions below y_hi, stage position 0 being the 16-byte-aligned
 // address `stage` (the blocks an LDS stage of the chunk would hold).
+//
+// Every thread issues exactly SPAN_GROUPS loads (y_hi > 0), with no branch
+// around them: a group outside its span rereads the stage's last group and
+// is never used.  The caller can then wait for the loads issued before these
+// with vmcnt(SPAN_GROUPS).
 __device__ __forceinline__ void fsst_span_load(u32x4 (&qv)[SPAN_GROUPS], const uint8_t* stage, uint32_t y_hi)
 {
   const uint32_t span = span_bytes(0, y_hi);
   const uint32_t s0   = threadIdx.x * span;
+  const uint32_t last = (y_hi - 1u) >> 4;
   const gptr<const u32x4> cg = gbl(reinterpret_cast<const u32x4*>(stage));
 #pragma unroll
   for (uint32_t k = 0; k < SPAN_GROUPS; ++k)
-    if (16u * k < span && s0 + 16u * k < y_hi) qv[k] = cg[(s0 >> 4) + k];
+    qv[k] = cg[16u * k < span && s0 + 16u * k < y_hi ? (s0 >> 4) + k : last];
 }
 
 // The last word of thread t's span (the four bytes before thread t + 1's
@@ -1385,17 +1396,22 @@ __device__ __forceinline__ void fsst_chunk(const FsstChunk ch, const FsstChunkIn
   // [dict_end - comp[r+1], dict_end - comp[r]).  Stage position 0 is the
   // 16-byte-aligned address at or below the chunk's first compressed byte;
   // each thread loads its first-walk span into registers.
-  uint32_t c_h          = 0;
-  const uint8_t* stage  = nullptr;
-  u32x4 qv[SPAN_GROUPS] = {};
+  uint32_t c_h         = 0;
+  const uint8_t* stage = nullptr;
   if (staged) {
     const uintptr_t a = reinterpret_cast<uintptr_t>(seg + dict_end - c_hi);
     c_h               = static_cast<uint32_t>(a & 15u);
     stage             = reinterpret_cast<const uint8_t*>(a - c_h);
-    fsst_span_load(qv, stage, c_h + (c_hi - c_lo));
   }
-  __builtin_amdgcn_s_waitcnt(WAITCNT_VMEM_DONE);
-  __syncthreads();
+  // The span loads go out last and unconditionally (a chunk with no
+  // compressed bytes reads a zero group), so the wait below covers the
+  // symbol table and length fields only: the compressed prefix runs while
+  // the spans still arrive.
+  const bool spans = staged && c_h + (c_hi - c_lo) != 0;
+  u32x4 qv[SPAN_GROUPS];
+  fsst_span_load(qv, spans ? stage : reinterpret_cast<const uint8_t*>(fsst_zero_group),
+                 spans ? c_h + (c_hi - c_lo) : 16u);
+  __asm__ volatile("s_waitcnt vmcnt(5) lgkmcnt(0)\n\ts_barrier" ::: "memory");
 
   // ---- compressed prefix comp[] (fsst.cu:132-188); thread t owns rows
   // 4t .. 4t + 3 here ----
