@@ -144,14 +144,23 @@ struct ByteReader {
   }
 };
 
+// A workgroup barrier for LDS alone: each wave's LDS operations complete
+// first, global loads stay in flight.  (__syncthreads() also waits for every
+// pending load while the compiler tracks LDS-direct loads, which it cannot
+// tell apart from register loads.)
+__device__ __forceinline__ void lds_barrier() { __asm__ volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+
 // Block-wide exclusive scan of one value per thread; returns the exclusive
-// prefix and writes the block total to *total.
+// prefix and writes the block total to *total.  LdsOnly: its barriers leave
+// global loads in flight.
+template <bool LdsOnly = false>
 __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* s_wave, uint32_t* total)
 {
   const uint32_t incl = wave_incl_sum_u32(v);
   const uint32_t w    = threadIdx.x >> 6;
   if (lane_id() == 63) s_wave[w] = incl;
-  __syncthreads();
+  if constexpr (LdsOnly) lds_barrier();
+  else __syncthreads();
   uint32_t before = 0, tot = 0;
 #pragma unroll
   for (uint32_t k = 0; k < FSST_WAVES; ++k) {
@@ -159,7 +168,8 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* s_wave
     before += k < w ? x : 0u;
     tot += x;
   }
-  __syncthreads();
+  if constexpr (LdsOnly) lds_barrier();
+  else __syncthreads();
   *total = tot;
   return before + incl - v;
 }
@@ -1411,6 +1421,7 @@ __device__ __forceinline__ void fsst_chunk(const FsstChunk ch, const FsstChunkIn
   u32x4 qv[SPAN_GROUPS];
   fsst_span_load(qv, spans ? stage : reinterpret_cast<const uint8_t*>(fsst_zero_group),
                  spans ? c_h + (c_hi - c_lo) : 16u);
+  static_assert(SPAN_GROUPS == 5, "the wait below leaves the five span loads in flight");
   __asm__ volatile("s_waitcnt vmcnt(5) lgkmcnt(0)\n\ts_barrier" ::: "memory");
 
   // ---- compressed prefix comp[] (fsst.cu:132-188); thread t owns rows
@@ -1430,7 +1441,7 @@ __device__ __forceinline__ void fsst_chunk(const FsstChunk ch, const FsstChunkIn
       sum += v;
     }
     uint32_t tot     = 0;
-    uint32_t run     = c_lo + block_excl_scan(sum, sm.wave, &tot);
+    uint32_t run     = c_lo + block_excl_scan<true>(sum, sm.wave, &tot);  // the spans still arrive
     if (tid == 0) sm.comp[0] = c_lo;
 #pragma unroll
     for (uint32_t i = 0; i < ROWS_PER_THR; ++i) {
