@@ -136,3 +136,74 @@ def test_threads_on_their_own_streams(work):
     assert not any(th.is_alive() for th in threads), "a decode thread did not finish"
     torch.cuda.synchronize()
     assert not errors, errors[:10]
+
+
+def test_batches_in_threads_share_descriptor_slots(work):
+    """Eight threads, each with its own sdr_batch on its own stream, launch in
+    turn a table of one 257-segment grid (descriptors through the shared
+    descriptor slots), a two-segment table (descriptors in the kernel
+    arguments) and a table with an FSST column (the batch's arena); every
+    output is checked."""
+    dev, fixed, strs, path, ranges = work
+    lib = L.load()
+    rows = 2048
+    big = E.encode_bitpacking(E.gen_bp_int32(0xC0C5, 257 * rows), rg_rows=rows)
+    assert len(big.segs) == 257
+    d_big = torch.from_numpy(big.blob).to(dev)
+    want_big = O.decode_encoded(big)
+    small, d_small, want_small = fixed[0]
+    senc, d_str, (o_offs, o_chars, _, _) = strs[0]
+    errors = []
+    barrier = threading.Barrier(THREADS)
+
+    def table(enc, d, out):
+        runs, keep = A.make_runs([(enc.codec, enc.segments(d.data_ptr()))])
+        col = A.Column()
+        col.type_size, col.total_rows, col.data, col.n_data = enc.type_size, enc.n_rows, runs, 1
+        col.d_out = out.data_ptr()
+        return (A.Column * 1)(col), keep
+
+    def worker(t):
+        b = C.c_void_p()
+        try:
+            L.check(lib.sdr_batch_create(C.byref(b)))
+            st = torch.cuda.Stream(device=dev)
+            sh = st.cuda_stream
+            with torch.cuda.stream(st):
+                barrier.wait()
+                for k in range(ROUNDS):
+                    form = (t + k) % 3  # 0: slot, 1: kernel arguments, 2: with strings (arena)
+                    enc, d, want = (small, d_small, want_small) if form == 1 else (big, d_big, want_big)
+                    out = torch.full((enc.n_rows * enc.type_size,), 0xCC, dtype=torch.uint8, device=dev)
+                    arr, keep = table(enc, d, out)
+                    L.check(lib.sdr_batch_clear(b))
+                    L.check(lib.sdr_batch_add_table(b, arr, 1))
+                    if form == 2:
+                        offs = torch.full((senc.n_rows + 1,), -1, dtype=torch.int32, device=dev)
+                        chars = torch.full((len(o_chars) + 64,), 0xCC, dtype=torch.uint8, device=dev)
+                        scol, skeep = string_column([(senc.codec, senc.segments(d_str.data_ptr()))], senc.n_rows,
+                                                    offs.data_ptr(), chars.data_ptr(), len(o_chars) + 64)
+                        L.check(lib.sdr_batch_add_strings(b, C.byref(scol)))
+                    L.check(lib.sdr_batch_launch(b, sh))
+                    L.check(lib.sdr_batch_finish(b, sh))
+                    st.synchronize()
+                    if not np.array_equal(out.cpu().numpy(), want):
+                        errors.append(f"thread {t} round {k}: form {form} fixed mismatch")
+                    if form == 2 and not (scol.chars_used == len(o_chars)
+                                          and np.array_equal(offs.cpu().numpy(), o_offs)
+                                          and np.array_equal(chars.cpu().numpy()[:len(o_chars)], o_chars)):
+                        errors.append(f"thread {t} round {k}: strings mismatch")
+        except Exception as ex:  # reported below with the thread that raised it
+            errors.append(f"thread {t}: {type(ex).__name__}: {ex}")
+        finally:
+            if b.value:
+                lib.sdr_batch_destroy(b)
+
+    threads = [threading.Thread(target=worker, args=(t,)) for t in range(THREADS)]
+    for th in threads:
+        th.start()
+    for th in threads:
+        th.join(timeout=100)
+    assert not any(th.is_alive() for th in threads), "a batch thread did not finish"
+    torch.cuda.synchronize()
+    assert not errors, errors[:10]
