@@ -5,7 +5,7 @@
 
 namespace sdr {
 
-__global__ __launch_bounds__(FSST_THREADS) void k_fsst(const FsstChunk* __restrict__ chunks, uint32_t n_chunks,
+__global__ __launch_bounds__(FSST_THREADS) __attribute__((amdgpu_waves_per_eu(6))) void k_fsst(const FsstChunk* __restrict__ chunks, uint32_t n_chunks,
                                                        const FsstChunkInfo* __restrict__ info,
                                                        const FsstSymTab* __restrict__ tabs, FsstState* __restrict__ st,
                                                        int32_t* __restrict__ offsets, uint8_t* __restrict__ chars,
