@@ -235,7 +235,9 @@ typedef enum sdr_batch_option {
   SDR_BATCH_USE_GRAPH          = 1,  /* 1: replay a recorded hipGraph; 0 (default) */
                                      /*   enqueue the kernels directly (faster on  */
                                      /*   gfx950: no graph-launch overhead)        */
-  SDR_BATCH_GRID               = 2,  /* blocks; 0 (default): resident capacity     */
+  SDR_BATCH_GRID               = 2,  /* blocks of each fixed-width group kernel;   */
+                                     /*   0 (default): resident capacity (the FSST */
+                                     /*   kernel runs one block per chunk)         */
   SDR_BATCH_FIXED_KERNELS      = 3,  /* fixed-width segments: 0 (default) the      */
                                      /*   per-codec kernels, one grid per codec    */
                                      /*   and value width over descriptors in the  */

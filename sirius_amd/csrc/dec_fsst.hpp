@@ -178,12 +178,12 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* s_wave
 constexpr uint32_t FSST_SYMSTAGE = 17u + 8u * FSST_NSYM;  // the largest symbol table (2057 bytes)
 constexpr uint32_t FSST_CSTAGE   = 16384;                  // chunk compressed bytes staged in LDS
 constexpr uint32_t FSST_NGRID    = (FSST_CSTAGE + 16u) / 16u + 2u;
-// Output windows.  The per-column kernel's 11 KiB window keeps its LDS at
-// 27.0 KB, and with at most 80 VGPRs it runs six workgroups per CU (1.6-3%
-// faster than five with 16 KiB); the batch kernel needs 96 VGPRs and stays at
-// five with 16 KiB (at six it spilled 13 VGPRs and ran 1.5% slower).
+// Output windows.  An 11 KiB window keeps a chunk kernel's LDS at 27.0 KB,
+// and with at most 80 VGPRs it runs six workgroups per CU (1.6-3% faster
+// than five with 16 KiB).  Both kernels decode one chunk per block: a
+// persistent loop around the chunk took the batch kernel to 96 VGPRs.
 constexpr uint32_t FSST_OUT_COLUMN = 11264;  // output window of the per-column kernel (6 blocks per CU)
-constexpr uint32_t FSST_OUT_BATCH  = 16384;  // ... of the persistent batch kernel
+constexpr uint32_t FSST_OUT_BATCH  = 11264;  // ... of the batch kernel (6 blocks per CU)
 // 16-byte groups of a thread's first-walk span of a chunk's compressed bytes
 // (at most FSST_CSTAGE of them after a head of up to 15 bytes), loaded into
 // registers at the chunk's start

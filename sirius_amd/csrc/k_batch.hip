@@ -7,14 +7,15 @@
 // (BatchGroup); each group is drained by its own persistent kernel,
 // specialised for that one decoder, so it keeps the register and LDS
 // footprint of the per-codec kernels instead of the union of all of them.
-// A group kernel's grid is its resident capacity.  Fixed-width entries are
-// independent, so block b takes entries b, b + grid, ... with no claim at
-// all; its four wavefronts decode an entry's units (an RLE slice is decoded
-// by the whole block).  FSST chunks are claimed in queue order with one
-// atomic on the group's counter, which is what lets a chunk wait on its
-// predecessors' offsets: a predecessor was claimed earlier by a running
-// block.  The host launches the groups' kernels as parallel branches of one
-// graph, so they share the GPU as their queues drain.
+// A fixed-width group kernel's grid is its resident capacity.  Fixed-width
+// entries are independent, so block b takes entries b, b + grid, ... with no
+// claim at all; its four wavefronts decode an entry's units (an RLE slice is
+// decoded by the whole block).  The FSST kernel runs one block per chunk,
+// claimed in queue order with one atomic on the group's counter, which is
+// what lets a chunk wait on its predecessors' offsets: a predecessor was
+// claimed earlier by a block that started earlier.  The group kernels run as
+// parallel branches (graph nodes) or in stream order, sharing the GPU as
+// their queues drain.
 #include "dec_fixed.hpp"
 #include "dec_fsst.hpp"
 #include "dec_rle.hpp"
@@ -28,23 +29,6 @@ namespace sdr {
 constexpr uint32_t BATCH_BP_STAGE = 4096;
 static_assert(BATCH_PREP_THREADS == FSST_THREADS, "pre-pass workgroup size");
 static_assert(BATCH_WAVES == WAVES_PER_BLOCK, "fixed-width group block");
-
-// Claim the next FSST entry: every thread gets its index in the queue, or
-// UINT32_MAX once the group is drained.  The group's `first` and `count`
-// are read once by the caller: reloading them after the barrier would put
-// two more dependent memory round trips in front of every chunk.  Claims
-// alternate between two LDS slots: slot k is written again two claims
-// later, after every thread has passed the next claim's barrier and so has
-// read it, so one barrier per claim suffices.
-__device__ __forceinline__ uint32_t claim(BatchQueue* q, uint32_t g, uint32_t first, uint32_t count,
-                                          uint32_t (&s_idx)[2], uint32_t& k)
-{
-  k ^= 1u;
-  if (threadIdx.x == 0) s_idx[k] = atomicAdd(&q->claimed[g].v, 1u);
-  __syncthreads();
-  const uint32_t i = s_idx[k];
-  return i < count ? first + i : UINT32_MAX;
-}
 
 template <typename T>
 __global__ __launch_bounds__(BLOCK_THREADS) void k_bgroup_bp(const BatchItem* __restrict__ items,
@@ -119,35 +103,39 @@ __global__ __launch_bounds__(BLOCK_THREADS) void k_bgroup_copy(const BatchItem* 
   }
 }
 
-__global__ __launch_bounds__(FSST_THREADS) __attribute__((amdgpu_waves_per_eu(5))) void k_bgroup_fsst(const BatchItem* __restrict__ items,
-                                                              BatchQueue* __restrict__ q, uint32_t g)
+// One block per FSST entry (the grid is the group's entry count).  The block
+// claims the next entry in queue order with one atomic, so a chunk's
+// predecessors in its column are held by blocks that started earlier, which
+// the look-back's progress needs (the per-column kernel's ticket does the
+// same).  One chunk per block, with no loop around it, keeps the kernel at
+// 80 VGPRs: six workgroups per CU with the 11 KiB window.
+__global__ __launch_bounds__(FSST_THREADS) __attribute__((amdgpu_waves_per_eu(6))) void k_bgroup_fsst(
+  const BatchItem* __restrict__ items, BatchQueue* __restrict__ q, uint32_t g)
 {
   __shared__ FsstSmemT<FSST_OUT_BATCH> sm;
-  __shared__ uint32_t s_idx[2];
-  uint32_t slot = 0;
-  const uint32_t spin = q->spin_limit;
+  __shared__ uint32_t s_idx;
   (void)items;  // the FSST group's entries are q->fsst_entries
-  const FsstEntry* entries    = q->fsst_entries;
-  const FsstColumn* cols      = q->fsst_cols;
+  // read before the claim, so they arrive while it does
+  const uint32_t first = q->first[g], count = q->count[g];
+  const uint32_t spin  = q->spin_limit;
+  if (threadIdx.x == 0) s_idx = atomicAdd(&q->claimed[g].v, 1u);
+  __syncthreads();
+  const uint32_t i = s_idx;
+  if (i >= count) return;  // the grid is the entry count: never taken
+  const uint32_t idx = first + i;
   // the entries list every column's chunks column by column, as the batch's
   // chunk and chunk-info arrays do: entry idx describes chunk idx of those
-  // arrays, so the chunk loads need not wait for the column's
-  const FsstChunk* chunks     = q->fsst_chunks;
-  const FsstChunkInfo* infos  = q->fsst_info;
-  const FsstSymTab* tabs      = q->fsst_tabs;
-  const uint32_t first = q->first[g], count = q->count[g];
-  for (uint32_t idx; (idx = claim(q, g, first, count, s_idx, slot)) != UINT32_MAX;) {
-    // every lane reads the same descriptors: kept in SGPRs
-    const FsstEntry e      = uniform(gload(entries + idx));
-    const FsstChunk ch     = uniform(gload(chunks + idx));
-    const FsstChunkInfo ci = uniform(gload(infos + idx));
-    // the column comes through the scalar cache (read-only for the whole
-    // launch), so its wait falls at the chunk's first barrier, behind the
-    // loads of the symbol table and the compressed bytes
-    const FsstColumn col = sload(cols + e.col);
-    fsst_chunk<FSST_OUT_BATCH>(ch, ci, tabs + (ch.seg == FSST_GAP ? 0u : ch.seg), e.chunk, col.n_chunks, col.state,
-                               col.offsets, col.chars, col.chars_cap, col.total_rows, spin, sm);
-  }
+  // arrays, so the chunk loads need not wait for the column's; every lane
+  // reads the same descriptors, kept in SGPRs
+  const FsstEntry e      = uniform(gload(q->fsst_entries + idx));
+  const FsstChunk ch     = uniform(gload(q->fsst_chunks + idx));
+  const FsstChunkInfo ci = uniform(gload(q->fsst_info + idx));
+  // the column comes through the scalar cache (read-only for the whole
+  // launch), so its wait falls behind the loads of the symbol table and
+  // the compressed bytes
+  const FsstColumn col = sload(q->fsst_cols + e.col);
+  fsst_chunk<FSST_OUT_BATCH>(ch, ci, q->fsst_tabs + (ch.seg == FSST_GAP ? 0u : ch.seg), e.chunk, col.n_chunks,
+                             col.state, col.offsets, col.chars, col.chars_cap, col.total_rows, spin, sm);
 }
 
 __global__ void k_batch_reset(BatchQueue* q)

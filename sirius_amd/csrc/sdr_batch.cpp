@@ -65,6 +65,7 @@ struct sdr_batch {
   std::vector<sdr::FixedLaunch> fixed;              // the per-codec grids of the last upload
   std::vector<sdr::FixedLaunch> graph_fixed;        // ... that the recorded graph launches
   size_t graph_fsegs   = 0;                         // ... and where their descriptors were
+  uint32_t graph_fsst  = 0;                         // ... and its FSST grid (one block per chunk)
   // where the per-codec grids of the last upload read their descriptors: the
   // arena, or (a batch of fixed-width segments alone, no graph) the kernel
   // arguments or a descriptor slot, as the batched table call passes them
@@ -564,12 +565,13 @@ int batch_launch(sdr_batch* b, hipStream_t s)
     if (!(mask >> g & 1u)) continue;
     grid[g] = std::max<uint32_t>(1, b->grid ? std::min(b->grid, b->group_cap[g]) : b->group_cap[g]);
   }
+  grid[BG_FSST] = cnt[BG_FSST];  // the FSST kernel decodes one chunk per block
   const bool strings = mask & (1u << BG_FSST);
   if (b->use_graph && !b->fixed.empty()) {
     // with per-codec grids the graph is recorded from the stream order (on
     // the batch's own stream), and recorded again when the grids change
     if (!b->gexec || b->graph_arena != b->d_arena || b->graph_mask != mask || b->graph_gridopt != b->grid ||
-        !(b->graph_fixed == b->fixed) || b->graph_fsegs != b->off_fsegs) {
+        !(b->graph_fixed == b->fixed) || b->graph_fsegs != b->off_fsegs || b->graph_fsst != cnt[BG_FSST]) {
       if (b->gexec) HIP_TRY(hipGraphExecDestroy(b->gexec));
       if (b->graph) HIP_TRY(hipGraphDestroy(b->graph));
       b->gexec = nullptr;
@@ -589,6 +591,7 @@ int batch_launch(sdr_batch* b, hipStream_t s)
       b->graph_gridopt = b->grid;
       b->graph_fixed   = b->fixed;
       b->graph_fsegs   = b->off_fsegs;
+      b->graph_fsst    = cnt[BG_FSST];
     }
     HIP_TRY(hipGraphLaunch(b->gexec, s));
   } else if (b->use_graph && mask) {
@@ -597,7 +600,7 @@ int batch_launch(sdr_batch* b, hipStream_t s)
     // addresses, entries change through the upload above, so one
     // instantiation replays every batch with the same groups
     if (!b->gexec || b->graph_arena != b->d_arena || b->graph_mask != mask || b->graph_gridopt != b->grid ||
-        !b->graph_fixed.empty()) {
+        !b->graph_fixed.empty() || b->graph_fsst != cnt[BG_FSST]) {
       if (b->gexec) HIP_TRY(hipGraphExecDestroy(b->gexec));
       if (b->graph) HIP_TRY(hipGraphDestroy(b->graph));
       b->gexec = nullptr;
@@ -642,6 +645,7 @@ int batch_launch(sdr_batch* b, hipStream_t s)
       b->graph_mask    = mask;
       b->graph_gridopt = b->grid;
       b->graph_fixed.clear();
+      b->graph_fsst    = cnt[BG_FSST];
     }
     HIP_TRY(hipGraphLaunch(b->gexec, s));
   } else {  // also a batch with no entries: its columns' look-back state is cleared
