@@ -216,9 +216,9 @@ int sdr_plan_destroy(sdr_plan* plan);
  * codec mix) into one launch: the fixed-width segments of each codec and
  * value width decode in one per-codec grid over descriptors in the batch's
  * arena (or, with SDR_BATCH_FIXED_KERNELS = 1, in persistent group kernels
- * that drain a work queue), and the VARCHAR chunks in a persistent kernel
- * that claims them from the queue.  The arena is uploaded from pinned memory
- * with one copy ahead of the kernels.  A batch of fixed-width segments alone
+ * that drain a work queue), and the VARCHAR chunks in a kernel of one
+ * block per chunk that claims them from the queue in order.  The arena is
+ * uploaded from pinned memory with one copy ahead of the kernels.  A batch of fixed-width segments alone
  * (no string column, validity mask or queue entry) passes its descriptors as
  * sdr_decode_table does instead: up to 256 per per-codec grid in the kernel
  * arguments, more through a reused descriptor buffer; it is never replayed
